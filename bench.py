@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Poisson solves/s at 768 x 384 x 384 (nx, nz, ny) on one MI355X.
+
+A step is one pressure Poisson solve (x FFT, z FFT, per-mode y Thomas,
+inverse z, inverse x) of the band-limited seed-1 RHS on the wall-refined
+channel grid, RHS resident in HBM, solution written to a second array.
+
+Prints one JSON line.  Extra keys: per-stage kernel times, p50/p99 solve
+latency, the relative discrete-operator residual of the result, the roofline
+of the dominant kernel and the CPU oracle baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dns-supercritical-ibm_amd"))
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "Poisson solves/s @768x384x384, mixed vs FP64 at 1e-10 resid; p99 latency"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-solves", type=int, default=3, help="timed oracle solves for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_fp64_768x384x384.json"),
+                    help="rocprofv3 FETCH/WRITE summary to report as roofline.traffic")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}), flush=True)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import dnsproj as D
+    from dnsproj import workloads as W
+
+    dev = torch.device("cuda:0")
+    c = W.config2()
+    nx, nz, ny = c["nx"], c["nz"], c["ny"]
+    N = nx * nz * ny
+    bands = D.mk_P_abc(c["yu"], False) + D.mk_P_abc(c["yu"], True)
+    solver = D.PoissonSolver(nx, nz, ny, c["inv_dx"], c["inv_dz"], bands)
+    f_host = c["rhs"]
+    f = torch.from_numpy(f_host).to(dev)
+    P = torch.empty_like(f)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        solver.solve(f, P)
+    torch.cuda.synchronize()
+
+    K = args.steps
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(K)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(K):
+        e = ev[s]
+        e[0].record(stream)
+        solver.stage_oop(f, P, D.STAGE_X_FWD)
+        e[1].record(stream)
+        for st in (D.STAGE_Z_FWD, D.STAGE_TRIDIAG, D.STAGE_Z_BWD, D.STAGE_X_BWD):
+            solver.stage(P, st)
+            e[st + 1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+
+    wall = t1 - t0
+    lat = np.array([ev[s][0].elapsed_time(ev[s][5]) for s in range(K)])
+    stage_ms = {D.STAGE_NAMES[i]: float(np.mean([ev[s][i].elapsed_time(ev[s][i + 1]) for s in range(K)]))
+                for i in range(5)}
+    solve_ms = float(np.mean(lat))
+
+    # correctness of what was timed: relative residual of the FP64 operator
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    p_host = P.cpu().numpy()
+    r = O.apply_operator(p_host, c["inv_dx"], c["inv_dz"], bands) - f_host
+    rel_res = float(np.abs(r).max() / np.abs(f_host).max())
+    del r
+
+    # roofline of the dominant kernel: algorithmic bytes = field read once + written once (16 B/pt)
+    dom = max(stage_ms, key=stage_ms.get)
+    alg_bytes = 16.0 * N
+    achieved = alg_bytes / (stage_ms[dom] * 1e-3) / 1e9
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        try:
+            tj = json.load(open(args.traffic))
+            traffic = tj.get("per_launch_bytes", {}).get(dom)
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                "alg_bytes_per_launch": alg_bytes}
+    solve_alg = 48.0 * N
+    solve_roof = {"alg_bytes": solve_alg, "achieved": round(solve_alg / (solve_ms * 1e-3) / 1e9, 1),
+                  "frac": round(solve_alg / (solve_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+
+    cpu = None
+    if args.cpu_solves > 0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        work = f_host.copy()
+        O.solve_inplace(work, c["inv_dx"], c["inv_dz"], bands)  # warm-up (page faults, plans)
+        ts = []
+        for _ in range(args.cpu_solves):
+            work[...] = f_host
+            a = time.perf_counter()
+            O.solve_inplace(work, c["inv_dx"], c["inv_dz"], bands)
+            ts.append(time.perf_counter() - a)
+        cpu = {"value": round(1.0 / float(np.median(ts)), 4), "unit": "solves/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{args.cpu_solves} full 768x384x384 FP64 solves of the same RHS (C oracle, "
+                         f"OpenMP {threads} threads), median"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(K / wall, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / K * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain FP64 pressure Poisson solve, nx=768 nz=384 ny=384, wall-refined y grid "
+                               "(uniform first 5%, x1.015/cell), band-limited RHS (kx=8,kz=16,sigma=4) seed 1",
+                   "form": "fp64"},
+        "p50_ms": round(float(np.percentile(lat, 50)), 4),
+        "p99_ms": round(float(np.percentile(lat, 99)), 4),
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "rel_residual": rel_res,
+        "roofline": roofline,
+        "solve_roofline": solve_roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

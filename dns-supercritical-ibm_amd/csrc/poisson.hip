@@ -1,0 +1,596 @@
+// FP64 FFT + wall-normal tridiagonal pressure Poisson solve for gfx950.
+//
+// Path (single GPU, one y-rank), all in place on the caller's device array:
+//   k_xfwd   x R2HC of every (j,k) line             poisson_solver_multigpu.f90:360-375
+//   k_zfwd   z R2HC of every (a,k) column            :377-393 (the transpose is the tile load)
+//   k_thomas per-(a,b)-mode Thomas along y           :395-422 with poi_C (:162-179) recomputed
+//   k_zbwd   z HC2R                                  :501-520
+//   k_xbwd   x HC2R and the 1/(nx*nz) scale          :523-551
+// The spectrum is the reference FFTW branch's real-real halfcomplex layout:
+// S(a,b,k) at a + nx*(b + nz*k), a the x halfcomplex index, b the z one.
+//
+// Real transforms are done two at a time through one complex FFT: two real
+// x-lines u, v form u + i v; along z, the real and imaginary halfcomplex
+// rows a and nx-a of one x wavenumber form the complex column X_a(z).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <vector>
+
+#include "common.h"
+#include "fft_lds.h"
+
+namespace dnsp {
+
+constexpr int kFftThreads = 256;
+// complex elements per workgroup in the FFT kernels (LDS = 16 B each)
+constexpr int kFftTileElems = 2048;
+
+// ---------------------------------------------------------------------------
+// x direction
+// ---------------------------------------------------------------------------
+
+// P (real lines of length nx, nlines of them) -> S halfcomplex, in place allowed.
+__global__ __launch_bounds__(kFftThreads) void k_xfwd(const double* P, double* S, int nx, long nlines,
+                                                      int G, FftPlan pl) {
+    extern __shared__ double2 sm[];
+    const long l0 = (long)blockIdx.x * 2 * G;
+    const int tot = 2 * G * nx;
+    double* smd = reinterpret_cast<double*>(sm);
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int r = e / nx;
+        const int t = e - r * nx;
+        const long line = l0 + r;
+        const double v = (line < nlines) ? P[line * nx + t] : 0.0;
+        smd[(size_t)((r >> 1) * nx + t) * 2 + (r & 1)] = v;
+    }
+    __syncthreads();
+    fft_lds<false>(sm, G, pl);
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int r = e / nx;
+        const int t = e - r * nx;
+        const long line = l0 + r;
+        if (line >= nlines) continue;
+        const bool lo = 2 * t <= nx;
+        const int u = lo ? t : nx - t;
+        const double2 C = sm[(r >> 1) * nx + u];
+        const double2 Cm = sm[(r >> 1) * nx + (u == 0 ? 0 : nx - u)];
+        double val;
+        if ((r & 1) == 0) val = lo ? 0.5 * (C.x + Cm.x) : 0.5 * (C.y - Cm.y);
+        else              val = lo ? 0.5 * (C.y + Cm.y) : 0.5 * (Cm.x - C.x);
+        S[line * nx + t] = val;
+    }
+}
+
+// S halfcomplex lines -> P real lines, times `scale`.  In place allowed.
+__global__ __launch_bounds__(kFftThreads) void k_xbwd(const double* S, double* P, int nx, long nlines,
+                                                      int G, FftPlan pl, double scale) {
+    extern __shared__ double2 sm[];
+    const long l0 = (long)blockIdx.x * 2 * G;
+    const int half = nx / 2 + 1;  // t in [0, nx/2]
+    const int tot = G * half;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int g = e / half;
+        const int t = e - g * half;
+        const long la = l0 + 2 * g, lb = la + 1;
+        const bool single = (t == 0) || (2 * t == nx);
+        const double xr = (la < nlines) ? S[la * nx + t] : 0.0;
+        const double xi = (!single && la < nlines) ? S[la * nx + nx - t] : 0.0;
+        const double yr = (lb < nlines) ? S[lb * nx + t] : 0.0;
+        const double yi = (!single && lb < nlines) ? S[lb * nx + nx - t] : 0.0;
+        // C_t = X_t + i Y_t ; C_{n-t} = conj(X_t) + i conj(Y_t)
+        sm[g * nx + t] = make_double2(xr - yi, xi + yr);
+        if (!single) sm[g * nx + nx - t] = make_double2(xr + yi, yr - xi);
+    }
+    __syncthreads();
+    fft_lds<true>(sm, G, pl);
+    const int tot2 = 2 * G * nx;
+    const double* smd = reinterpret_cast<const double*>(sm);
+    for (int e = threadIdx.x; e < tot2; e += blockDim.x) {
+        const int r = e / nx;
+        const int t = e - r * nx;
+        const long line = l0 + r;
+        if (line >= nlines) continue;
+        P[line * nx + t] = smd[(size_t)((r >> 1) * nx + t) * 2 + (r & 1)] * scale;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// z direction: complex column c pairs the halfcomplex x rows
+//   re_row(c) = c,  im_row(c) = nx - c (c >= 1), nx/2 (c == 0, nx even), none (c == 0, nx odd)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int zcol_row(int c, int half, int nx) {
+    if (half == 0) return c;
+    if (c == 0) return (nx % 2 == 0 && nx > 1) ? nx / 2 : -1;
+    return nx - c;
+}
+
+__global__ __launch_bounds__(kFftThreads) void k_zfwd(double* S, int nx, int nz, int ncol, int TA,
+                                                      FftPlan pl) {
+    extern __shared__ double2 sm[];
+    const int c0 = blockIdx.x * TA;
+    const size_t plane = (size_t)nx * nz;
+    double* Sk = S + (size_t)blockIdx.y * plane;
+    double* smd = reinterpret_cast<double*>(sm);
+    const int tot = nz * TA * 2;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int tt = e % TA;
+        const int rest = e / TA;
+        const int hf = rest & 1;
+        const int j = rest >> 1;
+        const int c = c0 + tt;
+        const int row = (c < ncol) ? zcol_row(c, hf, nx) : -1;
+        const double v = (row >= 0) ? Sk[(size_t)j * nx + row] : 0.0;
+        smd[(size_t)(tt * nz + j) * 2 + hf] = v;
+    }
+    __syncthreads();
+    fft_lds<false>(sm, TA, pl);
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int tt = e % TA;
+        const int rest = e / TA;
+        const int hf = rest & 1;
+        const int b = rest >> 1;
+        const int c = c0 + tt;
+        const int row = (c < ncol) ? zcol_row(c, hf, nx) : -1;
+        if (row < 0) continue;
+        const bool lo = 2 * b <= nz;
+        const int q = lo ? b : nz - b;
+        const double2 Z = sm[tt * nz + q];
+        const double2 Zm = sm[tt * nz + (q == 0 ? 0 : nz - q)];
+        double val;
+        if (hf == 0) val = lo ? 0.5 * (Z.x + Zm.x) : 0.5 * (Z.y - Zm.y);
+        else         val = lo ? 0.5 * (Z.y + Zm.y) : 0.5 * (Zm.x - Z.x);
+        Sk[(size_t)b * nx + row] = val;
+    }
+}
+
+__global__ __launch_bounds__(kFftThreads) void k_zbwd(double* S, int nx, int nz, int ncol, int TA,
+                                                      FftPlan pl) {
+    extern __shared__ double2 sm[];
+    const int c0 = blockIdx.x * TA;
+    const size_t plane = (size_t)nx * nz;
+    double* Sk = S + (size_t)blockIdx.y * plane;
+    const int half = nz / 2 + 1;
+    const int tot = half * TA;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int tt = e % TA;
+        const int q = e / TA;
+        const int c = c0 + tt;
+        const int ru = (c < ncol) ? zcol_row(c, 0, nx) : -1;
+        const int rv = (c < ncol) ? zcol_row(c, 1, nx) : -1;
+        const bool single = (q == 0) || (2 * q == nz);
+        const double ur = (ru >= 0) ? Sk[(size_t)q * nx + ru] : 0.0;
+        const double ui = (ru >= 0 && !single) ? Sk[(size_t)(nz - q) * nx + ru] : 0.0;
+        const double vr = (rv >= 0) ? Sk[(size_t)q * nx + rv] : 0.0;
+        const double vi = (rv >= 0 && !single) ? Sk[(size_t)(nz - q) * nx + rv] : 0.0;
+        sm[tt * nz + q] = make_double2(ur - vi, ui + vr);
+        if (!single) sm[tt * nz + nz - q] = make_double2(ur + vi, vr - ui);
+    }
+    __syncthreads();
+    fft_lds<true>(sm, TA, pl);
+    const double* smd = reinterpret_cast<const double*>(sm);
+    const int tot2 = nz * TA * 2;
+    for (int e = threadIdx.x; e < tot2; e += blockDim.x) {
+        const int tt = e % TA;
+        const int rest = e / TA;
+        const int hf = rest & 1;
+        const int j = rest >> 1;
+        const int c = c0 + tt;
+        const int row = (c < ncol) ? zcol_row(c, hf, nx) : -1;
+        if (row < 0) continue;
+        Sk[(size_t)j * nx + row] = smd[(size_t)(tt * nz + j) * 2 + hf];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y direction: per-mode Thomas.  bands = [a | b | c | a00 | b00 | c00], ny each.
+// Same operation order as the reference (no contraction), so this stage
+// reproduces a serial FP64 Thomas bit for bit.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_thomas(double* S, double* cpw, const double* __restrict__ ax,
+                                                const double* __restrict__ az,
+                                                const int* __restrict__ is0x,
+                                                const int* __restrict__ is0z,
+                                                const double* __restrict__ bands, int nx, int nz, int ny) {
+#pragma clang fp contract(off)
+    const long plane = (long)nx * nz;
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= plane) return;
+    const int i = (int)(m % nx);
+    const int j = (int)(m / nx);
+    const double a_xz = ax[i] + az[j];
+    const bool is00 = is0x[i] && is0z[j];
+    const double* A = bands + (is00 ? 3 * ny : 0);
+    const double* B = A + ny;
+    const double* C = B + ny;
+    double* d = S + m;
+    double* cp = cpw + m;
+    double den = B[0] + a_xz;
+    double c_prev = C[0] / den;
+    double d_prev = d[0] / den;
+    d[0] = d_prev;
+    cp[0] = c_prev;
+    for (int k = 1; k < ny; ++k) {
+        const size_t o = (size_t)k * plane;
+        den = B[k] + a_xz - A[k] * c_prev;
+        d_prev = (d[o] - A[k] * d_prev) / den;
+        c_prev = C[k] / den;
+        d[o] = d_prev;
+        cp[o] = c_prev;
+    }
+    double x = d_prev;
+    for (int k = ny - 2; k >= 0; --k) {
+        const size_t o = (size_t)k * plane;
+        x = d[o] - x * cp[o];
+        d[o] = x;
+    }
+}
+
+// min over modes and rows of |pivot| (init diagnostic); one value per block.
+__global__ __launch_bounds__(256) void k_min_pivot(double* out, const double* __restrict__ ax,
+                                                   const double* __restrict__ az,
+                                                   const int* __restrict__ is0x,
+                                                   const int* __restrict__ is0z,
+                                                   const double* __restrict__ bands, int nx, int nz,
+                                                   int ny) {
+#pragma clang fp contract(off)
+    __shared__ double red[256];
+    const long plane = (long)nx * nz;
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double mn = 1e20;
+    if (m < plane) {
+        const int i = (int)(m % nx);
+        const int j = (int)(m / nx);
+        const double a_xz = ax[i] + az[j];
+        const bool is00 = is0x[i] && is0z[j];
+        const double* A = bands + (is00 ? 3 * ny : 0);
+        const double* B = A + ny;
+        const double* C = B + ny;
+        double den = B[0] + a_xz;
+        mn = fabs(den);
+        double cp = C[0] / den;
+        for (int k = 1; k < ny; ++k) {
+            den = B[k] + a_xz - A[k] * cp;
+            mn = fmin(mn, fabs(den));
+            cp = C[k] / den;
+        }
+        if (!(mn == mn)) mn = -1.0;  // NaN pivot: report as invalid
+    }
+    red[threadIdx.x] = mn;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+static bool fft_factorize(int n, FftPlan& pl) {
+    pl.n = n;
+    pl.nf = 0;
+    int r = n;
+    auto push = [&](int f) {
+        if (pl.nf >= kFftMaxFactors) return false;
+        pl.fac[pl.nf++] = f;
+        r /= f;
+        return true;
+    };
+    while (r % 8 == 0 && r != 16) if (!push(8)) return false;
+    while (r % 4 == 0) if (!push(4)) return false;
+    while (r % 2 == 0) if (!push(2)) return false;
+    for (int p : {3, 5, 7})
+        while (r % p == 0) if (!push(p)) return false;
+    return r == 1;
+}
+
+static int fft_radix_limit(const FftPlan& pl) {
+    // largest G*n allowed by the register budget of each stage
+    int lim = 1 << 30;
+    for (int s = 0; s < pl.nf; ++s) {
+        int R = pl.fac[s];
+        int nb = (kFftRegBudget / R) > 0 ? (kFftRegBudget / R) : 1;
+        lim = std::min(lim, nb * R * kFftThreads);
+    }
+    return lim;
+}
+
+static int make_twiddles(int n, double2** dev) {
+    std::vector<double2> tw((size_t)n);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int t = 0; t < n; ++t) {
+        long double ang = 2.0L * pi * (long double)t / (long double)n;
+        tw[(size_t)t] = make_double2((double)cosl(ang), (double)(-sinl(ang)));
+    }
+    DNSP_HIP_TRY(hipMalloc(dev, sizeof(double2) * (size_t)n));
+    DNSP_HIP_TRY(hipMemcpy(*dev, tw.data(), sizeof(double2) * (size_t)n, hipMemcpyHostToDevice));
+    return DNSP_OK;
+}
+
+// poisson_solver_multigpu.f90:139-152 (FFTW halfcomplex index -> wavenumber)
+static int hc_wavenumber(int i, int n) {
+    if (i <= n / 2) return i;
+    int ii = (n + 1) / 2 - 1 - (i - (n / 2 + 1));
+    return ii > 1 ? ii : 1;
+}
+
+static void eigenvalues(int n, double inv_d, std::vector<double>& lam, std::vector<int>& is00) {
+    const double const_pi = 4.0 * std::atan(1.0);
+    lam.resize((size_t)n);
+    is00.resize((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        int ii = hc_wavenumber(i, n);
+        double s = std::sin(ii * const_pi / n);
+        lam[(size_t)i] = -4 * (inv_d * inv_d) * (s * s);
+        is00[(size_t)i] = (ii == 0) ? 1 : 0;
+    }
+}
+
+}  // namespace dnsp
+
+using namespace dnsp;
+
+struct dnsp_poisson {
+    int nx = 0, nz = 0, ny = 0;
+    double inv_dx = 0, inv_dz = 0;
+    int precision = 0;
+    int device = 0;
+    FftPlan px{}, pz{};
+    int Gx = 1, TAz = 1, ncol = 0;
+    double* d_ax = nullptr;
+    double* d_az = nullptr;
+    int* d_is0x = nullptr;
+    int* d_is0z = nullptr;
+    double* d_bands = nullptr;
+    double2* d_twx = nullptr;
+    double2* d_twz = nullptr;
+    double* d_cp = nullptr;    // Thomas c' workspace, nx*nz*ny
+    double* d_host = nullptr;  // staging for dnsp_poisson_run_host
+    double min_pivot = 0;
+};
+
+static void poisson_release(dnsp_poisson* h) {
+    if (!h) return;
+    int cur = 0;
+    if (hipGetDevice(&cur) == hipSuccess && cur != h->device) (void)hipSetDevice(h->device);
+    (void)hipFree(h->d_ax);
+    (void)hipFree(h->d_az);
+    (void)hipFree(h->d_is0x);
+    (void)hipFree(h->d_is0z);
+    (void)hipFree(h->d_bands);
+    (void)hipFree(h->d_twx);
+    (void)hipFree(h->d_twz);
+    (void)hipFree(h->d_cp);
+    (void)hipFree(h->d_host);
+    if (cur != h->device) (void)hipSetDevice(cur);
+    delete h;
+}
+
+static int poisson_setup(dnsp_poisson* h, const double* const bands[6]) {
+    DNSP_HIP_TRY(hipGetDevice(&h->device));
+    const int nx = h->nx, nz = h->nz, ny = h->ny;
+    if (!fft_factorize(nx, h->px) || !fft_factorize(nz, h->pz)) {
+        set_error("nx=%d / nz=%d: FFT lengths must factor into 2, 3, 5 and 7", nx, nz);
+        return DNSP_ERR_UNSUPPORTED;
+    }
+    const int limx = fft_radix_limit(h->px), limz = fft_radix_limit(h->pz);
+    if (nx > limx || nz > limz || nx > kFftTileElems * 2 || nz > kFftTileElems * 2) {
+        set_error("nx=%d / nz=%d: FFT length too large for the LDS kernels", nx, nz);
+        return DNSP_ERR_UNSUPPORTED;
+    }
+    h->Gx = std::max(1, std::min(kFftTileElems / nx, limx / nx));
+    h->TAz = std::max(1, std::min(kFftTileElems / nz, limz / nz));
+    h->ncol = (nx + 1) / 2;
+
+    std::vector<double> ax, az;
+    std::vector<int> i0x, i0z;
+    eigenvalues(nx, h->inv_dx, ax, i0x);
+    eigenvalues(nz, h->inv_dz, az, i0z);
+    DNSP_HIP_TRY(hipMalloc(&h->d_ax, sizeof(double) * nx));
+    DNSP_HIP_TRY(hipMalloc(&h->d_az, sizeof(double) * nz));
+    DNSP_HIP_TRY(hipMalloc(&h->d_is0x, sizeof(int) * nx));
+    DNSP_HIP_TRY(hipMalloc(&h->d_is0z, sizeof(int) * nz));
+    DNSP_HIP_TRY(hipMemcpy(h->d_ax, ax.data(), sizeof(double) * nx, hipMemcpyHostToDevice));
+    DNSP_HIP_TRY(hipMemcpy(h->d_az, az.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+    DNSP_HIP_TRY(hipMemcpy(h->d_is0x, i0x.data(), sizeof(int) * nx, hipMemcpyHostToDevice));
+    DNSP_HIP_TRY(hipMemcpy(h->d_is0z, i0z.data(), sizeof(int) * nz, hipMemcpyHostToDevice));
+
+    std::vector<double> hb((size_t)6 * ny);
+    for (int s = 0; s < 6; ++s) std::memcpy(hb.data() + (size_t)s * ny, bands[s], sizeof(double) * ny);
+    DNSP_HIP_TRY(hipMalloc(&h->d_bands, sizeof(double) * 6 * ny));
+    DNSP_HIP_TRY(hipMemcpy(h->d_bands, hb.data(), sizeof(double) * 6 * ny, hipMemcpyHostToDevice));
+
+    int rc = make_twiddles(nx, &h->d_twx);
+    if (rc) return rc;
+    rc = make_twiddles(nz, &h->d_twz);
+    if (rc) return rc;
+    h->px.tw = h->d_twx;
+    h->pz.tw = h->d_twz;
+
+    const size_t N = (size_t)nx * nz * ny;
+    DNSP_HIP_TRY(hipMalloc(&h->d_cp, sizeof(double) * N));
+
+    // min |pivot| diagnostic (init_spec_poisson_multigpu :181-222)
+    const long plane = (long)nx * nz;
+    const int nblk = (int)((plane + 255) / 256);
+    double* d_red = nullptr;
+    DNSP_HIP_TRY(hipMalloc(&d_red, sizeof(double) * nblk));
+    k_min_pivot<<<nblk, 256>>>(d_red, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands, nx, nz, ny);
+    hipError_t le = hipGetLastError();
+    std::vector<double> red((size_t)nblk);
+    hipError_t ce = hipMemcpy(red.data(), d_red, sizeof(double) * nblk, hipMemcpyDeviceToHost);
+    (void)hipFree(d_red);
+    DNSP_HIP_TRY(le);
+    DNSP_HIP_TRY(ce);
+    double mn = 1e20;
+    bool bad = false;
+    for (double v : red) {
+        if (v < 0) bad = true;
+        mn = std::min(mn, v);
+    }
+    h->min_pivot = mn;
+    if (bad || !(mn > 0) || !std::isfinite(mn)) {
+        set_error("tridiagonal pivot is zero or not finite (min |pivot| = %g)", mn);
+        return DNSP_ERR_SINGULAR;
+    }
+    return DNSP_OK;
+}
+
+extern "C" {
+
+int dnsp_poisson_init(int nx_glob, int nz_glob, int ny, double inv_dx, double inv_dz,
+                      const double* P_band_a, const double* P_band_b, const double* P_band_c,
+                      const double* P_band_a_00, const double* P_band_b_00, const double* P_band_c_00,
+                      int precision, dnsp_poisson** out) {
+    clear_error();
+    DNSP_ARG_CHECK(out != nullptr, "out is NULL");
+    *out = nullptr;
+    DNSP_ARG_CHECK(nx_glob >= 1 && nz_glob >= 1, "nx_glob=%d, nz_glob=%d must be >= 1", nx_glob, nz_glob);
+    DNSP_ARG_CHECK(ny >= 2, "ny_glob <2 (ny=%d)", ny);  // alloc_poisson_solver_multigpu.f90:34
+    DNSP_ARG_CHECK((long)nx_glob * nz_glob * ny < (1L << 40), "field too large");
+    DNSP_ARG_CHECK(std::isfinite(inv_dx) && std::isfinite(inv_dz), "inv_dx / inv_dz not finite");
+    DNSP_ARG_CHECK(precision == 0, "precision=%d: this entry point is the plain FP64 form (0)", precision);
+    const double* bands[6] = {P_band_a, P_band_b, P_band_c, P_band_a_00, P_band_b_00, P_band_c_00};
+    for (int s = 0; s < 6; ++s) {
+        DNSP_ARG_CHECK(bands[s] != nullptr, "band array %d is NULL", s);
+        for (int k = 0; k < ny; ++k)
+            DNSP_ARG_CHECK(std::isfinite(bands[s][k]), "band array %d row %d is not finite", s, k);
+    }
+    dnsp_poisson* h = new (std::nothrow) dnsp_poisson();
+    if (!h) {
+        set_error("host allocation failed");
+        return DNSP_ERR_NOMEM;
+    }
+    h->nx = nx_glob;
+    h->nz = nz_glob;
+    h->ny = ny;
+    h->inv_dx = inv_dx;
+    h->inv_dz = inv_dz;
+    h->precision = precision;
+    int rc = poisson_setup(h, bands);
+    if (rc) {
+        poisson_release(h);
+        return rc;
+    }
+    *out = h;
+    return DNSP_OK;
+}
+
+int dnsp_poisson_stage(dnsp_poisson* h, const double* src, double* P, int stage, void* stream) {
+    clear_error();
+    DNSP_ARG_CHECK(h && src && P, "NULL handle or array");
+    DNSP_ARG_CHECK(src == P || stage == DNSP_STAGE_X_FWD || stage == DNSP_STAGE_X_BWD,
+                   "stage %d runs in place only (src must equal dst)", stage);
+    hipStream_t st = as_stream(stream);
+    const long nlines = (long)h->nz * h->ny;
+    const int gx = (int)((nlines + 2 * h->Gx - 1) / (2 * h->Gx));
+    const dim3 gz((h->ncol + h->TAz - 1) / h->TAz, h->ny);
+    const size_t ldsx = sizeof(double2) * h->Gx * h->nx;
+    const size_t ldsz = sizeof(double2) * h->TAz * h->nz;
+    switch (stage) {
+        case DNSP_STAGE_X_FWD:
+            k_xfwd<<<gx, kFftThreads, ldsx, st>>>(src, P, h->nx, nlines, h->Gx, h->px);
+            break;
+        case DNSP_STAGE_Z_FWD:
+            k_zfwd<<<gz, kFftThreads, ldsz, st>>>(P, h->nx, h->nz, h->ncol, h->TAz, h->pz);
+            break;
+        case DNSP_STAGE_TRIDIAG: {
+            const long plane = (long)h->nx * h->nz;
+            const int nb = (int)((plane + 255) / 256);
+            k_thomas<<<nb, 256, 0, st>>>(P, h->d_cp, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands,
+                                         h->nx, h->nz, h->ny);
+            break;
+        }
+        case DNSP_STAGE_Z_BWD:
+            k_zbwd<<<gz, kFftThreads, ldsz, st>>>(P, h->nx, h->nz, h->ncol, h->TAz, h->pz);
+            break;
+        case DNSP_STAGE_X_BWD: {
+            const double inv_nxnz = 1.0 / ((double)h->nx * (double)h->nz + 0.0);
+            k_xbwd<<<gx, kFftThreads, ldsx, st>>>(src, P, h->nx, nlines, h->Gx, h->px, inv_nxnz);
+            break;
+        }
+        default:
+            set_error("unknown stage %d", stage);
+            return DNSP_ERR_ARG;
+    }
+    DNSP_LAUNCH_CHECK();
+    return DNSP_OK;
+}
+
+int dnsp_poisson_forward(dnsp_poisson* h, double* P, void* stream) {
+    int rc = dnsp_poisson_stage(h, P, P, DNSP_STAGE_X_FWD, stream);
+    return rc ? rc : dnsp_poisson_stage(h, P, P, DNSP_STAGE_Z_FWD, stream);
+}
+
+int dnsp_poisson_tridiag(dnsp_poisson* h, double* S, void* stream) {
+    return dnsp_poisson_stage(h, S, S, DNSP_STAGE_TRIDIAG, stream);
+}
+
+int dnsp_poisson_backward(dnsp_poisson* h, double* S, void* stream) {
+    int rc = dnsp_poisson_stage(h, S, S, DNSP_STAGE_Z_BWD, stream);
+    return rc ? rc : dnsp_poisson_stage(h, S, S, DNSP_STAGE_X_BWD, stream);
+}
+
+int dnsp_poisson_run(dnsp_poisson* h, double* P, void* stream) {
+    int rc = dnsp_poisson_forward(h, P, stream);
+    if (rc) return rc;
+    rc = dnsp_poisson_tridiag(h, P, stream);
+    if (rc) return rc;
+    return dnsp_poisson_backward(h, P, stream);
+}
+
+int dnsp_poisson_solve(dnsp_poisson* h, const double* rhs, double* P, void* stream) {
+    int rc = dnsp_poisson_stage(h, rhs, P, DNSP_STAGE_X_FWD, stream);
+    if (rc) return rc;
+    rc = dnsp_poisson_stage(h, P, P, DNSP_STAGE_Z_FWD, stream);
+    if (rc) return rc;
+    rc = dnsp_poisson_tridiag(h, P, stream);
+    if (rc) return rc;
+    return dnsp_poisson_backward(h, P, stream);
+}
+
+int dnsp_poisson_run_host(dnsp_poisson* h, double* P_host) {
+    clear_error();
+    DNSP_ARG_CHECK(h && P_host, "NULL handle or array");
+    const size_t bytes = sizeof(double) * (size_t)h->nx * h->nz * h->ny;
+    if (!h->d_host) DNSP_HIP_TRY(hipMalloc(&h->d_host, bytes));
+    DNSP_HIP_TRY(hipMemcpy(h->d_host, P_host, bytes, hipMemcpyHostToDevice));
+    int rc = dnsp_poisson_run(h, h->d_host, nullptr);
+    if (rc) return rc;
+    DNSP_HIP_TRY(hipMemcpy(P_host, h->d_host, bytes, hipMemcpyDeviceToHost));
+    DNSP_HIP_TRY(hipDeviceSynchronize());
+    return DNSP_OK;
+}
+
+int dnsp_poisson_min_pivot(const dnsp_poisson* h, double* out) {
+    clear_error();
+    DNSP_ARG_CHECK(h && out, "NULL handle or pointer");
+    *out = h->min_pivot;
+    return DNSP_OK;
+}
+
+int dnsp_poisson_shape(const dnsp_poisson* h, int* nx, int* nz, int* ny) {
+    clear_error();
+    DNSP_ARG_CHECK(h && nx && nz && ny, "NULL handle or pointer");
+    *nx = h->nx;
+    *nz = h->nz;
+    *ny = h->ny;
+    return DNSP_OK;
+}
+
+int dnsp_poisson_free(dnsp_poisson* h) {
+    clear_error();
+    poisson_release(h);
+    return DNSP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,242 @@
+"""Host-side mirror of the reference's pressure-Poisson interface over libdnsproj.so.
+
+The reference's Fortran module ``poisson_solver_multigpu`` exposes
+``init_spec_poisson_multigpu`` / ``run_spec_poisson_multigpu``
+(src/poisson_solver_mgpu_dtdma/poisson_solver_multigpu.f90:33-40, :328-335) and
+``get_lo_n_bounds_1d`` (:556-575); its band generator is ``mk_P_abc``
+(src/pycode/calc_bands.py:181-201).  This package exposes the same names with
+the same argument meaning, calling the HIP library through its C ABI
+(include/dnsproj.h).  Device arrays are torch tensors (float64, contiguous,
+on a HIP device); torch is only the allocator and stream provider here.
+
+There is no CPU fallback: when the library is missing every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_ROOT, "lib", "libdnsproj.so")
+
+DNSP_OK = 0
+DNSP_ERR_ARG = 1
+DNSP_ERR_HIP = 2
+DNSP_ERR_NOMEM = 3
+DNSP_ERR_SINGULAR = 4
+DNSP_ERR_UNSUPPORTED = 5
+
+STAGE_X_FWD, STAGE_Z_FWD, STAGE_TRIDIAG, STAGE_Z_BWD, STAGE_X_BWD = range(5)
+STAGE_NAMES = ("x_fwd", "z_fwd", "tridiag", "z_bwd", "x_bwd")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+_lib = None
+
+
+class DnspError(RuntimeError):
+    """A nonzero status from the C ABI; ``code`` is the DNSP_ERR_* value."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"dnsproj error {code}: {msg}")
+        self.code = code
+
+
+# (name, argtypes, restype) of every exported entry point
+_SIGNATURES = [
+    ("dnsp_last_error", [], ctypes.c_char_p),
+    ("dnsp_version", [], ctypes.c_char_p),
+    ("dnsp_poisson_init", [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                           _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    ("dnsp_poisson_run", [_vp, _vp, _vp], ctypes.c_int),
+    ("dnsp_poisson_solve", [_vp, _vp, _vp, _vp], ctypes.c_int),
+    ("dnsp_poisson_run_host", [_vp, _dp], ctypes.c_int),
+    ("dnsp_poisson_forward", [_vp, _vp, _vp], ctypes.c_int),
+    ("dnsp_poisson_tridiag", [_vp, _vp, _vp], ctypes.c_int),
+    ("dnsp_poisson_backward", [_vp, _vp, _vp], ctypes.c_int),
+    ("dnsp_poisson_stage", [_vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+    ("dnsp_poisson_min_pivot", [_vp, _dp], ctypes.c_int),
+    ("dnsp_poisson_shape", [_vp, _ip, _ip, _ip], ctypes.c_int),
+    ("dnsp_poisson_free", [_vp], ctypes.c_int),
+    ("dnsp_get_lo_n_bounds_1d", [ctypes.c_int, ctypes.c_int, _ip, _ip], ctypes.c_int),
+    ("dnsp_mk_P_abc", [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
+]
+
+
+def lib():
+    """Loads libdnsproj.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C {_PKG_ROOT}` "
+                               "or __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args, res in _SIGNATURES:
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_names():
+    return [s[0] for s in _SIGNATURES]
+
+
+def _check(rc):
+    if rc != DNSP_OK:
+        raise DnspError(rc, lib().dnsp_last_error().decode(errors="replace"))
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def mk_P_abc(yu, is_dirich_first=False):
+    """Poisson y bands for the faces ``yu`` (length ny+1): returns (a, b, c).
+
+    calc_bands.py:181-201 takes ``yp_ext`` = [yu0, cell centres, yu_ny]
+    (:226-227); this takes the faces and forms yp_ext itself.
+    """
+    yu, pyu = _f64(yu)
+    ny = yu.size - 1
+    a, b, c = (np.empty(ny) for _ in range(3))
+    _check(lib().dnsp_mk_P_abc(ny, pyu, int(bool(is_dirich_first)), a.ctypes.data_as(_dp),
+                               b.ctypes.data_as(_dp), c.ctypes.data_as(_dp)))
+    return a, b, c
+
+
+def get_lo_n_bounds_1d(mpi_divs, n_glob):
+    lo = np.zeros(mpi_divs, dtype=np.int32)
+    n = np.zeros(mpi_divs, dtype=np.int32)
+    _check(lib().dnsp_get_lo_n_bounds_1d(int(mpi_divs), int(n_glob), lo.ctypes.data_as(_ip),
+                                         n.ctypes.data_as(_ip)))
+    return lo, n
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def _device_ptr(P, numel):
+    import torch
+    if not isinstance(P, torch.Tensor):
+        raise TypeError("P must be a torch tensor on a HIP device")
+    if P.dtype != torch.float64 or not P.is_cuda or not P.is_contiguous():
+        raise ValueError("P must be a contiguous float64 tensor on a HIP device")
+    if P.numel() != numel:
+        raise ValueError(f"P has {P.numel()} elements, the solver expects {numel}")
+    return P.data_ptr()
+
+
+class PoissonSolver:
+    """One single-rank FP64 Poisson solver instance (the C handle).
+
+    ``bands`` = (P_band_a, P_band_b, P_band_c, P_band_a_00, P_band_b_00, P_band_c_00).
+    Arrays are (ny, nz, nx) C-order tensors, i.e. the reference's x-fastest P.
+    """
+
+    def __init__(self, nx, nz, ny, inv_dx, inv_dz, bands, precision=0):
+        L = lib()
+        bs = [_f64(b) for b in bands]
+        if len(bs) != 6 or any(b[0].size != ny for b in bs):
+            raise ValueError("six band arrays of length ny are required")
+        h = _vp()
+        _check(L.dnsp_poisson_init(int(nx), int(nz), int(ny), float(inv_dx), float(inv_dz),
+                                   *[b[1] for b in bs], int(precision), ctypes.byref(h)))
+        self._h = h
+        self._bands = bs  # keep alive
+        self.nx, self.nz, self.ny = int(nx), int(nz), int(ny)
+        self.inv_dx, self.inv_dz = float(inv_dx), float(inv_dz)
+
+    @property
+    def shape(self):
+        return (self.ny, self.nz, self.nx)
+
+    @property
+    def numel(self):
+        return self.nx * self.nz * self.ny
+
+    @property
+    def min_pivot(self):
+        out = ctypes.c_double()
+        _check(lib().dnsp_poisson_min_pivot(self._h, ctypes.byref(out)))
+        return out.value
+
+    def _call(self, fn, P, stream):
+        if self._h is None:
+            raise RuntimeError("solver has been freed")
+        _check(fn(self._h, _vp(_device_ptr(P, self.numel)), _vp(_stream_ptr(stream))))
+        return P
+
+    def run(self, P, stream=None):
+        """In-place solve of the device array P (RHS in, solution out)."""
+        return self._call(lib().dnsp_poisson_run, P, stream)
+
+    def solve(self, rhs, P, stream=None):
+        """Out-of-place solve: rhs (device, unchanged) -> P (device)."""
+        if self._h is None:
+            raise RuntimeError("solver has been freed")
+        _check(lib().dnsp_poisson_solve(self._h, _vp(_device_ptr(rhs, self.numel)),
+                                        _vp(_device_ptr(P, self.numel)), _vp(_stream_ptr(stream))))
+        return P
+
+    def forward(self, P, stream=None):
+        return self._call(lib().dnsp_poisson_forward, P, stream)
+
+    def tridiag(self, S, stream=None):
+        return self._call(lib().dnsp_poisson_tridiag, S, stream)
+
+    def backward(self, S, stream=None):
+        return self._call(lib().dnsp_poisson_backward, S, stream)
+
+    def stage(self, P, stage, stream=None):
+        """Launches one kernel of the solve in place (STAGE_* constants)."""
+        return self.stage_oop(P, P, stage, stream)
+
+    def stage_oop(self, src, P, stage, stream=None):
+        """One kernel reading src and writing P (src != P only for the x stages)."""
+        if self._h is None:
+            raise RuntimeError("solver has been freed")
+        _check(lib().dnsp_poisson_stage(self._h, _vp(_device_ptr(src, self.numel)),
+                                        _vp(_device_ptr(P, self.numel)), int(stage), _vp(_stream_ptr(stream))))
+        return P
+
+    def run_host(self, P):
+        """Solve a host numpy array in place (copies to and from the device)."""
+        P = np.asarray(P)
+        if P.dtype != np.float64 or not P.flags["C_CONTIGUOUS"] or P.size != self.numel:
+            raise ValueError("P must be a C-contiguous float64 array of nx*nz*ny elements")
+        _check(lib().dnsp_poisson_run_host(self._h, P.ctypes.data_as(_dp)))
+        return P
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().dnsp_poisson_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def init_spec_poisson_multigpu(P_band_a, P_band_b, P_band_c, P_band_a_00, P_band_b_00, P_band_c_00,
+                               nx_glob, nz_glob, ny_loc, inv_dx, inv_dz):
+    """Single-rank init (poisson_solver_multigpu.f90:33-326): returns the solver handle."""
+    return PoissonSolver(nx_glob, nz_glob, ny_loc, inv_dx, inv_dz,
+                         (P_band_a, P_band_b, P_band_c, P_band_a_00, P_band_b_00, P_band_c_00))
+
+
+def run_spec_poisson_multigpu(solver, P, stream=None):
+    """Single-rank solve (poisson_solver_multigpu.f90:328-554): P (device) RHS in, solution out."""
+    return solver.run(P, stream)

@@ -1,0 +1,119 @@
+/*
+ * dnsproj.h -- C ABI of the MI355X (gfx950) pressure-projection library.
+ *
+ * Plain C types only.  Device arrays are hipMalloc'd pointers passed as
+ * double*; streams are hipStream_t passed as void* (NULL = the null stream).
+ * Every entry point returns 0 on success or a DNSP_ERR_* code, and leaves a
+ * message for dnsp_last_error() (thread-local).  The reference aborts with
+ * `error stop` in these situations; its Fortran shim does the same on a
+ * nonzero status.
+ *
+ * Array layout is the reference's: Fortran column-major, x fastest,
+ *   P(0:nx-1, 0:nz-1, 0:ny-1)  ->  P[i + nx*(j + nz*k)].
+ */
+#ifndef DNSPROJ_H
+#define DNSPROJ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DNSP_OK               0
+#define DNSP_ERR_ARG          1  /* invalid size, NULL pointer, non-finite input */
+#define DNSP_ERR_HIP          2  /* a HIP runtime call failed */
+#define DNSP_ERR_NOMEM        3  /* device allocation failed */
+#define DNSP_ERR_SINGULAR     4  /* a zero or non-finite tridiagonal pivot */
+#define DNSP_ERR_UNSUPPORTED  5  /* FFT length with a prime factor the kernels do not handle */
+
+typedef struct dnsp_poisson dnsp_poisson;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* dnsp_last_error(void);
+
+/* Library version string. */
+const char* dnsp_version(void);
+
+/*
+ * Replaces init_spec_poisson_multigpu
+ * (src/poisson_solver_mgpu_dtdma/poisson_solver_multigpu.f90:33-326), single
+ * y-rank branch (mpi_divs_y == 1, :162-222), and the size checks of
+ * alloc_poisson_solver_multigpu.f90:3-38.
+ *   nx_glob, nz_glob : periodic sizes (any size >= 1 whose prime factors are in {2,3,5,7})
+ *   ny               : wall-normal size (>= 2)
+ *   inv_dx, inv_dz   : 1/dx, 1/dz (the reference's inv_dx, inv_dz)
+ *   P_band_*         : y bands (length ny) as read from geom_data/P_band_abc.dat,
+ *   P_band_*_00      : and the (0,0)-mode set from P_band_abc_00.dat
+ *   precision        : 0 = plain FP64 (the only form of this entry point)
+ * The eigenvalues a_x, a_z (:132-155) and the FFT twiddles are computed on the
+ * host; the reference's poi_C array (:162-179) is not stored.
+ * On success *out owns device memory on the current HIP device.
+ */
+int dnsp_poisson_init(int nx_glob, int nz_glob, int ny, double inv_dx, double inv_dz,
+                      const double* P_band_a, const double* P_band_b, const double* P_band_c,
+                      const double* P_band_a_00, const double* P_band_b_00, const double* P_band_c_00,
+                      int precision, dnsp_poisson** out);
+
+/*
+ * Replaces run_spec_poisson_multigpu (poisson_solver_multigpu.f90:328-554),
+ * single y-rank branch.  P is a device array of nx*nz*ny doubles holding the
+ * right-hand side on entry and the solution on exit (same pinned (0,0) mode as
+ * the reference; the mean is not removed).  Enqueued on `stream`; returns
+ * without synchronising.
+ */
+int dnsp_poisson_run(dnsp_poisson* h, double* P, void* stream);
+
+/* Out-of-place form of dnsp_poisson_run: reads the RHS from `rhs` (left
+ * unchanged unless rhs == P) and writes the solution to P.  Same kernels and
+ * HBM traffic as the in-place form. */
+int dnsp_poisson_solve(dnsp_poisson* h, const double* rhs, double* P, void* stream);
+
+/* Same as dnsp_poisson_run for a host array: copies in, solves, copies out,
+ * synchronises.  For callers whose P lives in host memory (the reference's
+ * CPU test driver). */
+int dnsp_poisson_run_host(dnsp_poisson* h, double* P_host);
+
+/* Stage entry points of the same solve (device arrays, in place):
+ *   forward  = x R2HC, z R2HC          (poisson_solver_multigpu.f90:360-393)
+ *   tridiag  = per-mode y Thomas       (:395-422)
+ *   backward = z HC2R, x HC2R, 1/(nx*nz) (:501-551)
+ * The spectrum uses FFTW's halfcomplex order along x and along z. */
+int dnsp_poisson_forward(dnsp_poisson* h, double* P, void* stream);
+int dnsp_poisson_tridiag(dnsp_poisson* h, double* S, void* stream);
+int dnsp_poisson_backward(dnsp_poisson* h, double* S, void* stream);
+
+/* One kernel of the solve, for timing and tuning:
+ *   DNSP_STAGE_X_FWD (:360-375), DNSP_STAGE_Z_FWD (:377-393), DNSP_STAGE_TRIDIAG (:395-422),
+ *   DNSP_STAGE_Z_BWD (:501-520), DNSP_STAGE_X_BWD (:523-551).
+ * Reads src, writes P; src may differ from P only for the two x stages.
+ * Running the five in order on one array equals dnsp_poisson_run. */
+#define DNSP_STAGE_X_FWD    0
+#define DNSP_STAGE_Z_FWD    1
+#define DNSP_STAGE_TRIDIAG  2
+#define DNSP_STAGE_Z_BWD    3
+#define DNSP_STAGE_X_BWD    4
+int dnsp_poisson_stage(dnsp_poisson* h, const double* src, double* P, int stage, void* stream);
+
+/* min |pivot| over all modes and rows: the init diagnostic min_abs_div_bacp
+ * (poisson_solver_multigpu.f90:181-222), computed on the device at init. */
+int dnsp_poisson_min_pivot(const dnsp_poisson* h, double* out);
+
+/* Sizes of the handle: nx, nz, ny. */
+int dnsp_poisson_shape(const dnsp_poisson* h, int* nx, int* nz, int* ny);
+
+/* Releases the handle's device memory (dealloc_poisson_solver_multigpu.f90). */
+int dnsp_poisson_free(dnsp_poisson* h);
+
+/* Block partition, remainder to the first blocks
+ * (poisson_solver_multigpu.f90:556-575).  Host only. */
+int dnsp_get_lo_n_bounds_1d(int mpi_divs, int n_glob, int* lo_blocks_mpi, int* n_blocks_mpi);
+
+/* Poisson y bands from the wall-normal faces yu(0:ny)
+ * (src/pycode/calc_bands.py:181-201 with yp_ext from :226-227).  Host only.
+ * is_dirich_first = 0 gives P_band_abc, 1 gives P_band_abc_00. */
+int dnsp_mk_P_abc(int ny, const double* yu, int is_dirich_first, double* a, double* b, double* c);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DNSPROJ_H */

@@ -1,0 +1,206 @@
+"""GPU parity of the FP64 Poisson solve (libdnsproj.so through its C ABI)
+against the CPU oracle on the same seeded inputs.
+
+Tolerances:
+  * tridiagonal stage: bit-exact (same operation order, no contraction);
+  * transform stages: 1e-13 relative to the max-abs of the stage output;
+  * whole solve: mean-removed max-abs error <= 1e-12 relative to the oracle's
+    mean-removed max-abs and <= 1e-10 absolute on the
+    reference-style random trials (main_poi_test.f90:259-262);
+  * full size (768 x 384 x 384): discrete-operator residual <= 1e-10
+    relative, forward/backward round trip, and agreement with the oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import dnsproj as D
+from dnsproj import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def to_dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to("cuda:0")
+
+
+def to_host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def make_solver(c):
+    bands = D.mk_P_abc(c["yu"], False) + D.mk_P_abc(c["yu"], True)
+    return D.PoissonSolver(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], bands), bands
+
+
+def rel_mean_removed(got, ref):
+    g = got - got.mean()
+    r = ref - ref.mean()
+    return np.abs(g - r).max() / max(np.abs(r).max(), 1e-300)
+
+
+def case_grid(nx, nz, ny, seed):
+    rng = np.random.default_rng(seed)
+    yu = W.tanh_faces(ny, float(rng.uniform(1.2, 2.2)))
+    return dict(nx=nx, nz=nz, ny=ny, inv_dx=nx / W.LX, inv_dz=nz / W.LZ, yu=yu,
+                rhs=W.rhs_uniform((ny, nz, nx), seed, yu))
+
+
+STAGE_CASES = [(64, 32, 64), (5, 7, 3), (6, 10, 9), (1, 1, 2), (2, 3, 4), (15, 14, 5), (768, 6, 3),
+               (12, 384, 3), (96, 96, 17), (35, 49, 6)]
+
+
+@pytest.mark.parametrize("nx,nz,ny", STAGE_CASES)
+def test_stages_match_oracle(cuda, nx, nz, ny):
+    say("stages", nx, nz, ny)
+    c = case_grid(nx, nz, ny, nx * 31 + nz * 7 + ny)
+    s, bands = make_solver(c)
+    f = c["rhs"]
+    # forward
+    t = to_dev(f)
+    s.forward(t)
+    Sg = to_host(t)
+    Sr = O.forward(f)
+    assert np.abs(Sg - Sr).max() <= 1e-13 * np.abs(Sr).max()
+    # tridiagonal on identical input: bit-exact
+    t = to_dev(Sr)
+    s.tridiag(t)
+    Tg = to_host(t)
+    Tr = O.tridiag(Sr, c["inv_dx"], c["inv_dz"], bands)
+    assert np.array_equal(Tg, Tr)
+    # backward
+    t = to_dev(Tr)
+    s.backward(t)
+    Pg = to_host(t)
+    Pr = O.backward(Tr)
+    assert np.abs(Pg - Pr).max() <= 1e-13 * np.abs(Pr).max()
+    s.close()
+
+
+def test_config1_solution_parity(cuda):
+    say("config1 solve")
+    c = W.config1()
+    s, bands = make_solver(c)
+    t = to_dev(c["rhs"])
+    D.run_spec_poisson_multigpu(s, t)
+    got = to_host(t)
+    ref = O.solve(c["rhs"], c["inv_dx"], c["inv_dz"], bands)
+    assert rel_mean_removed(got, ref) <= 1e-12
+    r = O.apply_operator(got, c["inv_dx"], c["inv_dz"], bands) - c["rhs"]
+    assert np.abs(r).max() <= 1e-11 * np.abs(c["rhs"]).max()
+    mp = s.min_pivot
+    assert abs(mp - O.min_pivot(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], bands)) <= 1e-14 * mp
+    s.close()
+
+
+def test_reference_style_random_trials(cuda):
+    for n, c in enumerate(W.reference_style_trials(10, seed=0)):
+        say("trial", n, c["nx"], c["nz"], c["ny"])
+        bands = O.bands_from_yu(c["yu"])
+        s = D.init_spec_poisson_multigpu(*bands, c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"])
+        t = to_dev(c["rhs"])
+        D.run_spec_poisson_multigpu(s, t)
+        got = to_host(t)
+        ref = O.solve(c["rhs"], c["inv_dx"], c["inv_dz"], bands)
+        err = np.abs((got - got.mean()) - (ref - ref.mean())).max()
+        assert err <= 1e-10
+        assert rel_mean_removed(got, ref) <= 1e-12
+        s.close()
+
+
+@pytest.mark.parametrize("nx,nz,ny", [(7, 25, 4), (45, 21, 8), (128, 64, 33), (256, 192, 12), (384, 128, 9),
+                                      (512, 256, 5), (192, 96, 40)])
+def test_more_sizes_solution_parity(cuda, nx, nz, ny):
+    say("sizes", nx, nz, ny)
+    c = case_grid(nx, nz, ny, nx + nz + ny)
+    s, bands = make_solver(c)
+    t = to_dev(c["rhs"])
+    s.run(t)
+    got = to_host(t)
+    ref = O.solve(c["rhs"], c["inv_dx"], c["inv_dz"], bands)
+    assert rel_mean_removed(got, ref) <= 1e-12
+    s.close()
+
+
+def test_run_host_matches_device_run(cuda):
+    say("run_host")
+    c = case_grid(48, 20, 10, 3)
+    s, bands = make_solver(c)
+    h = c["rhs"].copy()
+    s.run_host(h)
+    t = to_dev(c["rhs"])
+    s.run(t)
+    assert np.array_equal(h, to_host(t))
+    s.close()
+
+
+def test_unsupported_length_is_an_error(cuda):
+    say("unsupported")
+    c = case_grid(11, 8, 4, 0)
+    with pytest.raises(D.DnspError) as e:
+        make_solver(c)
+    assert e.value.code == D.DNSP_ERR_UNSUPPORTED
+
+
+def test_singular_bands_are_an_error(cuda):
+    say("singular")
+    ny = 4
+    yu = W.tanh_faces(ny, 1.5)
+    a, b, cc = D.mk_P_abc(yu, False)
+    z = np.zeros(ny)
+    # all-zero bands for the (0,0) mode: its first pivot b0 + 0 is exactly zero
+    with pytest.raises(D.DnspError) as e:
+        D.PoissonSolver(4, 4, ny, 1.0, 1.0, (a, b, cc, z, z, z))
+    assert e.value.code == D.DNSP_ERR_SINGULAR
+
+
+def test_repeated_runs_are_deterministic(cuda):
+    say("determinism")
+    c = case_grid(96, 48, 16, 9)
+    s, _ = make_solver(c)
+    outs = []
+    for _ in range(3):
+        t = to_dev(c["rhs"])
+        s.run(t)
+        outs.append(to_host(t))
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    s.close()
+
+
+@pytest.mark.slow
+def test_config2_full_size(cuda):
+    """768 x 384 x 384: residual, round trip and oracle agreement."""
+    import torch
+    say("config2 build rhs")
+    c = W.config2()
+    s, bands = make_solver(c)
+    f = c["rhs"]
+    say("config2 round trip")
+    t = to_dev(f)
+    s.forward(t)
+    s.backward(t)
+    rt = to_host(t)
+    assert np.abs(rt - f).max() <= 1e-13 * np.abs(f).max()
+    say("config2 solve")
+    t = to_dev(f)
+    s.run(t)
+    got = to_host(t)
+    del t
+    torch.cuda.empty_cache()
+    say("config2 residual (cpu)")
+    r = O.apply_operator(got, c["inv_dx"], c["inv_dz"], bands) - f
+    assert np.abs(r).max() <= 1e-10 * np.abs(f).max()
+    say("config2 oracle solve (cpu)")
+    ref = O.solve(f, c["inv_dx"], c["inv_dz"], bands)
+    assert rel_mean_removed(got, ref) <= 1e-12
+    s.close()
