@@ -23,6 +23,7 @@
 
 #include "common.h"
 #include "fft_lds.h"
+#include "poisson_ct.h"
 
 namespace dnsp {
 
@@ -335,6 +336,88 @@ static void eigenvalues(int n, double inv_d, std::vector<double>& lam, std::vect
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// dispatch of the compile-time-sized kernels
+// ---------------------------------------------------------------------------
+
+#define DNSP_CT_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768)
+
+static bool ct_fft_supported(int n) {
+    switch (n) {
+#define DNSP_CASE(N) case N:
+        DNSP_CT_SIZES(DNSP_CASE)
+#undef DNSP_CASE
+        return true;
+        default: return false;
+    }
+}
+
+template <int N>
+static void ct_x(bool inv, const double* src, double* dst, long nlines, const double2* tw, double scale,
+                 hipStream_t st) {
+    constexpr int G = ct_lines<N>();
+    const int nb = (int)((nlines + 2 * G - 1) / (2 * G));
+    if (!inv) kt_xfwd<N, G><<<nb, kCtThreads, 0, st>>>(src, dst, nlines, tw);
+    else      kt_xbwd<N, G><<<nb, kCtThreads, 0, st>>>(src, dst, nlines, tw, scale);
+}
+
+template <int N>
+static void ct_z(bool inv, double* S, int nx, int ny, const double2* tw, hipStream_t st) {
+    constexpr int TA = ct_cols<N>();
+    const int ncol = (nx + 1) / 2;
+    const int ntc = (ncol + TA - 1) / TA;
+    const int nb = ntc * ny;
+    if (!inv) kt_zfwd<N, TA><<<nb, kCtThreads, 0, st>>>(S, nx, ncol, ntc, tw);
+    else      kt_zbwd<N, TA><<<nb, kCtThreads, 0, st>>>(S, nx, ncol, ntc, tw);
+}
+
+static void ct_x_launch(int n, bool inv, const double* src, double* dst, long nlines, const double2* tw,
+                        double scale, hipStream_t st) {
+    switch (n) {
+#define DNSP_CASE(N) case N: ct_x<N>(inv, src, dst, nlines, tw, scale, st); break;
+        DNSP_CT_SIZES(DNSP_CASE)
+#undef DNSP_CASE
+        default: break;
+    }
+}
+
+static void ct_z_launch(int n, bool inv, double* S, int nx, int ny, const double2* tw, hipStream_t st) {
+    switch (n) {
+#define DNSP_CASE(N) case N: ct_z<N>(inv, S, nx, ny, tw, st); break;
+        DNSP_CT_SIZES(DNSP_CASE)
+#undef DNSP_CASE
+        default: break;
+    }
+}
+
+// partitioned tridiagonal: P = 32 chunks of MR rows, M = 16 modes per workgroup
+constexpr int kPartChunks = 32;
+constexpr int kPartModes = 16;
+#define DNSP_PART_ROWS(X) X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32)
+
+static int part_rows_for(int ny) {
+    for (int mr : {2, 3, 4, 6, 8, 12, 16, 24, 32})
+        if (mr * kPartChunks >= ny) return mr;
+    return 0;
+}
+
+static void part_launch(int mr, double* S, const double* ax, const double* az, const int* i0x,
+                        const int* i0z, const double* bands, int nx, int nz, int ny, hipStream_t st) {
+    const long plane = (long)nx * nz;
+    const int nb = (int)((plane + kPartModes - 1) / kPartModes);
+    switch (mr) {
+#define DNSP_CASE(MR)                                                                                  \
+    case MR:                                                                                           \
+        kt_tri_part<MR, kPartChunks, kPartModes><<<nb, kPartChunks * kPartModes, 0, st>>>(              \
+            S, ax, az, i0x, i0z, bands, nx, ny, plane);                                                \
+        break;
+        DNSP_PART_ROWS(DNSP_CASE)
+#undef DNSP_CASE
+        default: break;
+    }
+}
+
 }  // namespace dnsp
 
 using namespace dnsp;
@@ -353,7 +436,11 @@ struct dnsp_poisson {
     double* d_bands = nullptr;
     double2* d_twx = nullptr;
     double2* d_twz = nullptr;
-    double* d_cp = nullptr;    // Thomas c' workspace, nx*nz*ny
+    double* d_cp = nullptr;    // serial Thomas c' workspace, nx*nz*ny (allocated when used)
+    int fft_strategy = DNSP_FFT_AUTO;
+    int tri_strategy = DNSP_TRIDIAG_AUTO;
+    bool ct_x = false, ct_z = false;  // compile-time FFT kernels in use
+    int part_rows = 0;                // partitioned tridiagonal rows per chunk (0 = serial)
     double* d_host = nullptr;  // staging for dnsp_poisson_run_host
     double min_pivot = 0;
 };
@@ -416,8 +503,6 @@ static int poisson_setup(dnsp_poisson* h, const double* const bands[6]) {
     h->px.tw = h->d_twx;
     h->pz.tw = h->d_twz;
 
-    const size_t N = (size_t)nx * nz * ny;
-    DNSP_HIP_TRY(hipMalloc(&h->d_cp, sizeof(double) * N));
 
     // min |pivot| diagnostic (init_spec_poisson_multigpu :181-222)
     const long plane = (long)nx * nz;
@@ -445,7 +530,57 @@ static int poisson_setup(dnsp_poisson* h, const double* const bands[6]) {
     return DNSP_OK;
 }
 
+static int poisson_apply_strategy(dnsp_poisson* h) {
+    const bool want_ct = h->fft_strategy != DNSP_FFT_GENERIC;
+    h->ct_x = want_ct && ct_fft_supported(h->nx);
+    h->ct_z = want_ct && ct_fft_supported(h->nz);
+    if (h->fft_strategy == DNSP_FFT_FIXED && !(h->ct_x && h->ct_z)) {
+        set_error("no compile-time FFT kernels for nx=%d / nz=%d", h->nx, h->nz);
+        return DNSP_ERR_UNSUPPORTED;
+    }
+    const int mr = part_rows_for(h->ny);
+    if (h->tri_strategy == DNSP_TRIDIAG_PARTITION && mr == 0) {
+        set_error("ny=%d too large for the partitioned tridiagonal kernel", h->ny);
+        return DNSP_ERR_UNSUPPORTED;
+    }
+    const bool part = (h->tri_strategy == DNSP_TRIDIAG_PARTITION) ||
+                      (h->tri_strategy == DNSP_TRIDIAG_AUTO && mr > 0 && h->ny >= 32);
+    h->part_rows = part ? mr : 0;
+    if (!part && !h->d_cp) {
+        const size_t N = (size_t)h->nx * h->nz * h->ny;
+        DNSP_HIP_TRY(hipMalloc(&h->d_cp, sizeof(double) * N));
+    }
+    return DNSP_OK;
+}
+
 extern "C" {
+
+int dnsp_poisson_set_strategy(dnsp_poisson* h, int fft, int tridiag) {
+    clear_error();
+    DNSP_ARG_CHECK(h, "NULL handle");
+    DNSP_ARG_CHECK(fft >= DNSP_FFT_AUTO && fft <= DNSP_FFT_FIXED, "fft strategy %d", fft);
+    DNSP_ARG_CHECK(tridiag >= DNSP_TRIDIAG_AUTO && tridiag <= DNSP_TRIDIAG_PARTITION, "tridiag strategy %d",
+                   tridiag);
+    const int of = h->fft_strategy, ot = h->tri_strategy;
+    h->fft_strategy = fft;
+    h->tri_strategy = tridiag;
+    int rc = poisson_apply_strategy(h);
+    if (rc) {
+        h->fft_strategy = of;
+        h->tri_strategy = ot;
+        (void)poisson_apply_strategy(h);
+    }
+    return rc;
+}
+
+int dnsp_poisson_get_strategy(const dnsp_poisson* h, int* fft_ct_x, int* fft_ct_z, int* part_rows) {
+    clear_error();
+    DNSP_ARG_CHECK(h && fft_ct_x && fft_ct_z && part_rows, "NULL argument");
+    *fft_ct_x = h->ct_x ? 1 : 0;
+    *fft_ct_z = h->ct_z ? 1 : 0;
+    *part_rows = h->part_rows;
+    return DNSP_OK;
+}
 
 int dnsp_poisson_init(int nx_glob, int nz_glob, int ny, double inv_dx, double inv_dz,
                       const double* P_band_a, const double* P_band_b, const double* P_band_c,
@@ -477,6 +612,7 @@ int dnsp_poisson_init(int nx_glob, int nz_glob, int ny, double inv_dx, double in
     h->inv_dz = inv_dz;
     h->precision = precision;
     int rc = poisson_setup(h, bands);
+    if (!rc) rc = poisson_apply_strategy(h);
     if (rc) {
         poisson_release(h);
         return rc;
@@ -496,28 +632,36 @@ int dnsp_poisson_stage(dnsp_poisson* h, const double* src, double* P, int stage,
     const dim3 gz((h->ncol + h->TAz - 1) / h->TAz, h->ny);
     const size_t ldsx = sizeof(double2) * h->Gx * h->nx;
     const size_t ldsz = sizeof(double2) * h->TAz * h->nz;
+    const double inv_nxnz = 1.0 / ((double)h->nx * (double)h->nz + 0.0);
     switch (stage) {
         case DNSP_STAGE_X_FWD:
-            k_xfwd<<<gx, kFftThreads, ldsx, st>>>(src, P, h->nx, nlines, h->Gx, h->px);
+            if (h->ct_x) ct_x_launch(h->nx, false, src, P, nlines, h->d_twx, 1.0, st);
+            else k_xfwd<<<gx, kFftThreads, ldsx, st>>>(src, P, h->nx, nlines, h->Gx, h->px);
             break;
         case DNSP_STAGE_Z_FWD:
-            k_zfwd<<<gz, kFftThreads, ldsz, st>>>(P, h->nx, h->nz, h->ncol, h->TAz, h->pz);
+            if (h->ct_z) ct_z_launch(h->nz, false, P, h->nx, h->ny, h->d_twz, st);
+            else k_zfwd<<<gz, kFftThreads, ldsz, st>>>(P, h->nx, h->nz, h->ncol, h->TAz, h->pz);
             break;
         case DNSP_STAGE_TRIDIAG: {
-            const long plane = (long)h->nx * h->nz;
-            const int nb = (int)((plane + 255) / 256);
-            k_thomas<<<nb, 256, 0, st>>>(P, h->d_cp, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands,
-                                         h->nx, h->nz, h->ny);
+            if (h->part_rows > 0) {
+                part_launch(h->part_rows, P, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands, h->nx, h->nz,
+                            h->ny, st);
+            } else {
+                const long plane = (long)h->nx * h->nz;
+                const int nb = (int)((plane + 255) / 256);
+                k_thomas<<<nb, 256, 0, st>>>(P, h->d_cp, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands,
+                                             h->nx, h->nz, h->ny);
+            }
             break;
         }
         case DNSP_STAGE_Z_BWD:
-            k_zbwd<<<gz, kFftThreads, ldsz, st>>>(P, h->nx, h->nz, h->ncol, h->TAz, h->pz);
+            if (h->ct_z) ct_z_launch(h->nz, true, P, h->nx, h->ny, h->d_twz, st);
+            else k_zbwd<<<gz, kFftThreads, ldsz, st>>>(P, h->nx, h->nz, h->ncol, h->TAz, h->pz);
             break;
-        case DNSP_STAGE_X_BWD: {
-            const double inv_nxnz = 1.0 / ((double)h->nx * (double)h->nz + 0.0);
-            k_xbwd<<<gx, kFftThreads, ldsx, st>>>(src, P, h->nx, nlines, h->Gx, h->px, inv_nxnz);
+        case DNSP_STAGE_X_BWD:
+            if (h->ct_x) ct_x_launch(h->nx, true, src, P, nlines, h->d_twx, inv_nxnz, st);
+            else k_xbwd<<<gx, kFftThreads, ldsx, st>>>(src, P, h->nx, nlines, h->Gx, h->px, inv_nxnz);
             break;
-        }
         default:
             set_error("unknown stage %d", stage);
             return DNSP_ERR_ARG;

@@ -1,0 +1,411 @@
+// Compile-time-sized FP64 kernels of the Poisson solve: x and z transforms for
+// lengths 32..768 and the partitioned wall-normal tridiagonal solve.
+// Included by poisson.hip only.  Layout and halfcomplex conventions are those
+// of the runtime-sized kernels in poisson.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_ct.h"
+
+namespace dnsp {
+
+constexpr int kCtThreads = 256;
+// complex elements per FFT workgroup tile
+constexpr int kCtTile = 3072;
+
+template <int N>
+constexpr int ct_lines() {  // complex lines (= 2 real lines) per x workgroup
+    return (kCtTile / N) > 0 ? (kCtTile / N) : 1;
+}
+template <int N>
+constexpr int ct_cols() {  // complex columns per z workgroup, at most 16
+    return (kCtTile / N) > 16 ? 16 : ((kCtTile / N) > 0 ? (kCtTile / N) : 1);
+}
+
+__device__ __forceinline__ int zrow(int c, int half, int nx) {
+    if (half == 0) return c;
+    if (c == 0) return (nx % 2 == 0 && nx > 1) ? nx / 2 : -1;
+    return nx - c;
+}
+
+// ---------------------------------------------------------------------------
+// x forward: 2G real lines (one contiguous block of memory) per workgroup
+// ---------------------------------------------------------------------------
+template <int N, int G>
+__global__ __launch_bounds__(kCtThreads) void kt_xfwd(const double* P, double* S, long nlines,
+                                                      const double2* __restrict__ tw) {
+    constexpr int NT = kCtThreads;
+    constexpr int LP = ct_pitch(N);
+    constexpr int NL = (G * N + NT - 1) / NT;  // double2 loads per thread
+    __shared__ double2 sm[G * LP];
+    double* smd = reinterpret_cast<double*>(sm);
+    const long l0 = (long)xcd_tile(blockIdx.x, gridDim.x) * 2 * G;
+    const double2* src = reinterpret_cast<const double2*>(P + l0 * N);
+    double2 v[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int r = (2 * e) / N;
+        v[q] = ((G * N % NT == 0 || e < G * N) && l0 + r < nlines) ? src[e] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        if (G * N % NT == 0 || e < G * N) {
+            const int r = (2 * e) / N;
+            const int t = 2 * e - r * N;
+            const int g = r >> 1, c = r & 1;
+            smd[(g * LP + ct_pad(t)) * 2 + c] = v[q].x;
+            smd[(g * LP + ct_pad(t + 1)) * 2 + c] = v[q].y;
+        }
+    }
+    __syncthreads();
+    fft_ct<N, G, NT, false>(sm, tw);
+    double2* dst = reinterpret_cast<double2*>(S + l0 * N);
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int r = (2 * e) / N;
+        if ((G * N % NT == 0 || e < G * N) && l0 + r < nlines) {
+            const int t0 = 2 * e - r * N;
+            const int g = r >> 1;
+            double out[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int t = t0 + h;
+                const bool lo = 2 * t <= N;
+                const int u = lo ? t : N - t;
+                const double2 C = sm[g * LP + ct_pad(u)];
+                const double2 Cm = sm[g * LP + ct_pad(u == 0 ? 0 : N - u)];
+                if ((r & 1) == 0) out[h] = lo ? 0.5 * (C.x + Cm.x) : 0.5 * (C.y - Cm.y);
+                else              out[h] = lo ? 0.5 * (C.y + Cm.y) : 0.5 * (Cm.x - C.x);
+            }
+            dst[e] = make_double2(out[0], out[1]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x backward (+ scale): halfcomplex pairs of lines -> real lines
+// ---------------------------------------------------------------------------
+template <int N, int G>
+__global__ __launch_bounds__(kCtThreads) void kt_xbwd(const double* S, double* P, long nlines,
+                                                      const double2* __restrict__ tw, double scale) {
+    constexpr int NT = kCtThreads;
+    constexpr int LP = ct_pitch(N);
+    constexpr int H = N / 2 + 1;
+    constexpr int NI = (G * H + NT - 1) / NT;
+    constexpr int NL = (G * N + NT - 1) / NT;
+    __shared__ double2 sm[G * LP];
+    const long l0 = (long)xcd_tile(blockIdx.x, gridDim.x) * 2 * G;
+    const double* src = S + l0 * N;
+    double xr[NI], xi[NI], yr[NI], yi[NI];
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int g = e / H;
+        const int t = e - g * H;
+        const bool ok = e < G * H;
+        const bool single = (t == 0) || (2 * t == N);
+        const bool va = ok && (l0 + 2 * g < nlines);
+        const bool vb = ok && (l0 + 2 * g + 1 < nlines);
+        const double* la = src + (2 * g) * N;
+        const double* lb = la + N;
+        xr[q] = va ? la[t] : 0.0;
+        xi[q] = (va && !single) ? la[N - t] : 0.0;
+        yr[q] = vb ? lb[t] : 0.0;
+        yi[q] = (vb && !single) ? lb[N - t] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        if (e < G * H) {
+            const int g = e / H;
+            const int t = e - g * H;
+            const bool single = (t == 0) || (2 * t == N);
+            sm[g * LP + ct_pad(t)] = make_double2(xr[q] - yi[q], xi[q] + yr[q]);
+            if (!single) sm[g * LP + ct_pad(N - t)] = make_double2(xr[q] + yi[q], yr[q] - xi[q]);
+        }
+    }
+    __syncthreads();
+    fft_ct<N, G, NT, true>(sm, tw);
+    const double* smd = reinterpret_cast<const double*>(sm);
+    double2* dst = reinterpret_cast<double2*>(P + l0 * N);
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int r = (2 * e) / N;
+        if ((G * N % NT == 0 || e < G * N) && l0 + r < nlines) {
+            const int t = 2 * e - r * N;
+            const int g = r >> 1, c = r & 1;
+            dst[e] = make_double2(smd[(g * LP + ct_pad(t)) * 2 + c] * scale,
+                                  smd[(g * LP + ct_pad(t + 1)) * 2 + c] * scale);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// z forward: TA complex columns (halfcomplex x rows c and nx-c) of one plane
+// ---------------------------------------------------------------------------
+template <int N, int TA>
+__global__ __launch_bounds__(kCtThreads) void kt_zfwd(double* S, int nx, int ncol, int ntc,
+                                                      const double2* __restrict__ tw) {
+    constexpr int NT = kCtThreads;
+    constexpr int LP = ct_pitch(N);
+    constexpr int TOT = 2 * TA * N;
+    constexpr int NL = (TOT + NT - 1) / NT;
+    __shared__ double2 sm[TA * LP];
+    double* smd = reinterpret_cast<double*>(sm);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = tile / ntc;
+    const int c0 = (tile - k * ntc) * TA;
+    double* Sk = S + (size_t)k * nx * N;
+    double v[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int tt = e % TA;
+        const int rest = e / TA;
+        const int hf = rest & 1;
+        const int j = rest >> 1;
+        const int c = c0 + tt;
+        const int row = (c < ncol && (TOT % NT == 0 || e < TOT)) ? zrow(c, hf, nx) : -1;
+        v[q] = (row >= 0) ? Sk[(size_t)j * nx + row] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        if (TOT % NT == 0 || e < TOT) {
+            const int tt = e % TA;
+            const int rest = e / TA;
+            smd[(tt * LP + ct_pad(rest >> 1)) * 2 + (rest & 1)] = v[q];
+        }
+    }
+    __syncthreads();
+    fft_ct<N, TA, NT, false>(sm, tw);
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int tt = e % TA;
+        const int rest = e / TA;
+        const int hf = rest & 1;
+        const int b = rest >> 1;
+        const int c = c0 + tt;
+        const int row = (c < ncol && (TOT % NT == 0 || e < TOT)) ? zrow(c, hf, nx) : -1;
+        if (row >= 0) {
+            const bool lo = 2 * b <= N;
+            const int qq = lo ? b : N - b;
+            const double2 Z = sm[tt * LP + ct_pad(qq)];
+            const double2 Zm = sm[tt * LP + ct_pad(qq == 0 ? 0 : N - qq)];
+            double val;
+            if (hf == 0) val = lo ? 0.5 * (Z.x + Zm.x) : 0.5 * (Z.y - Zm.y);
+            else         val = lo ? 0.5 * (Z.y + Zm.y) : 0.5 * (Zm.x - Z.x);
+            Sk[(size_t)b * nx + row] = val;
+        }
+    }
+}
+
+template <int N, int TA>
+__global__ __launch_bounds__(kCtThreads) void kt_zbwd(double* S, int nx, int ncol, int ntc,
+                                                      const double2* __restrict__ tw) {
+    constexpr int NT = kCtThreads;
+    constexpr int LP = ct_pitch(N);
+    constexpr int H = N / 2 + 1;
+    constexpr int NI = (H * TA + NT - 1) / NT;
+    constexpr int TOT = 2 * TA * N;
+    constexpr int NL = (TOT + NT - 1) / NT;
+    __shared__ double2 sm[TA * LP];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = tile / ntc;
+    const int c0 = (tile - k * ntc) * TA;
+    double* Sk = S + (size_t)k * nx * N;
+    double ur[NI], ui[NI], vr[NI], vi[NI];
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int tt = e % TA;
+        const int qq = e / TA;
+        const int c = c0 + tt;
+        const bool ok = (e < H * TA) && c < ncol;
+        const int ru = ok ? zrow(c, 0, nx) : -1;
+        const int rv = ok ? zrow(c, 1, nx) : -1;
+        const bool single = (qq == 0) || (2 * qq == N);
+        ur[q] = (ru >= 0) ? Sk[(size_t)qq * nx + ru] : 0.0;
+        ui[q] = (ru >= 0 && !single) ? Sk[(size_t)(N - qq) * nx + ru] : 0.0;
+        vr[q] = (rv >= 0) ? Sk[(size_t)qq * nx + rv] : 0.0;
+        vi[q] = (rv >= 0 && !single) ? Sk[(size_t)(N - qq) * nx + rv] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        if (e < H * TA) {
+            const int tt = e % TA;
+            const int qq = e / TA;
+            const bool single = (qq == 0) || (2 * qq == N);
+            sm[tt * LP + ct_pad(qq)] = make_double2(ur[q] - vi[q], ui[q] + vr[q]);
+            if (!single) sm[tt * LP + ct_pad(N - qq)] = make_double2(ur[q] + vi[q], vr[q] - ui[q]);
+        }
+    }
+    __syncthreads();
+    fft_ct<N, TA, NT, true>(sm, tw);
+    const double* smd = reinterpret_cast<const double*>(sm);
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int tt = e % TA;
+        const int rest = e / TA;
+        const int hf = rest & 1;
+        const int j = rest >> 1;
+        const int c = c0 + tt;
+        const int row = (c < ncol && (TOT % NT == 0 || e < TOT)) ? zrow(c, hf, nx) : -1;
+        if (row >= 0) Sk[(size_t)j * nx + row] = smd[(tt * LP + ct_pad(j)) * 2 + hf];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y: partitioned tridiagonal solve, one workgroup = M consecutive modes x P
+// chunks of MR rows.  Thread (mode mi, chunk p) keeps its MR right-hand sides
+// in registers; rows past ny are identity rows (a = c = 0, b = 1, d = 0).
+//
+// Chunk rows s..e (e = s+MR-1).  Downward elimination from row s+1 keeps the
+// fill-in on x_s:  A_k x_s + B_k x_k + c_k x_{k+1} = D_k  (k = s+1..e), i.e.
+//   f = a_k / B_{k-1},  A_k = -f A_{k-1},  B_k = beta_k - f c_{k-1},  D_k = d_k - f D_{k-1}.
+// Interior rows then read x_k = D^_k - A^_k x_s - C^_k x_{k+1} (hat = divided
+// by B_k).  An upward pass folds them into x_{s+1} = P - Q x_s - R x_e, which
+// turns row s into  a_s x_{s-1} + (beta_s - c_s Q) x_s - c_s R x_e = d_s - c_s P.
+// The 2P tip rows (s and e of every chunk) form a tridiagonal system per mode,
+// solved by parallel cyclic reduction in LDS; the interior rows follow by the
+// backward recurrence above.
+// ---------------------------------------------------------------------------
+template <int MR, int P, int M>
+__global__ __launch_bounds__(M * P) void kt_tri_part(double* S, const double* __restrict__ ax,
+                                                     const double* __restrict__ az,
+                                                     const int* __restrict__ is0x,
+                                                     const int* __restrict__ is0z,
+                                                     const double* __restrict__ bands, int nx, int ny,
+                                                     long plane) {
+    constexpr int NT = M * P;
+    constexpr int NR = 2 * P;  // reduced rows per mode
+    __shared__ double ra[NR][M], rb[NR][M], rc[NR][M], rd[NR][M];  // mode fastest
+    const int mi = (int)threadIdx.x % M;
+    const int p = (int)threadIdx.x / M;
+    const long mode = (long)xcd_tile(blockIdx.x, gridDim.x) * M + mi;
+    const bool valid = mode < plane;
+    const long md = valid ? mode : 0;
+    const int i = (int)(md % nx);
+    const int j = (int)(md / nx);
+    const double lam = ax[i] + az[j];
+    const double* A = bands + ((is0x[i] && is0z[j]) ? 3 * ny : 0);
+    const double* B = A + ny;
+    const double* C = B + ny;
+    const int s = p * MR;
+
+    double d[MR];
+#pragma unroll
+    for (int r = 0; r < MR; ++r) {
+        const int row = s + r;
+        d[r] = (valid && row < ny) ? S[(size_t)row * plane + md] : 0.0;
+    }
+    auto band_a = [&](int row) { return row < ny ? A[row] : 0.0; };
+    auto band_c = [&](int row) { return row < ny ? C[row] : 0.0; };
+    auto beta = [&](int row) { return row < ny ? B[row] + lam : 1.0; };
+
+    double Ah[MR], Ch[MR];  // d[] is overwritten by D^ for interior rows
+    double Ae, Be, De;
+    {
+        double Ak = band_a(s + 1);
+        double Bk = beta(s + 1);
+        double Dk = d[1];
+        double cprev = band_c(s + 1);
+#pragma unroll
+        for (int l = 2; l < MR; ++l) {
+            // normalise row l-1 (interior) before moving on
+            const double rinv = 1.0 / Bk;
+            d[l - 1] = Dk * rinv;
+            Ah[l - 1] = Ak * rinv;
+            Ch[l - 1] = cprev * rinv;
+            const double f = band_a(s + l) * rinv;
+            Ak = -f * Ak;
+            Bk = beta(s + l) - f * cprev;
+            Dk = d[l] - f * Dk;
+            cprev = band_c(s + l);
+        }
+        Ae = Ak;
+        Be = Bk;
+        De = Dk;
+    }
+    // upward pass: x_{s+1} = Pp - Qq x_s - Rr x_e
+    double Pp = 0.0, Qq = 0.0, Rr = -1.0;
+#pragma unroll
+    for (int l = MR - 2; l >= 1; --l) {
+        Pp = d[l] - Ch[l] * Pp;
+        Qq = Ah[l] - Ch[l] * Qq;
+        Rr = -Ch[l] * Rr;
+    }
+    {
+        const double cs = band_c(s);
+        ra[2 * p][mi] = band_a(s);
+        rb[2 * p][mi] = beta(s) - cs * Qq;
+        rc[2 * p][mi] = -cs * Rr;
+        rd[2 * p][mi] = d[0] - cs * Pp;
+        ra[2 * p + 1][mi] = Ae;
+        rb[2 * p + 1][mi] = Be;
+        rc[2 * p + 1][mi] = band_c(s + MR - 1);
+        rd[2 * p + 1][mi] = De;
+    }
+    __syncthreads();
+    // parallel cyclic reduction on the NR tip rows of each mode: row q = 2*t0 + h
+    for (int st = 1; st < NR; st *= 2) {
+        double na[2], nb[2], nc[2], nd[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = (int)threadIdx.x + h * NT;
+            const int mm = e % M;
+            const int q = e / M;
+            const double aq = ra[q][mm], bq = rb[q][mm], cq = rc[q][mm], dq = rd[q][mm];
+            double a2 = 0.0, b2 = bq, c2 = 0.0, d2 = dq;
+            if (q - st >= 0) {
+                const double al = -aq / rb[q - st][mm];
+                a2 = al * ra[q - st][mm];
+                b2 += al * rc[q - st][mm];
+                d2 += al * rd[q - st][mm];
+            }
+            if (q + st < NR) {
+                const double ga = -cq / rb[q + st][mm];
+                c2 = ga * rc[q + st][mm];
+                b2 += ga * ra[q + st][mm];
+                d2 += ga * rd[q + st][mm];
+            }
+            na[h] = a2; nb[h] = b2; nc[h] = c2; nd[h] = d2;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = (int)threadIdx.x + h * NT;
+            const int mm = e % M;
+            const int q = e / M;
+            ra[q][mm] = na[h]; rb[q][mm] = nb[h]; rc[q][mm] = nc[h]; rd[q][mm] = nd[h];
+        }
+        __syncthreads();
+    }
+    const double xs = rd[2 * p][mi] / rb[2 * p][mi];
+    const double xe = rd[2 * p + 1][mi] / rb[2 * p + 1][mi];
+    d[0] = xs;
+    double xn = xe;
+#pragma unroll
+    for (int l = MR - 2; l >= 1; --l) {
+        xn = d[l] - Ah[l] * xs - Ch[l] * xn;
+        d[l] = xn;
+    }
+    d[MR - 1] = xe;
+    if (valid) {
+#pragma unroll
+        for (int r = 0; r < MR; ++r) {
+            const int row = s + r;
+            if (row < ny) S[(size_t)row * plane + md] = d[r];
+        }
+    }
+}
+
+}  // namespace dnsp

@@ -26,6 +26,9 @@ DNSP_ERR_NOMEM = 3
 DNSP_ERR_SINGULAR = 4
 DNSP_ERR_UNSUPPORTED = 5
 
+FFT_AUTO, FFT_GENERIC, FFT_FIXED = 0, 1, 2
+TRIDIAG_AUTO, TRIDIAG_SERIAL, TRIDIAG_PARTITION = 0, 1, 2
+
 STAGE_X_FWD, STAGE_Z_FWD, STAGE_TRIDIAG, STAGE_Z_BWD, STAGE_X_BWD = range(5)
 STAGE_NAMES = ("x_fwd", "z_fwd", "tridiag", "z_bwd", "x_bwd")
 
@@ -57,6 +60,8 @@ _SIGNATURES = [
     ("dnsp_poisson_tridiag", [_vp, _vp, _vp], ctypes.c_int),
     ("dnsp_poisson_backward", [_vp, _vp, _vp], ctypes.c_int),
     ("dnsp_poisson_stage", [_vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+    ("dnsp_poisson_set_strategy", [_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    ("dnsp_poisson_get_strategy", [_vp, _ip, _ip, _ip], ctypes.c_int),
     ("dnsp_poisson_min_pivot", [_vp, _dp], ctypes.c_int),
     ("dnsp_poisson_shape", [_vp, _ip, _ip, _ip], ctypes.c_int),
     ("dnsp_poisson_free", [_vp], ctypes.c_int),
@@ -164,6 +169,15 @@ class PoissonSolver:
     @property
     def numel(self):
         return self.nx * self.nz * self.ny
+
+    def set_strategy(self, fft=FFT_AUTO, tridiag=TRIDIAG_AUTO):
+        _check(lib().dnsp_poisson_set_strategy(self._h, int(fft), int(tridiag)))
+
+    @property
+    def strategy(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().dnsp_poisson_get_strategy(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"fft_ct_x": a.value, "fft_ct_z": b.value, "part_rows": c.value}
 
     @property
     def min_pivot(self):

@@ -93,6 +93,28 @@ int dnsp_poisson_backward(dnsp_poisson* h, double* S, void* stream);
 #define DNSP_STAGE_X_BWD    4
 int dnsp_poisson_stage(dnsp_poisson* h, const double* src, double* P, int stage, void* stream);
 
+/* Kernel strategy (also the knobs of the tuner):
+ *   fft:     DNSP_FFT_AUTO (compile-time-sized kernels where the length has
+ *            them, else the runtime-sized ones), DNSP_FFT_GENERIC (always the
+ *            runtime-sized kernels), DNSP_FFT_FIXED (compile-time-sized only;
+ *            DNSP_ERR_UNSUPPORTED if a length has none)
+ *   tridiag: DNSP_TRIDIAG_AUTO (partitioned for ny >= 32), DNSP_TRIDIAG_SERIAL
+ *            (one serial Thomas sweep per mode, the reference's operation order
+ *            at :406-421, bit-identical to it; needs an nx*nz*ny workspace),
+ *            DNSP_TRIDIAG_PARTITION (registers + parallel cyclic reduction on
+ *            the chunk tips; one HBM read and write per point; ny <= 1024)
+ * No counterpart in the reference, which has one kernel per stage. */
+#define DNSP_FFT_AUTO            0
+#define DNSP_FFT_GENERIC         1
+#define DNSP_FFT_FIXED           2
+#define DNSP_TRIDIAG_AUTO        0
+#define DNSP_TRIDIAG_SERIAL      1
+#define DNSP_TRIDIAG_PARTITION   2
+int dnsp_poisson_set_strategy(dnsp_poisson* h, int fft, int tridiag);
+/* What the handle currently runs: 1/0 compile-time x and z FFT kernels, and
+ * the partitioned tridiagonal rows per chunk (0 = serial Thomas). */
+int dnsp_poisson_get_strategy(const dnsp_poisson* h, int* fft_ct_x, int* fft_ct_z, int* part_rows);
+
 /* min |pivot| over all modes and rows: the init diagnostic min_abs_div_bacp
  * (poisson_solver_multigpu.f90:181-222), computed on the device at init. */
 int dnsp_poisson_min_pivot(const dnsp_poisson* h, double* out);

@@ -2,7 +2,8 @@
 against the CPU oracle on the same seeded inputs.
 
 Tolerances:
-  * tridiagonal stage: bit-exact (same operation order, no contraction);
+  * tridiagonal stage: the serial kernel bit-exact (same operation order, no
+    contraction); the partitioned kernel within 1e-11 of each mode's max-abs;
   * transform stages: 1e-13 relative to the max-abs of the stage output;
   * whole solve: mean-removed max-abs error <= 1e-12 relative to the oracle's
     mean-removed max-abs and <= 1e-10 absolute on the
@@ -57,7 +58,8 @@ def case_grid(nx, nz, ny, seed):
 
 
 STAGE_CASES = [(64, 32, 64), (5, 7, 3), (6, 10, 9), (1, 1, 2), (2, 3, 4), (15, 14, 5), (768, 6, 3),
-               (12, 384, 3), (96, 96, 17), (35, 49, 6)]
+               (12, 384, 3), (96, 96, 17), (35, 49, 6), (32, 768, 5), (768, 32, 7), (512, 96, 4),
+               (192, 256, 3), (128, 64, 70), (384, 128, 40)]
 
 
 @pytest.mark.parametrize("nx,nz,ny", STAGE_CASES)
@@ -72,12 +74,21 @@ def test_stages_match_oracle(cuda, nx, nz, ny):
     Sg = to_host(t)
     Sr = O.forward(f)
     assert np.abs(Sg - Sr).max() <= 1e-13 * np.abs(Sr).max()
-    # tridiagonal on identical input: bit-exact
+    # tridiagonal on identical input: the serial kernel is bit-exact
+    Tr = O.tridiag(Sr, c["inv_dx"], c["inv_dz"], bands)
+    s.set_strategy(D.FFT_AUTO, D.TRIDIAG_SERIAL)
     t = to_dev(Sr)
     s.tridiag(t)
-    Tg = to_host(t)
-    Tr = O.tridiag(Sr, c["inv_dx"], c["inv_dz"], bands)
-    assert np.array_equal(Tg, Tr)
+    assert np.array_equal(to_host(t), Tr)
+    # the partitioned kernel (registers + PCR on chunk tips) agrees per mode
+    if ny <= 1024:
+        s.set_strategy(D.FFT_AUTO, D.TRIDIAG_PARTITION)
+        t = to_dev(Sr)
+        s.tridiag(t)
+        Tp = to_host(t)
+        scale = np.abs(Tr).max(axis=0, keepdims=True)
+        assert np.all(np.abs(Tp - Tr) <= 1e-11 * scale + 1e-300)
+    s.set_strategy(D.FFT_AUTO, D.TRIDIAG_AUTO)
     # backward
     t = to_dev(Tr)
     s.backward(t)
@@ -129,6 +140,43 @@ def test_more_sizes_solution_parity(cuda, nx, nz, ny):
     got = to_host(t)
     ref = O.solve(c["rhs"], c["inv_dx"], c["inv_dz"], bands)
     assert rel_mean_removed(got, ref) <= 1e-12
+    s.close()
+
+
+STRATEGY_CASES = [(64, 32, 64), (768, 96, 40), (96, 384, 33), (256, 128, 100), (48, 40, 130), (384, 192, 384)]
+
+
+@pytest.mark.parametrize("nx,nz,ny", STRATEGY_CASES)
+def test_every_strategy_matches_oracle(cuda, nx, nz, ny):
+    say("strategies", nx, nz, ny)
+    c = case_grid(nx, nz, ny, 5 * nx + ny)
+    s, bands = make_solver(c)
+    ref = O.solve(c["rhs"], c["inv_dx"], c["inv_dz"], bands)
+    outs = {}
+    for fft in (D.FFT_GENERIC, D.FFT_AUTO):
+        for tri in (D.TRIDIAG_SERIAL, D.TRIDIAG_PARTITION):
+            s.set_strategy(fft, tri)
+            t = to_dev(c["rhs"])
+            s.run(t)
+            outs[(fft, tri)] = to_host(t)
+            assert rel_mean_removed(outs[(fft, tri)], ref) <= 1e-12, (fft, tri, s.strategy)
+    s.close()
+
+
+def test_strategy_reporting(cuda):
+    say("strategy report")
+    c = W.config2_grid()
+    bands = D.mk_P_abc(c["yu"], False) + D.mk_P_abc(c["yu"], True)
+    s = D.PoissonSolver(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], bands)
+    assert s.strategy == {"fft_ct_x": 1, "fft_ct_z": 1, "part_rows": 12}
+    s.set_strategy(D.FFT_GENERIC, D.TRIDIAG_SERIAL)
+    assert s.strategy == {"fft_ct_x": 0, "fft_ct_z": 0, "part_rows": 0}
+    s.close()
+    c = case_grid(7, 25, 4, 1)
+    s, _ = make_solver(c)
+    with pytest.raises(D.DnspError):
+        s.set_strategy(D.FFT_FIXED, D.TRIDIAG_AUTO)
+    assert s.strategy["fft_ct_x"] == 0
     s.close()
 
 
