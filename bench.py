@@ -56,6 +56,7 @@ def main():
     P = torch.empty_like(f)
     stream = torch.cuda.current_stream()
 
+    tune = solver.autotune(work=P, reps=5)
     for _ in range(args.warmup):
         solver.solve(f, P)
     torch.cuda.synchronize()
@@ -144,6 +145,8 @@ def main():
         "p99_ms": round(float(np.percentile(lat, 99)), 4),
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "rel_residual": rel_res,
+        "kernel_variants": {"x": tune["picked"][0], "z": tune["picked"][1], "tridiag": tune["picked"][2],
+                            "tuner_ms": {k: [round(v, 4) for v in tune[k]] for k in ("x_ms", "z_ms", "tri_ms")}},
         "roofline": roofline,
         "solve_roofline": solve_roof,
         "cpu_baseline": cpu,

@@ -24,6 +24,7 @@
 #include "common.h"
 #include "fft_lds.h"
 #include "poisson_ct.h"
+#include "poisson_ct_launch.h"
 
 namespace dnsp {
 
@@ -337,63 +338,8 @@ static void eigenvalues(int n, double inv_d, std::vector<double>& lam, std::vect
 }
 
 
-// ---------------------------------------------------------------------------
-// dispatch of the compile-time-sized kernels
-// ---------------------------------------------------------------------------
-
-#define DNSP_CT_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768)
-
-static bool ct_fft_supported(int n) {
-    switch (n) {
-#define DNSP_CASE(N) case N:
-        DNSP_CT_SIZES(DNSP_CASE)
-#undef DNSP_CASE
-        return true;
-        default: return false;
-    }
-}
-
-template <int N>
-static void ct_x(bool inv, const double* src, double* dst, long nlines, const double2* tw, double scale,
-                 hipStream_t st) {
-    constexpr int G = ct_lines<N>();
-    const int nb = (int)((nlines + 2 * G - 1) / (2 * G));
-    if (!inv) kt_xfwd<N, G><<<nb, kCtThreads, 0, st>>>(src, dst, nlines, tw);
-    else      kt_xbwd<N, G><<<nb, kCtThreads, 0, st>>>(src, dst, nlines, tw, scale);
-}
-
-template <int N>
-static void ct_z(bool inv, double* S, int nx, int ny, const double2* tw, hipStream_t st) {
-    constexpr int TA = ct_cols<N>();
-    const int ncol = (nx + 1) / 2;
-    const int ntc = (ncol + TA - 1) / TA;
-    const int nb = ntc * ny;
-    if (!inv) kt_zfwd<N, TA><<<nb, kCtThreads, 0, st>>>(S, nx, ncol, ntc, tw);
-    else      kt_zbwd<N, TA><<<nb, kCtThreads, 0, st>>>(S, nx, ncol, ntc, tw);
-}
-
-static void ct_x_launch(int n, bool inv, const double* src, double* dst, long nlines, const double2* tw,
-                        double scale, hipStream_t st) {
-    switch (n) {
-#define DNSP_CASE(N) case N: ct_x<N>(inv, src, dst, nlines, tw, scale, st); break;
-        DNSP_CT_SIZES(DNSP_CASE)
-#undef DNSP_CASE
-        default: break;
-    }
-}
-
-static void ct_z_launch(int n, bool inv, double* S, int nx, int ny, const double2* tw, hipStream_t st) {
-    switch (n) {
-#define DNSP_CASE(N) case N: ct_z<N>(inv, S, nx, ny, tw, st); break;
-        DNSP_CT_SIZES(DNSP_CASE)
-#undef DNSP_CASE
-        default: break;
-    }
-}
-
 // partitioned tridiagonal: P = 32 chunks of MR rows, M = 16 modes per workgroup
 constexpr int kPartChunks = 32;
-constexpr int kPartModes = 16;
 #define DNSP_PART_ROWS(X) X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32)
 
 static int part_rows_for(int ny) {
@@ -402,20 +348,31 @@ static int part_rows_for(int ny) {
     return 0;
 }
 
-static void part_launch(int mr, double* S, const double* ax, const double* az, const int* i0x,
-                        const int* i0z, const double* bands, int nx, int nz, int ny, hipStream_t st) {
+// variant 0: 16 modes x 32 chunks (512 threads); variant 1: 8 modes x 32 chunks (256 threads)
+constexpr int kPartVariants = 2;
+
+template <int M>
+static void part_launch_m(int mr, double* S, const double* ax, const double* az, const int* i0x,
+                          const int* i0z, const double* bands, int nx, int nz, int ny, hipStream_t st) {
     const long plane = (long)nx * nz;
-    const int nb = (int)((plane + kPartModes - 1) / kPartModes);
+    const int nb = (int)((plane + M - 1) / M);
+    const size_t lds = 6 * sizeof(double) * ny;
     switch (mr) {
 #define DNSP_CASE(MR)                                                                                  \
     case MR:                                                                                           \
-        kt_tri_part<MR, kPartChunks, kPartModes><<<nb, kPartChunks * kPartModes, 0, st>>>(              \
-            S, ax, az, i0x, i0z, bands, nx, ny, plane);                                                \
+        kt_tri_part<MR, kPartChunks, M><<<nb, kPartChunks * M, lds, st>>>(S, ax, az, i0x, i0z, bands, \
+                                                                         nx, ny, plane);               \
         break;
         DNSP_PART_ROWS(DNSP_CASE)
 #undef DNSP_CASE
         default: break;
     }
+}
+
+static void part_launch(int variant, int mr, double* S, const double* ax, const double* az, const int* i0x,
+                        const int* i0z, const double* bands, int nx, int nz, int ny, hipStream_t st) {
+    if (variant == 1) part_launch_m<8>(mr, S, ax, az, i0x, i0z, bands, nx, nz, ny, st);
+    else part_launch_m<16>(mr, S, ax, az, i0x, i0z, bands, nx, nz, ny, st);
 }
 
 }  // namespace dnsp
@@ -440,6 +397,8 @@ struct dnsp_poisson {
     int fft_strategy = DNSP_FFT_AUTO;
     int tri_strategy = DNSP_TRIDIAG_AUTO;
     bool ct_x = false, ct_z = false;  // compile-time FFT kernels in use
+    int xvar = 0, zvar = 0;           // their (tile, threads) variants
+    int tvar = 0;                     // partitioned tridiagonal variant
     int part_rows = 0;                // partitioned tridiagonal rows per chunk (0 = serial)
     double* d_host = nullptr;  // staging for dnsp_poisson_run_host
     double min_pivot = 0;
@@ -582,6 +541,103 @@ int dnsp_poisson_get_strategy(const dnsp_poisson* h, int* fft_ct_x, int* fft_ct_
     return DNSP_OK;
 }
 
+int dnsp_poisson_set_variants(dnsp_poisson* h, int xvar, int zvar, int tvar) {
+    clear_error();
+    DNSP_ARG_CHECK(h, "NULL handle");
+    DNSP_ARG_CHECK(xvar >= 0 && xvar < kCtVariants && zvar >= 0 && zvar < kCtVariants,
+                   "FFT kernel variants must be in [0, %d)", kCtVariants);
+    DNSP_ARG_CHECK(tvar >= 0 && tvar < kPartVariants, "tridiagonal variant must be in [0, %d)", kPartVariants);
+    h->xvar = xvar;
+    h->zvar = zvar;
+    h->tvar = tvar;
+    return DNSP_OK;
+}
+
+int dnsp_poisson_get_variants(const dnsp_poisson* h, int* xvar, int* zvar, int* tvar) {
+    clear_error();
+    DNSP_ARG_CHECK(h && xvar && zvar && tvar, "NULL argument");
+    *xvar = h->xvar;
+    *zvar = h->zvar;
+    *tvar = h->tvar;
+    return DNSP_OK;
+}
+
+// Times `reps` forward+inverse pairs of each FFT kernel variant per direction,
+// and pairs of tridiagonal solves per partitioned variant, on `work`
+// (zero-filled first; NULL = a temporary device array) and keeps the fastest.
+// times_ms (optional, 2*kCtVariants + kPartVariants entries) receives the
+// mean milliseconds of x, z, then tridiagonal variants (-1 where not applicable).
+int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream, double* times_ms) {
+    clear_error();
+    DNSP_ARG_CHECK(h, "NULL handle");
+    DNSP_ARG_CHECK(reps >= 1 && reps <= 1000, "reps=%d", reps);
+    hipStream_t st = as_stream(stream);
+    const size_t N = (size_t)h->nx * h->nz * h->ny;
+    double* buf = work;
+    if (!buf) DNSP_HIP_TRY(hipMalloc(&buf, sizeof(double) * N));
+    int rc = DNSP_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double best[3] = {1e30, 1e30, 1e30};
+    int pick[3] = {h->xvar, h->zvar, h->tvar};
+    const int keep_x = h->xvar, keep_z = h->zvar, keep_t = h->tvar;
+    if (times_ms)
+        for (int i = 0; i < 2 * kCtVariants + kPartVariants; ++i) times_ms[i] = -1.0;
+    do {
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+            hipMemsetAsync(buf, 0, sizeof(double) * N, st) != hipSuccess) {
+            set_error("autotune: event or memset failed");
+            rc = DNSP_ERR_HIP;
+            break;
+        }
+        for (int dir = 0; dir < 3 && rc == DNSP_OK; ++dir) {
+            const bool has = dir == 0 ? h->ct_x : dir == 1 ? h->ct_z : h->part_rows > 0;
+            if (!has) continue;
+            const int nvar = dir == 2 ? kPartVariants : kCtVariants;
+            for (int v = 0; v < nvar && rc == DNSP_OK; ++v) {
+                if (dir == 0) h->xvar = v; else if (dir == 1) h->zvar = v; else h->tvar = v;
+                const int sf = dir == 0 ? DNSP_STAGE_X_FWD : dir == 1 ? DNSP_STAGE_Z_FWD : DNSP_STAGE_TRIDIAG;
+                const int sb = dir == 0 ? DNSP_STAGE_X_BWD : dir == 1 ? DNSP_STAGE_Z_BWD : DNSP_STAGE_TRIDIAG;
+                rc = dnsp_poisson_stage(h, buf, buf, sf, stream);  // warm-up
+                if (!rc) rc = dnsp_poisson_stage(h, buf, buf, sb, stream);
+                if (rc) break;
+                (void)hipEventRecord(e0, st);
+                for (int r = 0; r < reps && !rc; ++r) {
+                    rc = dnsp_poisson_stage(h, buf, buf, sf, stream);
+                    if (!rc) rc = dnsp_poisson_stage(h, buf, buf, sb, stream);
+                }
+                if (rc) break;
+                (void)hipEventRecord(e1, st);
+                if (hipEventSynchronize(e1) != hipSuccess) {
+                    set_error("autotune: event synchronize failed");
+                    rc = DNSP_ERR_HIP;
+                    break;
+                }
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                const double per = (double)ms / reps;
+                if (times_ms) times_ms[dir * kCtVariants + v] = per;  // dir 2 starts at 2*kCtVariants
+                if (per < best[dir]) {
+                    best[dir] = per;
+                    pick[dir] = v;
+                }
+            }
+        }
+    } while (0);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (!work && buf) (void)hipFree(buf);
+    if (rc) {
+        h->xvar = keep_x;
+        h->zvar = keep_z;
+        h->tvar = keep_t;
+        return rc;
+    }
+    h->xvar = pick[0];
+    h->zvar = pick[1];
+    h->tvar = pick[2];
+    return DNSP_OK;
+}
+
 int dnsp_poisson_init(int nx_glob, int nz_glob, int ny, double inv_dx, double inv_dz,
                       const double* P_band_a, const double* P_band_b, const double* P_band_c,
                       const double* P_band_a_00, const double* P_band_b_00, const double* P_band_c_00,
@@ -635,16 +691,16 @@ int dnsp_poisson_stage(dnsp_poisson* h, const double* src, double* P, int stage,
     const double inv_nxnz = 1.0 / ((double)h->nx * (double)h->nz + 0.0);
     switch (stage) {
         case DNSP_STAGE_X_FWD:
-            if (h->ct_x) ct_x_launch(h->nx, false, src, P, nlines, h->d_twx, 1.0, st);
+            if (h->ct_x) ct_x_launch(h->nx, h->xvar, false, src, P, nlines, h->d_twx, 1.0, st);
             else k_xfwd<<<gx, kFftThreads, ldsx, st>>>(src, P, h->nx, nlines, h->Gx, h->px);
             break;
         case DNSP_STAGE_Z_FWD:
-            if (h->ct_z) ct_z_launch(h->nz, false, P, h->nx, h->ny, h->d_twz, st);
+            if (h->ct_z) ct_z_launch(h->nz, h->zvar, false, P, h->nx, h->ny, h->d_twz, st);
             else k_zfwd<<<gz, kFftThreads, ldsz, st>>>(P, h->nx, h->nz, h->ncol, h->TAz, h->pz);
             break;
         case DNSP_STAGE_TRIDIAG: {
             if (h->part_rows > 0) {
-                part_launch(h->part_rows, P, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands, h->nx, h->nz,
+                part_launch(h->tvar, h->part_rows, P, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands, h->nx, h->nz,
                             h->ny, st);
             } else {
                 const long plane = (long)h->nx * h->nz;
@@ -655,11 +711,11 @@ int dnsp_poisson_stage(dnsp_poisson* h, const double* src, double* P, int stage,
             break;
         }
         case DNSP_STAGE_Z_BWD:
-            if (h->ct_z) ct_z_launch(h->nz, true, P, h->nx, h->ny, h->d_twz, st);
+            if (h->ct_z) ct_z_launch(h->nz, h->zvar, true, P, h->nx, h->ny, h->d_twz, st);
             else k_zbwd<<<gz, kFftThreads, ldsz, st>>>(P, h->nx, h->nz, h->ncol, h->TAz, h->pz);
             break;
         case DNSP_STAGE_X_BWD:
-            if (h->ct_x) ct_x_launch(h->nx, true, src, P, nlines, h->d_twx, inv_nxnz, st);
+            if (h->ct_x) ct_x_launch(h->nx, h->xvar, true, src, P, nlines, h->d_twx, inv_nxnz, st);
             else k_xbwd<<<gx, kFftThreads, ldsx, st>>>(src, P, h->nx, nlines, h->Gx, h->px, inv_nxnz);
             break;
         default:

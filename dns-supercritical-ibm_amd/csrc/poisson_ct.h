@@ -10,17 +10,19 @@
 
 namespace dnsp {
 
-constexpr int kCtThreads = 256;
-// complex elements per FFT workgroup tile
-constexpr int kCtTile = 3072;
+// FFT kernel variants: (complex elements per workgroup tile, threads).
+// The tuner picks one per direction and handle.
+constexpr int kCtVariants = 4;
+__host__ __device__ constexpr int ct_var_tile(int v) { return v == 0 ? 3072 : v == 1 ? 3072 : v == 2 ? 1536 : 6144; }
+__host__ __device__ constexpr int ct_var_threads(int v) { return v == 0 ? 256 : v == 1 ? 512 : v == 2 ? 256 : 512; }
 
-template <int N>
+template <int N, int TILE>
 constexpr int ct_lines() {  // complex lines (= 2 real lines) per x workgroup
-    return (kCtTile / N) > 0 ? (kCtTile / N) : 1;
+    return (TILE / N) > 0 ? (TILE / N) : 1;
 }
-template <int N>
+template <int N, int TILE>
 constexpr int ct_cols() {  // complex columns per z workgroup, at most 16
-    return (kCtTile / N) > 16 ? 16 : ((kCtTile / N) > 0 ? (kCtTile / N) : 1);
+    return (TILE / N) > 16 ? 16 : ((TILE / N) > 0 ? (TILE / N) : 1);
 }
 
 __device__ __forceinline__ int zrow(int c, int half, int nx) {
@@ -32,10 +34,9 @@ __device__ __forceinline__ int zrow(int c, int half, int nx) {
 // ---------------------------------------------------------------------------
 // x forward: 2G real lines (one contiguous block of memory) per workgroup
 // ---------------------------------------------------------------------------
-template <int N, int G>
-__global__ __launch_bounds__(kCtThreads) void kt_xfwd(const double* P, double* S, long nlines,
+template <int N, int G, int NT>
+__global__ __launch_bounds__(NT) void kt_xfwd(const double* P, double* S, long nlines,
                                                       const double2* __restrict__ tw) {
-    constexpr int NT = kCtThreads;
     constexpr int LP = ct_pitch(N);
     constexpr int NL = (G * N + NT - 1) / NT;  // double2 loads per thread
     __shared__ double2 sm[G * LP];
@@ -89,10 +90,9 @@ __global__ __launch_bounds__(kCtThreads) void kt_xfwd(const double* P, double* S
 // ---------------------------------------------------------------------------
 // x backward (+ scale): halfcomplex pairs of lines -> real lines
 // ---------------------------------------------------------------------------
-template <int N, int G>
-__global__ __launch_bounds__(kCtThreads) void kt_xbwd(const double* S, double* P, long nlines,
+template <int N, int G, int NT>
+__global__ __launch_bounds__(NT) void kt_xbwd(const double* S, double* P, long nlines,
                                                       const double2* __restrict__ tw, double scale) {
-    constexpr int NT = kCtThreads;
     constexpr int LP = ct_pitch(N);
     constexpr int H = N / 2 + 1;
     constexpr int NI = (G * H + NT - 1) / NT;
@@ -148,10 +148,9 @@ __global__ __launch_bounds__(kCtThreads) void kt_xbwd(const double* S, double* P
 // ---------------------------------------------------------------------------
 // z forward: TA complex columns (halfcomplex x rows c and nx-c) of one plane
 // ---------------------------------------------------------------------------
-template <int N, int TA>
-__global__ __launch_bounds__(kCtThreads) void kt_zfwd(double* S, int nx, int ncol, int ntc,
+template <int N, int TA, int NT>
+__global__ __launch_bounds__(NT) void kt_zfwd(double* S, int nx, int ncol, int ntc,
                                                       const double2* __restrict__ tw) {
-    constexpr int NT = kCtThreads;
     constexpr int LP = ct_pitch(N);
     constexpr int TOT = 2 * TA * N;
     constexpr int NL = (TOT + NT - 1) / NT;
@@ -206,10 +205,9 @@ __global__ __launch_bounds__(kCtThreads) void kt_zfwd(double* S, int nx, int nco
     }
 }
 
-template <int N, int TA>
-__global__ __launch_bounds__(kCtThreads) void kt_zbwd(double* S, int nx, int ncol, int ntc,
+template <int N, int TA, int NT>
+__global__ __launch_bounds__(NT) void kt_zbwd(double* S, int nx, int ncol, int ntc,
                                                       const double2* __restrict__ tw) {
-    constexpr int NT = kCtThreads;
     constexpr int LP = ct_pitch(N);
     constexpr int H = N / 2 + 1;
     constexpr int NI = (H * TA + NT - 1) / NT;
@@ -276,8 +274,17 @@ __global__ __launch_bounds__(kCtThreads) void kt_zbwd(double* S, int nx, int nco
 // turns row s into  a_s x_{s-1} + (beta_s - c_s Q) x_s - c_s R x_e = d_s - c_s P.
 // The 2P tip rows (s and e of every chunk) form a tridiagonal system per mode,
 // solved by parallel cyclic reduction in LDS; the interior rows follow by the
-// backward recurrence above.
+// backward recurrence above.  Reciprocals are v_rcp_f64 refined by two
+// Newton steps (full double precision for the nonzero pivots met here).
 // ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double rcp64(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    r = fma(r, fma(-b, r, 1.0), r);
+    return r;
+}
+
 template <int MR, int P, int M>
 __global__ __launch_bounds__(M * P) void kt_tri_part(double* S, const double* __restrict__ ax,
                                                      const double* __restrict__ az,
@@ -288,6 +295,8 @@ __global__ __launch_bounds__(M * P) void kt_tri_part(double* S, const double* __
     constexpr int NT = M * P;
     constexpr int NR = 2 * P;  // reduced rows per mode
     __shared__ double ra[NR][M], rb[NR][M], rc[NR][M], rd[NR][M];  // mode fastest
+    extern __shared__ double smb[];                                  // 6*ny band values
+    for (int e = (int)threadIdx.x; e < 6 * ny; e += NT) smb[e] = bands[e];
     const int mi = (int)threadIdx.x % M;
     const int p = (int)threadIdx.x / M;
     const long mode = (long)xcd_tile(blockIdx.x, gridDim.x) * M + mi;
@@ -296,9 +305,6 @@ __global__ __launch_bounds__(M * P) void kt_tri_part(double* S, const double* __
     const int i = (int)(md % nx);
     const int j = (int)(md / nx);
     const double lam = ax[i] + az[j];
-    const double* A = bands + ((is0x[i] && is0z[j]) ? 3 * ny : 0);
-    const double* B = A + ny;
-    const double* C = B + ny;
     const int s = p * MR;
 
     double d[MR];
@@ -307,6 +313,10 @@ __global__ __launch_bounds__(M * P) void kt_tri_part(double* S, const double* __
         const int row = s + r;
         d[r] = (valid && row < ny) ? S[(size_t)row * plane + md] : 0.0;
     }
+    __syncthreads();
+    const double* A = smb + ((is0x[i] && is0z[j]) ? 3 * ny : 0);
+    const double* B = A + ny;
+    const double* C = B + ny;
     auto band_a = [&](int row) { return row < ny ? A[row] : 0.0; };
     auto band_c = [&](int row) { return row < ny ? C[row] : 0.0; };
     auto beta = [&](int row) { return row < ny ? B[row] + lam : 1.0; };
@@ -320,8 +330,7 @@ __global__ __launch_bounds__(M * P) void kt_tri_part(double* S, const double* __
         double cprev = band_c(s + 1);
 #pragma unroll
         for (int l = 2; l < MR; ++l) {
-            // normalise row l-1 (interior) before moving on
-            const double rinv = 1.0 / Bk;
+            const double rinv = rcp64(Bk);
             d[l - 1] = Dk * rinv;
             Ah[l - 1] = Ak * rinv;
             Ch[l - 1] = cprev * rinv;
@@ -355,7 +364,7 @@ __global__ __launch_bounds__(M * P) void kt_tri_part(double* S, const double* __
         rd[2 * p + 1][mi] = De;
     }
     __syncthreads();
-    // parallel cyclic reduction on the NR tip rows of each mode: row q = 2*t0 + h
+    // parallel cyclic reduction on the NR tip rows of each mode
     for (int st = 1; st < NR; st *= 2) {
         double na[2], nb[2], nc[2], nd[2];
 #pragma unroll
@@ -366,13 +375,13 @@ __global__ __launch_bounds__(M * P) void kt_tri_part(double* S, const double* __
             const double aq = ra[q][mm], bq = rb[q][mm], cq = rc[q][mm], dq = rd[q][mm];
             double a2 = 0.0, b2 = bq, c2 = 0.0, d2 = dq;
             if (q - st >= 0) {
-                const double al = -aq / rb[q - st][mm];
+                const double al = -aq * rcp64(rb[q - st][mm]);
                 a2 = al * ra[q - st][mm];
                 b2 += al * rc[q - st][mm];
                 d2 += al * rd[q - st][mm];
             }
             if (q + st < NR) {
-                const double ga = -cq / rb[q + st][mm];
+                const double ga = -cq * rcp64(rb[q + st][mm]);
                 c2 = ga * rc[q + st][mm];
                 b2 += ga * ra[q + st][mm];
                 d2 += ga * rd[q + st][mm];
@@ -389,8 +398,8 @@ __global__ __launch_bounds__(M * P) void kt_tri_part(double* S, const double* __
         }
         __syncthreads();
     }
-    const double xs = rd[2 * p][mi] / rb[2 * p][mi];
-    const double xe = rd[2 * p + 1][mi] / rb[2 * p + 1][mi];
+    const double xs = rd[2 * p][mi] * rcp64(rb[2 * p][mi]);
+    const double xe = rd[2 * p + 1][mi] * rcp64(rb[2 * p + 1][mi]);
     d[0] = xs;
     double xn = xe;
 #pragma unroll

@@ -62,6 +62,9 @@ _SIGNATURES = [
     ("dnsp_poisson_stage", [_vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     ("dnsp_poisson_set_strategy", [_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     ("dnsp_poisson_get_strategy", [_vp, _ip, _ip, _ip], ctypes.c_int),
+    ("dnsp_poisson_set_variants", [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    ("dnsp_poisson_get_variants", [_vp, _ip, _ip, _ip], ctypes.c_int),
+    ("dnsp_poisson_autotune", [_vp, _vp, ctypes.c_int, _vp, _dp], ctypes.c_int),
     ("dnsp_poisson_min_pivot", [_vp, _dp], ctypes.c_int),
     ("dnsp_poisson_shape", [_vp, _ip, _ip, _ip], ctypes.c_int),
     ("dnsp_poisson_free", [_vp], ctypes.c_int),
@@ -178,6 +181,24 @@ class PoissonSolver:
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(lib().dnsp_poisson_get_strategy(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return {"fft_ct_x": a.value, "fft_ct_z": b.value, "part_rows": c.value}
+
+    def set_variants(self, xvar, zvar, tvar=0):
+        _check(lib().dnsp_poisson_set_variants(self._h, int(xvar), int(zvar), int(tvar)))
+
+    @property
+    def variants(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().dnsp_poisson_get_variants(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return (a.value, b.value, c.value)
+
+    def autotune(self, work=None, reps=5, stream=None):
+        """Picks the fastest kernel variants; returns the timing table (ms per pair)."""
+        times = np.zeros(10)
+        wp = _vp(_device_ptr(work, self.numel)) if work is not None else _vp(None)
+        _check(lib().dnsp_poisson_autotune(self._h, wp, int(reps), _vp(_stream_ptr(stream)),
+                                           times.ctypes.data_as(_dp)))
+        return {"x_ms": times[:4].tolist(), "z_ms": times[4:8].tolist(), "tri_ms": times[8:].tolist(),
+                "picked": self.variants}
 
     @property
     def min_pivot(self):

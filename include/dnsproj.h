@@ -115,6 +115,23 @@ int dnsp_poisson_set_strategy(dnsp_poisson* h, int fft, int tridiag);
  * the partitioned tridiagonal rows per chunk (0 = serial Thomas). */
 int dnsp_poisson_get_strategy(const dnsp_poisson* h, int* fft_ct_x, int* fft_ct_z, int* part_rows);
 
+/* Kernel variants.  xvar, zvar (0..3): the compile-time-sized FFT kernels'
+ * (complex elements per workgroup tile, threads) = (3072,256), (3072,512),
+ * (1536,256), (6144,512); ignored where a length has no such kernels.
+ * tvar (0..1): partitioned tridiagonal workgroups of 16 or 8 modes (x 32
+ * chunks).  No counterpart in the reference. */
+int dnsp_poisson_set_variants(dnsp_poisson* h, int xvar, int zvar, int tvar);
+int dnsp_poisson_get_variants(const dnsp_poisson* h, int* xvar, int* zvar, int* tvar);
+
+/* Autotuner: times `reps` forward+inverse pairs of every FFT kernel variant
+ * of each direction, and pairs of tridiagonal solves of every partitioned
+ * variant, on `work` (nx*nz*ny device doubles, overwritten; NULL = a
+ * temporary allocation) and keeps the fastest in the handle.  times_ms
+ * (optional, 10 doubles) receives the mean ms per pair: 4 x variants, 4 z
+ * variants, 2 tridiagonal variants; -1 where not applicable.  Synchronises
+ * `stream`. */
+int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream, double* times_ms);
+
 /* min |pivot| over all modes and rows: the init diagnostic min_abs_div_bacp
  * (poisson_solver_multigpu.f90:181-222), computed on the device at init. */
 int dnsp_poisson_min_pivot(const dnsp_poisson* h, double* out);

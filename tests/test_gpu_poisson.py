@@ -163,6 +163,28 @@ def test_every_strategy_matches_oracle(cuda, nx, nz, ny):
     s.close()
 
 
+@pytest.mark.parametrize("nx,nz,ny", [(768, 384, 24), (96, 768, 40), (512, 64, 130), (32, 192, 7)])
+def test_every_kernel_variant_matches_oracle(cuda, nx, nz, ny):
+    say("variants", nx, nz, ny)
+    c = case_grid(nx, nz, ny, nx + 3 * ny)
+    s, bands = make_solver(c)
+    ref = O.solve(c["rhs"], c["inv_dx"], c["inv_dz"], bands)
+    for v in range(4):
+        for tv in range(2):
+            s.set_variants(v, (v + 1) % 4, tv)
+            t = to_dev(c["rhs"])
+            s.run(t)
+            assert rel_mean_removed(to_host(t), ref) <= 1e-12, (v, tv)
+    say("autotune", nx, nz, ny)
+    table = s.autotune(reps=2)
+    assert table["picked"] == s.variants
+    assert all(v > 0 for v in table["x_ms"] + table["z_ms"])
+    t = to_dev(c["rhs"])
+    s.run(t)
+    assert rel_mean_removed(to_host(t), ref) <= 1e-12
+    s.close()
+
+
 def test_strategy_reporting(cuda):
     say("strategy report")
     c = W.config2_grid()
