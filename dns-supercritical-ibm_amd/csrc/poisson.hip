@@ -353,7 +353,8 @@ constexpr int kPartVariants = 2;
 
 template <int M>
 static void part_launch_m(int mr, double* S, const double* ax, const double* az, const int* i0x,
-                          const int* i0z, const double* bands, int nx, int nz, int ny, hipStream_t st) {
+                          const int* i0z, const double* bands, int nx, int nz, int ny, double pcr_min,
+                          hipStream_t st) {
     const long plane = (long)nx * nz;
     const int nb = (int)((plane + M - 1) / M);
     const size_t lds = 6 * sizeof(double) * ny;
@@ -361,7 +362,7 @@ static void part_launch_m(int mr, double* S, const double* ax, const double* az,
 #define DNSP_CASE(MR)                                                                                  \
     case MR:                                                                                           \
         kt_tri_part<MR, kPartChunks, M><<<nb, kPartChunks * M, lds, st>>>(S, ax, az, i0x, i0z, bands, \
-                                                                         nx, ny, plane);               \
+                                                                         nx, ny, plane, pcr_min);      \
         break;
         DNSP_PART_ROWS(DNSP_CASE)
 #undef DNSP_CASE
@@ -370,9 +371,10 @@ static void part_launch_m(int mr, double* S, const double* ax, const double* az,
 }
 
 static void part_launch(int variant, int mr, double* S, const double* ax, const double* az, const int* i0x,
-                        const int* i0z, const double* bands, int nx, int nz, int ny, hipStream_t st) {
-    if (variant == 1) part_launch_m<8>(mr, S, ax, az, i0x, i0z, bands, nx, nz, ny, st);
-    else part_launch_m<16>(mr, S, ax, az, i0x, i0z, bands, nx, nz, ny, st);
+                        const int* i0z, const double* bands, int nx, int nz, int ny, double pcr_min,
+                        hipStream_t st) {
+    if (variant == 1) part_launch_m<8>(mr, S, ax, az, i0x, i0z, bands, nx, nz, ny, pcr_min, st);
+    else part_launch_m<16>(mr, S, ax, az, i0x, i0z, bands, nx, nz, ny, pcr_min, st);
 }
 
 }  // namespace dnsp
@@ -402,6 +404,8 @@ struct dnsp_poisson {
     int part_rows = 0;                // partitioned tridiagonal rows per chunk (0 = serial)
     double* d_host = nullptr;  // staging for dnsp_poisson_run_host
     double min_pivot = 0;
+    // modes with |lambda| below this solve their reduced system serially
+    double pcr_min = 0;
 };
 
 static void poisson_release(dnsp_poisson* h) {
@@ -452,6 +456,13 @@ static int poisson_setup(dnsp_poisson* h, const double* const bands[6]) {
 
     std::vector<double> hb((size_t)6 * ny);
     for (int s = 0; s < 6; ++s) std::memcpy(hb.data() + (size_t)s * ny, bands[s], sizeof(double) * ny);
+    // weakest coupling a_k + c_k of the plain bands (interior rows when there are any)
+    {
+        double mn = 1e300;
+        const int k0 = ny > 2 ? 1 : 0, k1 = ny > 2 ? ny - 1 : ny;
+        for (int k = k0; k < k1; ++k) mn = std::min(mn, std::fabs(bands[0][k]) + std::fabs(bands[2][k]));
+        h->pcr_min = mn;
+    }
     DNSP_HIP_TRY(hipMalloc(&h->d_bands, sizeof(double) * 6 * ny));
     DNSP_HIP_TRY(hipMemcpy(h->d_bands, hb.data(), sizeof(double) * 6 * ny, hipMemcpyHostToDevice));
 
@@ -701,7 +712,7 @@ int dnsp_poisson_stage(dnsp_poisson* h, const double* src, double* P, int stage,
         case DNSP_STAGE_TRIDIAG: {
             if (h->part_rows > 0) {
                 part_launch(h->tvar, h->part_rows, P, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands, h->nx, h->nz,
-                            h->ny, st);
+                            h->ny, h->pcr_min, st);
             } else {
                 const long plane = (long)h->nx * h->nz;
                 const int nb = (int)((plane + 255) / 256);

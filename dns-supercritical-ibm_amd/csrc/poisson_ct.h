@@ -272,10 +272,14 @@ __global__ __launch_bounds__(NT) void kt_zbwd(double* S, int nx, int ncol, int n
 // Interior rows then read x_k = D^_k - A^_k x_s - C^_k x_{k+1} (hat = divided
 // by B_k).  An upward pass folds them into x_{s+1} = P - Q x_s - R x_e, which
 // turns row s into  a_s x_{s-1} + (beta_s - c_s Q) x_s - c_s R x_e = d_s - c_s P.
-// The 2P tip rows (s and e of every chunk) form a tridiagonal system per mode,
-// solved by parallel cyclic reduction in LDS; the interior rows follow by the
-// backward recurrence above.  Reciprocals are v_rcp_f64 refined by two
-// Newton steps (full double precision for the nonzero pivots met here).
+// The 2P tip rows (s and e of every chunk) form a tridiagonal system per mode.
+// A workgroup whose modes all have |lambda| >= pcr_min solves it by parallel
+// cyclic reduction in LDS.  Cyclic reduction loses digits by cancellation on
+// the nearly singular low-wavenumber modes, so a workgroup holding any mode
+// with |lambda| < pcr_min (the (0,0) mode among them) solves it by a serial
+// Thomas sweep per mode instead.  The interior rows follow by the backward
+// recurrence above.  Reciprocals in the chunk sweeps are v_rcp_f64 refined by
+// two Newton steps (full double precision for the nonzero pivots met here).
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ double rcp64(double b) {
@@ -291,7 +295,7 @@ __global__ __launch_bounds__(M * P) void kt_tri_part(double* S, const double* __
                                                      const int* __restrict__ is0x,
                                                      const int* __restrict__ is0z,
                                                      const double* __restrict__ bands, int nx, int ny,
-                                                     long plane) {
+                                                     long plane, double pcr_min) {
     constexpr int NT = M * P;
     constexpr int NR = 2 * P;  // reduced rows per mode
     __shared__ double ra[NR][M], rb[NR][M], rc[NR][M], rd[NR][M];  // mode fastest
@@ -363,9 +367,28 @@ __global__ __launch_bounds__(M * P) void kt_tri_part(double* S, const double* __
         rc[2 * p + 1][mi] = band_c(s + MR - 1);
         rd[2 * p + 1][mi] = De;
     }
-    __syncthreads();
+    const bool serial = __syncthreads_or(valid && fabs(lam) < pcr_min) != 0;
+    if (serial) {
+        // one thread per mode: Thomas on the NR tip rows; x overwrites rd, rb = 1
+        if (p == 0) {
+            for (int q = 1; q < NR; ++q) {
+                const double m = ra[q][mi] / rb[q - 1][mi];
+                rb[q][mi] = rb[q][mi] - m * rc[q - 1][mi];
+                rd[q][mi] = rd[q][mi] - m * rd[q - 1][mi];
+            }
+            double xq = rd[NR - 1][mi] / rb[NR - 1][mi];
+            rd[NR - 1][mi] = xq;
+            rb[NR - 1][mi] = 1.0;
+            for (int q = NR - 2; q >= 0; --q) {
+                xq = (rd[q][mi] - rc[q][mi] * xq) / rb[q][mi];
+                rd[q][mi] = xq;
+                rb[q][mi] = 1.0;
+            }
+        }
+        __syncthreads();
+    }
     // parallel cyclic reduction on the NR tip rows of each mode
-    for (int st = 1; st < NR; st *= 2) {
+    for (int st = 1; !serial && st < NR; st *= 2) {
         double na[2], nb[2], nc[2], nd[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -398,8 +421,8 @@ __global__ __launch_bounds__(M * P) void kt_tri_part(double* S, const double* __
         }
         __syncthreads();
     }
-    const double xs = rd[2 * p][mi] * rcp64(rb[2 * p][mi]);
-    const double xe = rd[2 * p + 1][mi] * rcp64(rb[2 * p + 1][mi]);
+    const double xs = serial ? rd[2 * p][mi] : rd[2 * p][mi] * rcp64(rb[2 * p][mi]);
+    const double xe = serial ? rd[2 * p + 1][mi] : rd[2 * p + 1][mi] * rcp64(rb[2 * p + 1][mi]);
     d[0] = xs;
     double xn = xe;
 #pragma unroll
