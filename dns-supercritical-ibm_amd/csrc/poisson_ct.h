@@ -25,12 +25,6 @@ constexpr int ct_cols() {  // complex columns per z workgroup, at most 16
     return (TILE / N) > 16 ? 16 : ((TILE / N) > 0 ? (TILE / N) : 1);
 }
 
-__device__ __forceinline__ int zrow(int c, int half, int nx) {
-    if (half == 0) return c;
-    if (c == 0) return (nx % 2 == 0 && nx > 1) ? nx / 2 : -1;
-    return nx - c;
-}
-
 // ---------------------------------------------------------------------------
 // x forward: 2G real lines (one contiguous block of memory) per workgroup
 // ---------------------------------------------------------------------------
@@ -146,39 +140,41 @@ __global__ __launch_bounds__(NT) void kt_xbwd(const double* S, double* P, long n
 }
 
 // ---------------------------------------------------------------------------
-// z forward: TA complex columns (halfcomplex x rows c and nx-c) of one plane
+// z: TA complex columns of one plane per workgroup.  Complex column c pairs
+// the adjacent halfcomplex x rows 2c (real part) and 2c+1 (imaginary part),
+// so a tile is 2*TA contiguous doubles of every z row; for odd nx the last
+// row is paired with zero.  (Any two real rows can share one complex
+// transform; the runtime-sized kernels pair rows c and nx-c instead.)
 // ---------------------------------------------------------------------------
 template <int N, int TA, int NT>
 __global__ __launch_bounds__(NT) void kt_zfwd(double* S, int nx, int ncol, int ntc,
-                                                      const double2* __restrict__ tw) {
+                                              const double2* __restrict__ tw) {
     constexpr int LP = ct_pitch(N);
-    constexpr int TOT = 2 * TA * N;
+    constexpr int W = 2 * TA;  // real rows per tile
+    constexpr int TOT = W * N;
     constexpr int NL = (TOT + NT - 1) / NT;
     __shared__ double2 sm[TA * LP];
     double* smd = reinterpret_cast<double*>(sm);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int k = tile / ntc;
-    const int c0 = (tile - k * ntc) * TA;
+    const int r0 = (tile - k * ntc) * W;  // first real row of the tile
     double* Sk = S + (size_t)k * nx * N;
     double v[NL];
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
         const int e = (int)threadIdx.x + q * NT;
-        const int tt = e % TA;
-        const int rest = e / TA;
-        const int hf = rest & 1;
-        const int j = rest >> 1;
-        const int c = c0 + tt;
-        const int row = (c < ncol && (TOT % NT == 0 || e < TOT)) ? zrow(c, hf, nx) : -1;
-        v[q] = (row >= 0) ? Sk[(size_t)j * nx + row] : 0.0;
+        const int r = e % W;
+        const int j = e / W;
+        const bool ok = (TOT % NT == 0 || e < TOT) && r0 + r < nx;
+        v[q] = ok ? Sk[(size_t)j * nx + r0 + r] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
         const int e = (int)threadIdx.x + q * NT;
         if (TOT % NT == 0 || e < TOT) {
-            const int tt = e % TA;
-            const int rest = e / TA;
-            smd[(tt * LP + ct_pad(rest >> 1)) * 2 + (rest & 1)] = v[q];
+            const int r = e % W;
+            const int j = e / W;
+            smd[((r >> 1) * LP + ct_pad(j)) * 2 + (r & 1)] = v[q];
         }
     }
     __syncthreads();
@@ -186,37 +182,35 @@ __global__ __launch_bounds__(NT) void kt_zfwd(double* S, int nx, int ncol, int n
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
         const int e = (int)threadIdx.x + q * NT;
-        const int tt = e % TA;
-        const int rest = e / TA;
-        const int hf = rest & 1;
-        const int b = rest >> 1;
-        const int c = c0 + tt;
-        const int row = (c < ncol && (TOT % NT == 0 || e < TOT)) ? zrow(c, hf, nx) : -1;
-        if (row >= 0) {
+        const int r = e % W;
+        const int b = e / W;
+        if ((TOT % NT == 0 || e < TOT) && r0 + r < nx) {
+            const int tt = r >> 1;
             const bool lo = 2 * b <= N;
             const int qq = lo ? b : N - b;
             const double2 Z = sm[tt * LP + ct_pad(qq)];
             const double2 Zm = sm[tt * LP + ct_pad(qq == 0 ? 0 : N - qq)];
             double val;
-            if (hf == 0) val = lo ? 0.5 * (Z.x + Zm.x) : 0.5 * (Z.y - Zm.y);
-            else         val = lo ? 0.5 * (Z.y + Zm.y) : 0.5 * (Zm.x - Z.x);
-            Sk[(size_t)b * nx + row] = val;
+            if ((r & 1) == 0) val = lo ? 0.5 * (Z.x + Zm.x) : 0.5 * (Z.y - Zm.y);
+            else              val = lo ? 0.5 * (Z.y + Zm.y) : 0.5 * (Zm.x - Z.x);
+            Sk[(size_t)b * nx + r0 + r] = val;
         }
     }
 }
 
 template <int N, int TA, int NT>
 __global__ __launch_bounds__(NT) void kt_zbwd(double* S, int nx, int ncol, int ntc,
-                                                      const double2* __restrict__ tw) {
+                                              const double2* __restrict__ tw) {
     constexpr int LP = ct_pitch(N);
+    constexpr int W = 2 * TA;
     constexpr int H = N / 2 + 1;
     constexpr int NI = (H * TA + NT - 1) / NT;
-    constexpr int TOT = 2 * TA * N;
+    constexpr int TOT = W * N;
     constexpr int NL = (TOT + NT - 1) / NT;
     __shared__ double2 sm[TA * LP];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int k = tile / ntc;
-    const int c0 = (tile - k * ntc) * TA;
+    const int r0 = (tile - k * ntc) * W;
     double* Sk = S + (size_t)k * nx * N;
     double ur[NI], ui[NI], vr[NI], vi[NI];
 #pragma unroll
@@ -224,15 +218,14 @@ __global__ __launch_bounds__(NT) void kt_zbwd(double* S, int nx, int ncol, int n
         const int e = (int)threadIdx.x + q * NT;
         const int tt = e % TA;
         const int qq = e / TA;
-        const int c = c0 + tt;
-        const bool ok = (e < H * TA) && c < ncol;
-        const int ru = ok ? zrow(c, 0, nx) : -1;
-        const int rv = ok ? zrow(c, 1, nx) : -1;
+        const bool ok = e < H * TA;
+        const int ru = r0 + 2 * tt, rv = ru + 1;
         const bool single = (qq == 0) || (2 * qq == N);
-        ur[q] = (ru >= 0) ? Sk[(size_t)qq * nx + ru] : 0.0;
-        ui[q] = (ru >= 0 && !single) ? Sk[(size_t)(N - qq) * nx + ru] : 0.0;
-        vr[q] = (rv >= 0) ? Sk[(size_t)qq * nx + rv] : 0.0;
-        vi[q] = (rv >= 0 && !single) ? Sk[(size_t)(N - qq) * nx + rv] : 0.0;
+        const bool hu = ok && ru < nx, hv = ok && rv < nx;
+        ur[q] = hu ? Sk[(size_t)qq * nx + ru] : 0.0;
+        vr[q] = hv ? Sk[(size_t)qq * nx + rv] : 0.0;
+        ui[q] = (hu && !single) ? Sk[(size_t)(N - qq) * nx + ru] : 0.0;
+        vi[q] = (hv && !single) ? Sk[(size_t)(N - qq) * nx + rv] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
@@ -251,13 +244,10 @@ __global__ __launch_bounds__(NT) void kt_zbwd(double* S, int nx, int ncol, int n
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
         const int e = (int)threadIdx.x + q * NT;
-        const int tt = e % TA;
-        const int rest = e / TA;
-        const int hf = rest & 1;
-        const int j = rest >> 1;
-        const int c = c0 + tt;
-        const int row = (c < ncol && (TOT % NT == 0 || e < TOT)) ? zrow(c, hf, nx) : -1;
-        if (row >= 0) Sk[(size_t)j * nx + row] = smd[(tt * LP + ct_pad(j)) * 2 + hf];
+        const int r = e % W;
+        const int j = e / W;
+        if ((TOT % NT == 0 || e < TOT) && r0 + r < nx)
+            Sk[(size_t)j * nx + r0 + r] = smd[((r >> 1) * LP + ct_pad(j)) * 2 + (r & 1)];
     }
 }
 

@@ -10,7 +10,7 @@ template <int N, int V>
 static void ct_z(bool inv, double* S, int nx, int ny, const double2* tw, hipStream_t st) {
     constexpr int TA = ct_cols<N, ct_var_tile(V)>();
     constexpr int NT = ct_var_threads(V);
-    const int ncol = (nx + 1) / 2;
+    const int ncol = (nx + 1) / 2;  // complex columns = pairs of adjacent x rows
     const int ntc = (ncol + TA - 1) / TA;
     const int nb = ntc * ny;
     if (!inv) kt_zfwd<N, TA, NT><<<nb, NT, 0, st>>>(S, nx, ncol, ntc, tw);

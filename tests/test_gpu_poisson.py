@@ -59,7 +59,8 @@ def case_grid(nx, nz, ny, seed):
 
 STAGE_CASES = [(64, 32, 64), (5, 7, 3), (6, 10, 9), (1, 1, 2), (2, 3, 4), (15, 14, 5), (768, 6, 3),
                (12, 384, 3), (96, 96, 17), (35, 49, 6), (32, 768, 5), (768, 32, 7), (512, 96, 4),
-               (192, 256, 3), (128, 64, 70), (384, 128, 40)]
+               (192, 256, 3), (128, 64, 70), (384, 128, 40), (7, 64, 5), (1, 96, 3), (3, 384, 4),
+               (45, 128, 6), (2, 32, 3)]
 
 
 @pytest.mark.parametrize("nx,nz,ny", STAGE_CASES)
