@@ -31,6 +31,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_fp64_768x384x384.json"),
                     help="rocprofv3 FETCH/WRITE summary to report as roofline.traffic")
+    ap.add_argument("--mode", default="bench", choices=["bench", "counters"],
+                    help="counters: one 906 MB device copy (calibration) then 3 solves, for rocprofv3 --pmc")
     return ap.parse_args()
 
 
@@ -57,6 +59,17 @@ def main():
     stream = torch.cuda.current_stream()
 
     tune = solver.autotune(work=P, reps=5)
+    if args.mode == "counters":
+        # a plain copy of known size (N doubles read, N written) for scaling the counters
+        torch.cuda.synchronize()
+        P.copy_(f)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            solver.solve(f, P)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "counters", "copy_bytes_each_way": 8 * N, "variants": tune["picked"]}),
+              flush=True)
+        return
     for _ in range(args.warmup):
         solver.solve(f, P)
     torch.cuda.synchronize()
