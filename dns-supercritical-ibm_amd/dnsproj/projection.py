@@ -1,0 +1,224 @@
+"""Projection stencils and IBM forcing over libdnsproj.so.
+
+Names follow the reference (src/mod_cfd_subroutines.f90, src/mod_utils.f90,
+src/diezdecomp_api_ibm.f90, src/ibm_setup/).  Halo arrays are torch float64
+tensors of shape (ny+2, nz+2, nx+2) on a HIP device (the reference's
+(-1:nx, -1:nz, -1:ny) fields); P is (ny, nz, nx).  1-D coefficients, rho_min
+and wall values are device tensors too.
+"""
+import ctypes
+
+import numpy as np
+
+from . import DnspError, _check, _dp, _ip, _stream_ptr, _vp, lib
+
+_i, _d = ctypes.c_int, ctypes.c_double
+
+_SIGS = [
+    ("dnsp_halo_periodic_xz", [_vp, _i, _i, _i, _vp]),
+    ("dnsp_set_val_slice", [_vp, _i, _i, _i, _i, _i, _d, _vp]),
+    ("dnsp_neumann_ghosts_y", [_vp, _i, _i, _i, _vp]),
+    ("dnsp_build_p_rhs", [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d, _d, _d, _vp]),
+    ("dnsp_build_p_rhs_varprop", [_vp] * 11 + [_i, _i, _i] + [_d] * 5 + [_vp]),
+    ("dnsp_rho_min", [_vp, _i, _i, _i, _vp, _vp]),
+    ("dnsp_update_vars", [_vp] * 15 + [_i, _i, _i, _i, _d, _d, _d, _i, _vp]),
+    ("dnsp_geom_coeffs", [_i, _dp, _dp, _dp, _dp, _dp, _dp]),
+    ("dnsp_ibm_create", [_i, _i, _i, _i, _i, _i, _ip, _dp, ctypes.POINTER(_vp)]),
+    ("dnsp_ibm_exec", [_vp, _vp, _vp]),
+    ("dnsp_ibm_info", [_vp, _ip, _ip, _ip]),
+    ("dnsp_ibm_free", [_vp]),
+    ("dnsp_ibm_read_coeffs", [ctypes.c_char_p, _i, _i, _i, _ip, _ip, _dp]),
+    ("dnsp_ibm_div_coeffs", [_i, _ip, _dp, _i, _i, _i, _d, _d, _dp]),
+    ("dnsp_ibm_read_slab", [ctypes.c_char_p, _i, _i, _i, _i, _i, _ip]),
+    ("dnsp_ibm_part2", [_vp, _vp, _vp, _i, _i, _i, _i, _d, _d, _vp]),
+]
+
+_ready = False
+
+
+def _lib():
+    global _ready
+    L = lib()
+    if not _ready:
+        for name, args in _SIGS:
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        _ready = True
+    return L
+
+
+def exported_names():
+    return [s[0] for s in _SIGS]
+
+
+def _dev(t, shape=None):
+    import torch
+    if t is None:
+        return _vp(None)
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
+        raise ValueError("expected a contiguous float64 tensor on a HIP device")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"expected shape {shape}, got {tuple(t.shape)}")
+    return _vp(t.data_ptr())
+
+
+def _idev(t, n):
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.int32 or t.numel() != n:
+        raise ValueError("expected a contiguous int32 device tensor of the right size")
+    return _vp(t.data_ptr())
+
+
+def _halo_shape(A):
+    ny2, nz2, nx2 = A.shape
+    return nx2 - 2, nz2 - 2, ny2 - 2
+
+
+def halo_periodic_xz(A, stream=None):
+    nx, nz, ny = _halo_shape(A)
+    _check(_lib().dnsp_halo_periodic_xz(_dev(A), nx, nz, ny, _vp(_stream_ptr(stream))))
+    return A
+
+
+def set_val_slice(A, k0, k1, val, stream=None):
+    nx, nz, ny = _halo_shape(A)
+    _check(_lib().dnsp_set_val_slice(_dev(A), nx, nz, ny, int(k0), int(k1), float(val), _vp(_stream_ptr(stream))))
+    return A
+
+
+def neumann_ghosts_y(A, stream=None):
+    nx, nz, ny = _halo_shape(A)
+    _check(_lib().dnsp_neumann_ghosts_y(_dev(A), nx, nz, ny, _vp(_stream_ptr(stream))))
+    return A
+
+
+def build_p_rhs(P, U_hat, W_hat, V_hat, inv_dy_p, inv_dtmax, inv_dx, inv_dz, stream=None):
+    ny, nz, nx = P.shape
+    hs = (ny + 2, nz + 2, nx + 2)
+    _check(_lib().dnsp_build_p_rhs(_dev(P), _dev(U_hat, hs), _dev(W_hat, hs), _dev(V_hat, hs), _dev(inv_dy_p, (ny,)),
+                                   nx, nz, ny, float(inv_dtmax), float(inv_dx), float(inv_dz),
+                                   _vp(_stream_ptr(stream))))
+    return P
+
+
+def build_p_rhs_varprop(P, U_hat, W_hat, V_hat, P_now, P_old, rho, div_now, inv_dy_p, inv_dy_p_pad, rho_min,
+                        inv_dtmax, inv_dx, inv_dz, inv_dx2, inv_dz2, stream=None):
+    ny, nz, nx = P.shape
+    hs = (ny + 2, nz + 2, nx + 2)
+    _check(_lib().dnsp_build_p_rhs_varprop(
+        _dev(P), _dev(U_hat, hs), _dev(W_hat, hs), _dev(V_hat, hs), _dev(P_now, hs), _dev(P_old, hs), _dev(rho, hs),
+        _dev(div_now, hs), _dev(inv_dy_p, (ny,)), _dev(inv_dy_p_pad, (ny + 1,)), _dev(rho_min, (1,)), nx, nz, ny,
+        float(inv_dtmax), float(inv_dx), float(inv_dz), float(inv_dx2), float(inv_dz2), _vp(_stream_ptr(stream))))
+    return P
+
+
+def rho_min(rho, out, stream=None):
+    nx, nz, ny = _halo_shape(rho)
+    _check(_lib().dnsp_rho_min(_dev(rho), nx, nz, ny, _dev(out, (1,)), _vp(_stream_ptr(stream))))
+    return out
+
+
+def update_vars(P, P_per, P_now, P_old, U, V, W, U_hat, V_hat, W_hat, rho, dPdy_ddy_hi, dPdy_ddy_lo, rho_min,
+                wall_bc, kmin_Vhat, dtmax, inv_dx, inv_dz, use_incompressible=True, stream=None):
+    ny, nz, nx = P.shape
+    hs = (ny + 2, nz + 2, nx + 2)
+    _check(_lib().dnsp_update_vars(
+        _dev(P, (ny, nz, nx)), _dev(P_per, hs), _dev(P_now, hs), _dev(P_old, hs), _dev(U, hs), _dev(V, hs),
+        _dev(W, hs), _dev(U_hat, hs), _dev(V_hat, hs), _dev(W_hat, hs), _dev(rho, hs), _dev(dPdy_ddy_hi, (ny,)),
+        _dev(dPdy_ddy_lo, (ny,)), _dev(rho_min, (1,)), _dev(wall_bc, (6,)), nx, nz, ny, int(kmin_Vhat),
+        float(dtmax), float(inv_dx), float(inv_dz), int(bool(use_incompressible)), _vp(_stream_ptr(stream))))
+
+
+def geom_coeffs(yu):
+    """Host: inv_dy_p, inv_dy_u, inv_dy_p_pad, dPdy_ddy_lo, dPdy_ddy_hi for faces yu."""
+    yu = np.ascontiguousarray(yu, dtype=np.float64)
+    ny = yu.size - 1
+    out = dict(inv_dy_p=np.empty(ny), inv_dy_u=np.empty(ny + 1), inv_dy_p_pad=np.empty(ny + 1),
+               dPdy_ddy_lo=np.empty(ny), dPdy_ddy_hi=np.empty(ny))
+    _check(_lib().dnsp_geom_coeffs(ny, yu.ctypes.data_as(_dp), *[out[k].ctypes.data_as(_dp) for k in
+                                                                  ("inv_dy_p", "inv_dy_u", "inv_dy_p_pad",
+                                                                   "dPdy_ddy_lo", "dPdy_ddy_hi")]))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# IBM
+# ---------------------------------------------------------------------------
+class IbmForcing:
+    """diezDecomp_ibm_type on one GPU: create once, exec on a padded field."""
+
+    def __init__(self, nx, nz, ny, nhalo, nskip, ijk, c):
+        ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 9)
+        c = np.ascontiguousarray(c, dtype=np.float64).reshape(-1, 3)
+        if ijk.shape[0] != c.shape[0]:
+            raise ValueError("ijk and c must have the same number of records")
+        h = _vp()
+        _check(_lib().dnsp_ibm_create(int(nx), int(nz), int(ny), int(nhalo), int(nskip), ijk.shape[0],
+                                      ijk.ctypes.data_as(_ip), c.ctypes.data_as(_dp), ctypes.byref(h)))
+        self._h = h
+        self.nx, self.nz, self.ny, self.nside = int(nx), int(nz), int(ny), int(nhalo) + int(nskip)
+
+    @property
+    def padded_shape(self):
+        s = 2 * self.nside
+        return (self.ny + s, self.nz + s, self.nx + s)
+
+    @property
+    def info(self):
+        a, b, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(_lib().dnsp_ibm_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(d)))
+        return {"n_forcings": a.value, "n_images": b.value, "disjoint": bool(d.value)}
+
+    def exec(self, A, stream=None):
+        _check(_lib().dnsp_ibm_exec(self._h, _dev(A, self.padded_shape), _vp(_stream_ptr(stream))))
+        return A
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            _lib().dnsp_ibm_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ibm_read_coeffs(path, nx, nz, ny):
+    """Returns ijk (n, 9) int32 and c (n, 3) = (cw, c1, c2)."""
+    n = ctypes.c_int()
+    p = str(path).encode()
+    _check(_lib().dnsp_ibm_read_coeffs(p, nx, nz, ny, ctypes.byref(n), None, None))
+    ijk = np.zeros((n.value, 9), dtype=np.int32)
+    c = np.zeros((n.value, 3))
+    _check(_lib().dnsp_ibm_read_coeffs(p, nx, nz, ny, ctypes.byref(n), ijk.ctypes.data_as(_ip),
+                                       c.ctypes.data_as(_dp)))
+    return ijk, c
+
+
+def ibm_div_coeffs(ijk, yp, nx, nz, inv_dx, inv_dz):
+    ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 9)
+    yp = np.ascontiguousarray(yp, dtype=np.float64)
+    c = np.zeros((ijk.shape[0], 3))
+    _check(_lib().dnsp_ibm_div_coeffs(ijk.shape[0], ijk.ctypes.data_as(_ip), yp.ctypes.data_as(_dp), int(nx),
+                                      int(nz), yp.size, float(inv_dx), float(inv_dz), c.ctypes.data_as(_dp)))
+    return c
+
+
+def ibm_read_slab(path, nx, nz, ny, k_first=0, is_top=False):
+    out = np.zeros((nz, nx), dtype=np.int32)
+    _check(_lib().dnsp_ibm_read_slab(str(path).encode(), int(nx), int(nz), int(ny), int(k_first), int(bool(is_top)),
+                                     out.ctypes.data_as(_ip)))
+    return out
+
+
+def ibm_part2(A, bot, top, kmax, wall_bot, wall_top, stream=None):
+    nx, nz, ny = _halo_shape(A)
+    _check(_lib().dnsp_ibm_part2(_dev(A), _idev(bot, nx * nz), _idev(top, nx * nz), nx, nz, ny, int(kmax),
+                                 float(wall_bot), float(wall_top), _vp(_stream_ptr(stream))))
+    return A
+
+
+__all__ = ["DnspError"] + [n for n in dir() if not n.startswith("_")]

@@ -151,6 +151,117 @@ int dnsp_get_lo_n_bounds_1d(int mpi_divs, int n_glob, int* lo_blocks_mpi, int* n
  * is_dirich_first = 0 gives P_band_abc, 1 gives P_band_abc_00. */
 int dnsp_mk_P_abc(int ny, const double* yu, int is_dirich_first, double* a, double* b, double* c);
 
+
+/* ------------------------------------------------------------------------ */
+/* Projection stencils (src/mod_cfd_subroutines.f90).  "Halo arrays" are the  */
+/* reference's (-1:nx, -1:nz, -1:ny) fields: (nx+2)*(nz+2)*(ny+2) doubles, x  */
+/* fastest; P is (0:nx-1, 0:nz-1, 0:ny-1).  1-D coefficient arrays, rho_min  */
+/* and wall values passed by pointer are device arrays.  All entry points    */
+/* enqueue on `stream` and return without synchronising.                     */
+/* ------------------------------------------------------------------------ */
+
+/* One-rank effect of diezdecomp_halos_execute_generic with the x and z
+ * periodic halo objects (inc_set_alloc_arrays.f90:338-356): periodic wrap of
+ * the x and z halo cells over every padded plane.  The y object is not
+ * periodic and changes nothing on one rank. */
+int dnsp_halo_periodic_xz(double* A, int nx, int nz, int ny, void* stream);
+
+/* set_val_slice (src/mod_utils.f90:38-61): planes k0..k1 of the 0-based
+ * padded view (0 = the k = -1 ghost; negative values count from the end)
+ * set to val over the whole padded plane. */
+int dnsp_set_val_slice(double* A, int nx, int nz, int ny, int k0, int k1, double val, void* stream);
+
+/* Neumann y ghosts A(:,:,-1) = A(:,:,0), A(:,:,ny) = A(:,:,ny-1)
+ * (mod_cfd_subroutines.f90:1100-1120, periodicity_pressure :1022-1038). */
+int dnsp_neumann_ghosts_y(double* A, int nx, int nz, int ny, void* stream);
+
+/* Poisson right-hand side, incompressible (mod_cfd_subroutines.f90:527-537):
+ * P = inv_dtmax*((dU*inv_dx + dW*inv_dz) + dV*inv_dy_p(k)); hats are halo arrays
+ * with valid x/z halos and V_hat wall planes. */
+int dnsp_build_p_rhs(double* P, const double* U_hat, const double* W_hat, const double* V_hat,
+                     const double* inv_dy_p, int nx, int nz, int ny, double inv_dtmax, double inv_dx,
+                     double inv_dz, void* stream);
+
+/* Poisson right-hand side, variable density with the rho_min splitting
+ * (mod_cfd_subroutines.f90:538-574).  P_now, P_old, rho, div_now: halo arrays;
+ * inv_dy_p_pad: ny+1 values; rho_min: one device double. */
+int dnsp_build_p_rhs_varprop(double* P, const double* U_hat, const double* W_hat, const double* V_hat,
+                             const double* P_now, const double* P_old, const double* rho, const double* div_now,
+                             const double* inv_dy_p, const double* inv_dy_p_pad, const double* rho_min, int nx,
+                             int nz, int ny, double inv_dtmax, double inv_dx, double inv_dz, double inv_dx2,
+                             double inv_dz2, void* stream);
+
+/* Minimum of rho over the interior into out (one device double)
+ * (mod_cfd_subroutines.f90:470-499; the MPI_Allreduce is the identity on one GPU). */
+int dnsp_rho_min(const double* rho, int nx, int nz, int ny, double* out, void* stream);
+
+/* Gradient correction update_vars (mod_cfd_subroutines.f90:1087-1216,
+ * :1225-1244; bulk forcing :1217-1221 not included):
+ *   P_per <- P extended periodically in x/z and by Neumann ghosts in y (NULL to skip
+ *            in the incompressible form);
+ *   U, W (all k) and V (k >= kmin_Vhat) corrected from the hats;
+ *   variable density (use_incompressible = 0): P_old <- P_now, P_now <- P_per;
+ *   U, V, W periodic x/z halos, then wall planes from wall_bc (device, 6 doubles:
+ *   U_bot, V_bot, W_bot, U_top, V_top, W_top; U, W at k = -1 and ny, V at k = -1, 0, ny).
+ * dPdy_ddy_hi/lo: ny values each. */
+int dnsp_update_vars(const double* P, double* P_per, double* P_now, double* P_old, double* U, double* V, double* W,
+                     const double* U_hat, const double* V_hat, const double* W_hat, const double* rho,
+                     const double* dPdy_ddy_hi, const double* dPdy_ddy_lo, const double* rho_min,
+                     const double* wall_bc, int nx, int nz, int ny, int kmin_Vhat, double dtmax, double inv_dx,
+                     double inv_dz, int use_incompressible, void* stream);
+
+/* Geometry coefficients from the faces yu(0:ny) (src/pycode/calc_bands.py:272-277,
+ * :336-343): inv_dy_p (ny), inv_dy_u (ny+1), inv_dy_p_pad (ny+1), dPdy_ddy_lo and
+ * dPdy_ddy_hi (ny, as read at inc_set_alloc_arrays.f90:51-52).  Any output may be
+ * NULL.  Host only. */
+int dnsp_geom_coeffs(int ny, const double* yu, double* inv_dy_p, double* inv_dy_u, double* inv_dy_p_pad,
+                     double* dPdy_ddy_lo, double* dPdy_ddy_hi);
+
+/* ------------------------------------------------------------------------ */
+/* Immersed-boundary forcing (src/diezdecomp_api_ibm.f90, src/ibm_setup/)     */
+/* ------------------------------------------------------------------------ */
+typedef struct dnsp_ibm dnsp_ibm;
+
+/* Replaces diezdecomp_ibm_init (diezdecomp_api_ibm.f90:61-371) on one rank.
+ * A field handled by exec is padded by nside = nhalo + nskip cells on every
+ * side: (nx+2*nside)*(nz+2*nside)*(ny+2*nside) doubles, x fastest.
+ * ijk: n records of 9 ints (target, source 1, source 2; each i, j, k),
+ * wrapped mod (nx, nz, ny) here; c: n records of 3 doubles (cw, c1, c2), the
+ * reference's all_c(0:2) order.  Repeated targets are rejected. */
+int dnsp_ibm_create(int nx, int nz, int ny, int nhalo, int nskip, int n, const int* ijk, const double* c,
+                    dnsp_ibm** out);
+
+/* Replaces diezdecomp_ibm_exec (:374-445): A[t] = c1*A[s1] + c2*A[s2] + cw for
+ * every record (all sources read first), then the new values copied into the
+ * targets' periodic images inside the padded array. */
+int dnsp_ibm_exec(dnsp_ibm* h, double* A, void* stream);
+
+/* Records, halo images, and 1 when targets and sources are disjoint (single-kernel pass). */
+int dnsp_ibm_info(const dnsp_ibm* h, int* n_forcings, int* n_images, int* disjoint);
+int dnsp_ibm_free(dnsp_ibm* h);
+
+/* ibm_read_coeffs (src/mod_utils.f90:181-212).  Host only.  With ijk == NULL
+ * only the record count is returned in *n; else ijk (9*n ints) and c (3*n:
+ * cw, c1, c2) are filled, indices wrapped mod (nx, nz, ny). */
+int dnsp_ibm_read_coeffs(const char* path, int nx, int nz, int ny, int* n, int* ijk, double* c);
+
+/* div-field extrapolation coefficients from the T records
+ * (src/ibm_setup/set_ibm_bands.f90:57-77): c = (0, 1 + d_ta/d_ab, -d_ta/d_ab);
+ * yp: the ny cell centres.  Host only. */
+int dnsp_ibm_div_coeffs(int n, const int* ijk, const double* yp, int nx, int nz, int ny, double inv_dx,
+                        double inv_dz, double* c);
+
+/* Rough-wall column slab (ibm_fill_slabs_2d, src/mod_utils.f90:215-239) minus
+ * k_first, clipped as set_ibm_bands.f90:196-218 (bottom min(v, ny-1), top
+ * max(v, 0)).  out: nx*nz ints, i fastest.  Host only. */
+int dnsp_ibm_read_slab(const char* path, int nx, int nz, int ny, int k_first, int is_top, int* out);
+
+/* ibm_part_2 for one field (src/ibm_setup/mod_cfd_ibm.f90:47-141): on the halo
+ * array A, k in [0, bot(i,j)] <- wall_bot and k in [top(i,j), kmax] <- wall_top;
+ * kmax = ny-1 for U, W, T and ny for V.  bot, top: device nx*nz ints. */
+int dnsp_ibm_part2(double* A, const int* bot, const int* top, int nx, int nz, int ny, int kmax, double wall_bot,
+                   double wall_top, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

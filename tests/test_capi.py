@@ -26,7 +26,8 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 12
     for n in names:
         assert hasattr(L, n), n
-    assert set(names) == set(D.exported_names())
+    import dnsproj.projection as DP
+    assert set(names) == set(D.exported_names()) | set(DP.exported_names())
 
 
 def test_version_and_error_strings():
