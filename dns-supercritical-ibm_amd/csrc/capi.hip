@@ -31,6 +31,12 @@ const char* dnsp_last_error(void) { return g_err; }
 
 const char* dnsp_version(void) { return "dnsproj 0.1.0 (gfx950)"; }
 
+int dnsp_stream_synchronize(void* stream) {
+    clear_error();
+    DNSP_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return DNSP_OK;
+}
+
 // poisson_solver_multigpu.f90:556-575: n_glob split in mpi_divs blocks, the
 // remainder going one each to the first blocks.
 int dnsp_get_lo_n_bounds_1d(int mpi_divs, int n_glob, int* lo_blocks_mpi, int* n_blocks_mpi) {

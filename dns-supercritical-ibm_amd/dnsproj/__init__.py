@@ -51,6 +51,7 @@ class DnspError(RuntimeError):
 _SIGNATURES = [
     ("dnsp_last_error", [], ctypes.c_char_p),
     ("dnsp_version", [], ctypes.c_char_p),
+    ("dnsp_stream_synchronize", [_vp], ctypes.c_int),
     ("dnsp_poisson_init", [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                            _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     ("dnsp_poisson_run", [_vp, _vp, _vp], ctypes.c_int),

@@ -33,6 +33,10 @@ const char* dnsp_last_error(void);
 /* Library version string. */
 const char* dnsp_version(void);
 
+/* Waits for the work enqueued on `stream` (the reference synchronises the
+ * device after every stage; callers that need it use this). */
+int dnsp_stream_synchronize(void* stream);
+
 /*
  * Replaces init_spec_poisson_multigpu
  * (src/poisson_solver_mgpu_dtdma/poisson_solver_multigpu.f90:33-326), single
