@@ -1,4 +1,4 @@
-// Compile-time-sized in-LDS batched complex FFT (FP64) for gfx950.
+// Compile-time-sized in-LDS batched complex FFT (FP64 or FP32) for gfx950.
 //
 // Same Stockham formulation as fft_lds.h, with the length N, the radix
 // sequence, the lines per workgroup G and the thread count NT known at
@@ -44,14 +44,14 @@ __host__ __device__ constexpr int ct_pad(int i) { return i + (i >> 3); }
 // LDS pitch (in complex elements) of one padded line of length n
 __host__ __device__ constexpr int ct_pitch(int n) { return n + (n >> 3) + 1; }
 
-template <int N, int R, int NS, int G, int NT, bool INV>
-__device__ __forceinline__ void ct_stage(double2* buf, const double2* __restrict__ tw) {
+template <int N, int R, int NS, int G, int NT, bool INV, typename V>
+__device__ __forceinline__ void ct_stage(V* buf, const V* __restrict__ tw) {
     constexpr int m = N / R;
     constexpr int tstep = N / (NS * R);
     constexpr int total = G * m;
     constexpr int NB = (total + NT - 1) / NT;
     constexpr int LP = ct_pitch(N);
-    double2 v[NB][R];
+    V v[NB][R];
     int base[NB], dst[NB];
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
@@ -66,7 +66,7 @@ __device__ __forceinline__ void ct_stage(double2* buf, const double2* __restrict
             if (NS > 1 && k != 0) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    const double2 w = tw[r * k * tstep];
+                    const V w = tw[r * k * tstep];
                     v[q][r] = INV ? cmulc(v[q][r], w) : cmul(v[q][r], w);
                 }
             }
@@ -86,20 +86,21 @@ __device__ __forceinline__ void ct_stage(double2* buf, const double2* __restrict
     __syncthreads();
 }
 
-template <int N, int S, int G, int NT, bool INV>
-__device__ __forceinline__ void ct_stages(double2* buf, const double2* __restrict__ tw) {
+template <int N, int S, int G, int NT, bool INV, typename V>
+__device__ __forceinline__ void ct_stages(V* buf, const V* __restrict__ tw) {
     constexpr CtPlan P = ct_plan(N);
     static_assert(P.nf >= 0, "FFT length must factor into 2, 3, 5, 7");
     if constexpr (S < P.nf) {
-        ct_stage<N, P.r[S], ct_ns(N, S), G, NT, INV>(buf, tw);
-        ct_stages<N, S + 1, G, NT, INV>(buf, tw);
+        ct_stage<N, P.r[S], ct_ns(N, S), G, NT, INV, V>(buf, tw);
+        ct_stages<N, S + 1, G, NT, INV, V>(buf, tw);
     }
 }
 
 // Transforms G padded lines (pitch ct_pitch(N)) in LDS.  All NT threads call it.
-template <int N, int G, int NT, bool INV>
-__device__ __forceinline__ void fft_ct(double2* buf, const double2* __restrict__ tw) {
-    ct_stages<N, 0, G, NT, INV>(buf, tw);
+// V is double2 or float2; tw holds exp(-2 pi i t/N) rounded to that type.
+template <int N, int G, int NT, bool INV, typename V>
+__device__ __forceinline__ void fft_ct(V* buf, const V* __restrict__ tw) {
+    ct_stages<N, 0, G, NT, INV, V>(buf, tw);
 }
 
 // Maps a linear workgroup id to a logical tile id so that each of the 8 XCDs

@@ -29,9 +29,22 @@ struct FftPlan {
     const double2* tw;  // device table, n entries
 };
 
+// Complex helpers and radix kernels for T = float or double.
+template <typename T>
+struct Cx;
+template <>
+struct Cx<double> {
+    using V = double2;
+    __device__ static __forceinline__ V mk(double a, double b) { return make_double2(a, b); }
+};
+template <>
+struct Cx<float> {
+    using V = float2;
+    __device__ static __forceinline__ V mk(float a, float b) { return make_float2(a, b); }
+};
+
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -39,56 +52,73 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
     return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
 // Multiplication by -i (forward) or +i (inverse).
-template <bool INV>
-__device__ __forceinline__ double2 rot(double2 a) {
-    return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+template <bool INV, typename V>
+__device__ __forceinline__ V rot(V a) {
+    V r;
+    if (INV) { r.x = -a.y; r.y = a.x; }
+    else     { r.x = a.y;  r.y = -a.x; }
+    return r;
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft2(double2* v) {
-    double2 a = v[0], b = v[1];
+template <bool INV, typename V>
+__device__ __forceinline__ void dft2(V* v) {
+    V a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft3(double2* v) {
-    constexpr double c = -0.5;
-    constexpr double s = INV ? 0.86602540378443864676 : -0.86602540378443864676;
-    double2 t = cadd(v[1], v[2]);
-    double2 d = csub(v[1], v[2]);
-    double2 m = make_double2(v[0].x + c * t.x, v[0].y + c * t.y);
-    double2 isd = make_double2(-s * d.y, s * d.x);
+template <bool INV, typename V>
+__device__ __forceinline__ void dft3(V* v) {
+    using T = decltype(v[0].x);
+    const T c = T(-0.5);
+    const T s = INV ? T(0.86602540378443864676) : T(-0.86602540378443864676);
+    V t = cadd(v[1], v[2]);
+    V d = csub(v[1], v[2]);
+    V m, isd;
+    m.x = v[0].x + c * t.x; m.y = v[0].y + c * t.y;
+    isd.x = -s * d.y; isd.y = s * d.x;
     v[0] = cadd(v[0], t);
     v[1] = cadd(m, isd);
     v[2] = csub(m, isd);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft4(double2* v) {
-    double2 t0 = cadd(v[0], v[2]);
-    double2 t1 = csub(v[0], v[2]);
-    double2 t2 = cadd(v[1], v[3]);
-    double2 t3 = rot<INV>(csub(v[1], v[3]));
+template <bool INV, typename V>
+__device__ __forceinline__ void dft4(V* v) {
+    V t0 = cadd(v[0], v[2]);
+    V t1 = csub(v[0], v[2]);
+    V t2 = cadd(v[1], v[3]);
+    V t3 = rot<INV>(csub(v[1], v[3]));
     v[0] = cadd(t0, t2);
     v[2] = csub(t0, t2);
     v[1] = cadd(t1, t3);
     v[3] = csub(t1, t3);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft5(double2* v) {
-    constexpr double c1 = 0.30901699437494742410;   // cos(2pi/5)
-    constexpr double c2 = -0.80901699437494742410;  // cos(4pi/5)
-    constexpr double s1 = 0.95105651629515357212;   // sin(2pi/5)
-    constexpr double s2 = 0.58778525229247312917;   // sin(4pi/5)
-    double2 t1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
-    double2 t2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
-    double2 m1 = make_double2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
-    double2 m2 = make_double2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
-    double2 e1 = rot<INV>(make_double2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
-    double2 e2 = rot<INV>(make_double2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
+template <bool INV, typename V>
+__device__ __forceinline__ void dft5(V* v) {
+    using T = decltype(v[0].x);
+    const T c1 = T(0.30901699437494742410);   // cos(2pi/5)
+    const T c2 = T(-0.80901699437494742410);  // cos(4pi/5)
+    const T s1 = T(0.95105651629515357212);   // sin(2pi/5)
+    const T s2 = T(0.58778525229247312917);   // sin(4pi/5)
+    V t1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
+    V t2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+    V m1, m2, e1, e2;
+    m1.x = v[0].x + c1 * t1.x + c2 * t2.x; m1.y = v[0].y + c1 * t1.y + c2 * t2.y;
+    m2.x = v[0].x + c2 * t1.x + c1 * t2.x; m2.y = v[0].y + c2 * t1.y + c1 * t2.y;
+    e1.x = s1 * d1.x + s2 * d2.x; e1.y = s1 * d1.y + s2 * d2.y;
+    e2.x = s2 * d1.x - s1 * d2.x; e2.y = s2 * d1.y - s1 * d2.y;
+    e1 = rot<INV>(e1);
+    e2 = rot<INV>(e2);
     v[0] = cadd(v[0], cadd(t1, t2));
     v[1] = cadd(m1, e1);
     v[4] = csub(m1, e1);
@@ -96,21 +126,22 @@ __device__ __forceinline__ void dft5(double2* v) {
     v[3] = csub(m2, e2);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft8(double2* v) {
-    constexpr double h = 0.70710678118654752440;  // sqrt(1/2)
-    double2 e[4] = {v[0], v[2], v[4], v[6]};
-    double2 o[4] = {v[1], v[3], v[5], v[7]};
+template <bool INV, typename V>
+__device__ __forceinline__ void dft8(V* v) {
+    using T = decltype(v[0].x);
+    const T h = T(0.70710678118654752440);  // sqrt(1/2)
+    V e[4] = {v[0], v[2], v[4], v[6]};
+    V o[4] = {v[1], v[3], v[5], v[7]};
     dft4<INV>(e);
     dft4<INV>(o);
     // o1 *= w8, o2 *= w8^2 (= -i fwd / +i inv), o3 *= w8^3
-    double2 o1 = o[1], o3 = o[3];
+    V o1 = o[1], o3 = o[3];
     if (!INV) {
-        o[1] = make_double2(h * (o1.x + o1.y), h * (o1.y - o1.x));
-        o[3] = make_double2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+        o[1].x = h * (o1.x + o1.y); o[1].y = h * (o1.y - o1.x);
+        o[3].x = h * (o3.y - o3.x); o[3].y = -h * (o3.x + o3.y);
     } else {
-        o[1] = make_double2(h * (o1.x - o1.y), h * (o1.x + o1.y));
-        o[3] = make_double2(-h * (o3.x + o3.y), h * (o3.x - o3.y));
+        o[1].x = h * (o1.x - o1.y); o[1].y = h * (o1.x + o1.y);
+        o[3].x = -h * (o3.x + o3.y); o[3].y = h * (o3.x - o3.y);
     }
     o[2] = rot<INV>(o[2]);
 #pragma unroll
@@ -121,15 +152,15 @@ __device__ __forceinline__ void dft8(double2* v) {
 }
 
 // Any radix, twiddles read from the length-n table (rs = n/R).
-template <int R, bool INV>
-__device__ __forceinline__ void dft_generic(double2* v, const double2* __restrict__ tw, int rs) {
-    double2 o[R];
+template <int R, bool INV, typename V>
+__device__ __forceinline__ void dft_generic(V* v, const V* __restrict__ tw, int rs) {
+    V o[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-        double2 s = v[0];
+        V s = v[0];
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-            const double2 w = tw[((q * r) % R) * rs];
+            const V w = tw[((q * r) % R) * rs];
             s = cadd(s, INV ? cmulc(v[r], w) : cmul(v[r], w));
         }
         o[q] = s;
@@ -138,8 +169,8 @@ __device__ __forceinline__ void dft_generic(double2* v, const double2* __restric
     for (int q = 0; q < R; ++q) v[q] = o[q];
 }
 
-template <int R, bool INV>
-__device__ __forceinline__ void dft(double2* v, const double2* __restrict__ tw, int rs) {
+template <int R, bool INV, typename V>
+__device__ __forceinline__ void dft(V* v, const V* __restrict__ tw, int rs) {
     if constexpr (R == 2) dft2<INV>(v);
     else if constexpr (R == 3) dft3<INV>(v);
     else if constexpr (R == 4) dft4<INV>(v);

@@ -25,6 +25,7 @@
 #include "fft_lds.h"
 #include "poisson_ct.h"
 #include "poisson_ct_launch.h"
+#include "poisson_internal.h"
 
 namespace dnsp {
 
@@ -338,75 +339,9 @@ static void eigenvalues(int n, double inv_d, std::vector<double>& lam, std::vect
 }
 
 
-// partitioned tridiagonal: P = 32 chunks of MR rows, M = 16 modes per workgroup
-constexpr int kPartChunks = 32;
-#define DNSP_PART_ROWS(X) X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32)
-
-static int part_rows_for(int ny) {
-    for (int mr : {2, 3, 4, 6, 8, 12, 16, 24, 32})
-        if (mr * kPartChunks >= ny) return mr;
-    return 0;
-}
-
-// variant 0: 16 modes x 32 chunks (512 threads); variant 1: 8 modes x 32 chunks (256 threads)
-constexpr int kPartVariants = 2;
-
-template <int M>
-static void part_launch_m(int mr, double* S, const double* ax, const double* az, const int* i0x,
-                          const int* i0z, const double* bands, int nx, int nz, int ny, double pcr_min,
-                          hipStream_t st) {
-    const long plane = (long)nx * nz;
-    const int nb = (int)((plane + M - 1) / M);
-    const size_t lds = 6 * sizeof(double) * ny;
-    switch (mr) {
-#define DNSP_CASE(MR)                                                                                  \
-    case MR:                                                                                           \
-        kt_tri_part<MR, kPartChunks, M><<<nb, kPartChunks * M, lds, st>>>(S, ax, az, i0x, i0z, bands, \
-                                                                         nx, ny, plane, pcr_min);      \
-        break;
-        DNSP_PART_ROWS(DNSP_CASE)
-#undef DNSP_CASE
-        default: break;
-    }
-}
-
-static void part_launch(int variant, int mr, double* S, const double* ax, const double* az, const int* i0x,
-                        const int* i0z, const double* bands, int nx, int nz, int ny, double pcr_min,
-                        hipStream_t st) {
-    if (variant == 1) part_launch_m<8>(mr, S, ax, az, i0x, i0z, bands, nx, nz, ny, pcr_min, st);
-    else part_launch_m<16>(mr, S, ax, az, i0x, i0z, bands, nx, nz, ny, pcr_min, st);
-}
-
 }  // namespace dnsp
 
 using namespace dnsp;
-
-struct dnsp_poisson {
-    int nx = 0, nz = 0, ny = 0;
-    double inv_dx = 0, inv_dz = 0;
-    int precision = 0;
-    int device = 0;
-    FftPlan px{}, pz{};
-    int Gx = 1, TAz = 1, ncol = 0;
-    double* d_ax = nullptr;
-    double* d_az = nullptr;
-    int* d_is0x = nullptr;
-    int* d_is0z = nullptr;
-    double* d_bands = nullptr;
-    double2* d_twx = nullptr;
-    double2* d_twz = nullptr;
-    double* d_cp = nullptr;    // serial Thomas c' workspace, nx*nz*ny (allocated when used)
-    int fft_strategy = DNSP_FFT_AUTO;
-    int tri_strategy = DNSP_TRIDIAG_AUTO;
-    bool ct_x = false, ct_z = false;  // compile-time FFT kernels in use
-    int xvar = 0, zvar = 0;           // their (tile, threads) variants
-    int tvar = 0;                     // partitioned tridiagonal variant
-    int part_rows = 0;                // partitioned tridiagonal rows per chunk (0 = serial)
-    double* d_host = nullptr;  // staging for dnsp_poisson_run_host
-    double min_pivot = 0;
-    // modes with |lambda| below this solve their reduced system serially
-    double pcr_min = 0;
-};
 
 static void poisson_release(dnsp_poisson* h) {
     if (!h) return;
@@ -421,12 +356,14 @@ static void poisson_release(dnsp_poisson* h) {
     (void)hipFree(h->d_twz);
     (void)hipFree(h->d_cp);
     (void)hipFree(h->d_host);
+    mixed_release(h);
     if (cur != h->device) (void)hipSetDevice(cur);
     delete h;
 }
 
 static int poisson_setup(dnsp_poisson* h, const double* const bands[6]) {
     DNSP_HIP_TRY(hipGetDevice(&h->device));
+    DNSP_HIP_TRY(hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device));
     const int nx = h->nx, nz = h->nz, ny = h->ny;
     if (!fft_factorize(nx, h->px) || !fft_factorize(nz, h->pz)) {
         set_error("nx=%d / nz=%d: FFT lengths must factor into 2, 3, 5 and 7", nx, nz);
@@ -575,9 +512,12 @@ int dnsp_poisson_get_variants(const dnsp_poisson* h, int* xvar, int* zvar, int* 
 
 // Times `reps` forward+inverse pairs of each FFT kernel variant per direction,
 // and pairs of tridiagonal solves per partitioned variant, on `work`
-// (zero-filled first; NULL = a temporary device array) and keeps the fastest.
-// times_ms (optional, 2*kCtVariants + kPartVariants entries) receives the
-// mean milliseconds of x, z, then tridiagonal variants (-1 where not applicable).
+// (zero-filled first; NULL = a temporary device array) and keeps the fastest;
+// for a mixed-precision handle the same for its kernels.  times_ms (optional,
+// DNSP_TUNE_SLOTS entries): mean ms of the x, z and tridiagonal variants (4
+// slots each), then the same for the mixed-precision kernels; -1 where not
+// applicable.
+static_assert(kCtVariants == 4 && kPartVariants == 4, "tuner table layout");
 int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream, double* times_ms) {
     clear_error();
     DNSP_ARG_CHECK(h, "NULL handle");
@@ -592,7 +532,7 @@ int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream,
     int pick[3] = {h->xvar, h->zvar, h->tvar};
     const int keep_x = h->xvar, keep_z = h->zvar, keep_t = h->tvar;
     if (times_ms)
-        for (int i = 0; i < 2 * kCtVariants + kPartVariants; ++i) times_ms[i] = -1.0;
+        for (int i = 0; i < DNSP_TUNE_SLOTS; ++i) times_ms[i] = -1.0;
     do {
         if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
             hipMemsetAsync(buf, 0, sizeof(double) * N, st) != hipSuccess) {
@@ -626,7 +566,7 @@ int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream,
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, e0, e1);
                 const double per = (double)ms / reps;
-                if (times_ms) times_ms[dir * kCtVariants + v] = per;  // dir 2 starts at 2*kCtVariants
+                if (times_ms) times_ms[dir * 4 + v] = per;
                 if (per < best[dir]) {
                     best[dir] = per;
                     pick[dir] = v;
@@ -636,8 +576,8 @@ int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream,
     } while (0);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    if (!work && buf) (void)hipFree(buf);
     if (rc) {
+        if (!work && buf) (void)hipFree(buf);
         h->xvar = keep_x;
         h->zvar = keep_z;
         h->tvar = keep_t;
@@ -646,7 +586,9 @@ int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream,
     h->xvar = pick[0];
     h->zvar = pick[1];
     h->tvar = pick[2];
-    return DNSP_OK;
+    if (h->precision == DNSP_PRECISION_MIXED) rc = mixed_autotune(h, buf, reps, st, times_ms ? times_ms + 12 : nullptr);
+    if (!work && buf) (void)hipFree(buf);
+    return rc;
 }
 
 int dnsp_poisson_init(int nx_glob, int nz_glob, int ny, double inv_dx, double inv_dz,
@@ -660,7 +602,8 @@ int dnsp_poisson_init(int nx_glob, int nz_glob, int ny, double inv_dx, double in
     DNSP_ARG_CHECK(ny >= 2, "ny_glob <2 (ny=%d)", ny);  // alloc_poisson_solver_multigpu.f90:34
     DNSP_ARG_CHECK((long)nx_glob * nz_glob * ny < (1L << 40), "field too large");
     DNSP_ARG_CHECK(std::isfinite(inv_dx) && std::isfinite(inv_dz), "inv_dx / inv_dz not finite");
-    DNSP_ARG_CHECK(precision == 0, "precision=%d: this entry point is the plain FP64 form (0)", precision);
+    DNSP_ARG_CHECK(precision == DNSP_PRECISION_FP64 || precision == DNSP_PRECISION_MIXED,
+                   "precision=%d: 0 (FP64) or 1 (FP64 plus the mixed-precision form)", precision);
     const double* bands[6] = {P_band_a, P_band_b, P_band_c, P_band_a_00, P_band_b_00, P_band_c_00};
     for (int s = 0; s < 6; ++s) {
         DNSP_ARG_CHECK(bands[s] != nullptr, "band array %d is NULL", s);
@@ -680,6 +623,7 @@ int dnsp_poisson_init(int nx_glob, int nz_glob, int ny, double inv_dx, double in
     h->precision = precision;
     int rc = poisson_setup(h, bands);
     if (!rc) rc = poisson_apply_strategy(h);
+    if (!rc) rc = mixed_setup(h);
     if (rc) {
         poisson_release(h);
         return rc;
@@ -711,8 +655,8 @@ int dnsp_poisson_stage(dnsp_poisson* h, const double* src, double* P, int stage,
             break;
         case DNSP_STAGE_TRIDIAG: {
             if (h->part_rows > 0) {
-                part_launch(h->tvar, h->part_rows, P, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands, h->nx, h->nz,
-                            h->ny, h->pcr_min, st);
+                part_launch<double>(h->tvar, h->part_rows, P, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands,
+                                    h->nx, h->nz, h->ny, h->pcr_min, h->ncu, nullptr, nullptr, 0.0, st);
             } else {
                 const long plane = (long)h->nx * h->nz;
                 const int nb = (int)((plane + 255) / 256);

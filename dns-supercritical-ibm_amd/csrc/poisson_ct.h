@@ -1,6 +1,5 @@
-// Compile-time-sized FP64 kernels of the Poisson solve: x and z transforms for
-// lengths 32..768 and the partitioned wall-normal tridiagonal solve.
-// Included by poisson.hip only.  Layout and halfcomplex conventions are those
+// Compile-time-sized kernels of the Poisson solve: x and z transforms for
+// lengths 32..768 and the partitioned wall-normal tridiagonal solve.  Layout and halfcomplex conventions are those
 // of the runtime-sized kernels in poisson.hip.
 #pragma once
 
@@ -146,27 +145,33 @@ __global__ __launch_bounds__(NT) void kt_xbwd(const double* S, double* P, long n
 // row is paired with zero.  (Any two real rows can share one complex
 // transform; the runtime-sized kernels pair rows c and nx-c instead.)
 // ---------------------------------------------------------------------------
-template <int N, int TA, int NT>
-__global__ __launch_bounds__(NT) void kt_zfwd(double* S, int nx, int ncol, int ntc,
-                                              const double2* __restrict__ tw) {
+// T is the storage (and arithmetic) type, double or float.  When lsum is not
+// NULL, the workgroup holding x row 0 of plane k also writes
+// msum[k] = sum_j lsum[k*N + j] (FP64 line sums -> plane sum, fixed order).
+template <int N, int TA, int NT, typename T = double>
+__global__ __launch_bounds__(NT) void kt_zfwd(T* S, int nx, int ncol, int ntc,
+                                              const typename Cx<T>::V* __restrict__ tw,
+                                              const double* __restrict__ lsum = nullptr,
+                                              double* __restrict__ msum = nullptr) {
+    using V = typename Cx<T>::V;
     constexpr int LP = ct_pitch(N);
     constexpr int W = 2 * TA;  // real rows per tile
     constexpr int TOT = W * N;
     constexpr int NL = (TOT + NT - 1) / NT;
-    __shared__ double2 sm[TA * LP];
-    double* smd = reinterpret_cast<double*>(sm);
+    __shared__ V sm[TA * LP];
+    T* smd = reinterpret_cast<T*>(sm);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int k = tile / ntc;
     const int r0 = (tile - k * ntc) * W;  // first real row of the tile
-    double* Sk = S + (size_t)k * nx * N;
-    double v[NL];
+    T* Sk = S + (size_t)k * nx * N;
+    T v[NL];
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
         const int e = (int)threadIdx.x + q * NT;
         const int r = e % W;
         const int j = e / W;
         const bool ok = (TOT % NT == 0 || e < TOT) && r0 + r < nx;
-        v[q] = ok ? Sk[(size_t)j * nx + r0 + r] : 0.0;
+        v[q] = ok ? Sk[(size_t)j * nx + r0 + r] : T(0);
     }
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
@@ -188,31 +193,40 @@ __global__ __launch_bounds__(NT) void kt_zfwd(double* S, int nx, int ncol, int n
             const int tt = r >> 1;
             const bool lo = 2 * b <= N;
             const int qq = lo ? b : N - b;
-            const double2 Z = sm[tt * LP + ct_pad(qq)];
-            const double2 Zm = sm[tt * LP + ct_pad(qq == 0 ? 0 : N - qq)];
-            double val;
-            if ((r & 1) == 0) val = lo ? 0.5 * (Z.x + Zm.x) : 0.5 * (Z.y - Zm.y);
-            else              val = lo ? 0.5 * (Z.y + Zm.y) : 0.5 * (Zm.x - Z.x);
+            const V Z = sm[tt * LP + ct_pad(qq)];
+            const V Zm = sm[tt * LP + ct_pad(qq == 0 ? 0 : N - qq)];
+            const T h = T(0.5);
+            T val;
+            if ((r & 1) == 0) val = lo ? h * (Z.x + Zm.x) : h * (Z.y - Zm.y);
+            else              val = lo ? h * (Z.y + Zm.y) : h * (Zm.x - Z.x);
             Sk[(size_t)b * nx + r0 + r] = val;
         }
     }
+    if (lsum != nullptr && r0 == 0 && threadIdx.x < 64) {
+        double acc = 0.0;
+        for (int j = (int)threadIdx.x; j < N; j += 64) acc += lsum[(size_t)k * N + j];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (threadIdx.x == 0) msum[k] = acc;
+    }
 }
 
-template <int N, int TA, int NT>
-__global__ __launch_bounds__(NT) void kt_zbwd(double* S, int nx, int ncol, int ntc,
-                                              const double2* __restrict__ tw) {
+template <int N, int TA, int NT, typename T = double>
+__global__ __launch_bounds__(NT) void kt_zbwd(T* S, int nx, int ncol, int ntc,
+                                              const typename Cx<T>::V* __restrict__ tw) {
+    using V = typename Cx<T>::V;
     constexpr int LP = ct_pitch(N);
     constexpr int W = 2 * TA;
     constexpr int H = N / 2 + 1;
     constexpr int NI = (H * TA + NT - 1) / NT;
     constexpr int TOT = W * N;
     constexpr int NL = (TOT + NT - 1) / NT;
-    __shared__ double2 sm[TA * LP];
+    __shared__ V sm[TA * LP];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int k = tile / ntc;
     const int r0 = (tile - k * ntc) * W;
-    double* Sk = S + (size_t)k * nx * N;
-    double ur[NI], ui[NI], vr[NI], vi[NI];
+    T* Sk = S + (size_t)k * nx * N;
+    T ur[NI], ui[NI], vr[NI], vi[NI];
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
         const int e = (int)threadIdx.x + q * NT;
@@ -222,10 +236,10 @@ __global__ __launch_bounds__(NT) void kt_zbwd(double* S, int nx, int ncol, int n
         const int ru = r0 + 2 * tt, rv = ru + 1;
         const bool single = (qq == 0) || (2 * qq == N);
         const bool hu = ok && ru < nx, hv = ok && rv < nx;
-        ur[q] = hu ? Sk[(size_t)qq * nx + ru] : 0.0;
-        vr[q] = hv ? Sk[(size_t)qq * nx + rv] : 0.0;
-        ui[q] = (hu && !single) ? Sk[(size_t)(N - qq) * nx + ru] : 0.0;
-        vi[q] = (hv && !single) ? Sk[(size_t)(N - qq) * nx + rv] : 0.0;
+        ur[q] = hu ? Sk[(size_t)qq * nx + ru] : T(0);
+        vr[q] = hv ? Sk[(size_t)qq * nx + rv] : T(0);
+        ui[q] = (hu && !single) ? Sk[(size_t)(N - qq) * nx + ru] : T(0);
+        vi[q] = (hv && !single) ? Sk[(size_t)(N - qq) * nx + rv] : T(0);
     }
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
@@ -234,13 +248,13 @@ __global__ __launch_bounds__(NT) void kt_zbwd(double* S, int nx, int ncol, int n
             const int tt = e % TA;
             const int qq = e / TA;
             const bool single = (qq == 0) || (2 * qq == N);
-            sm[tt * LP + ct_pad(qq)] = make_double2(ur[q] - vi[q], ui[q] + vr[q]);
-            if (!single) sm[tt * LP + ct_pad(N - qq)] = make_double2(ur[q] + vi[q], vr[q] - ui[q]);
+            sm[tt * LP + ct_pad(qq)] = Cx<T>::mk(ur[q] - vi[q], ui[q] + vr[q]);
+            if (!single) sm[tt * LP + ct_pad(N - qq)] = Cx<T>::mk(ur[q] + vi[q], vr[q] - ui[q]);
         }
     }
     __syncthreads();
     fft_ct<N, TA, NT, true>(sm, tw);
-    const double* smd = reinterpret_cast<const double*>(sm);
+    const T* smd = reinterpret_cast<const T*>(sm);
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
         const int e = (int)threadIdx.x + q * NT;
@@ -279,153 +293,209 @@ __device__ __forceinline__ double rcp64(double b) {
     return r;
 }
 
-template <int MR, int P, int M>
-__global__ __launch_bounds__(M * P) void kt_tri_part(double* S, const double* __restrict__ ax,
+// Without LOOP a workgroup solves one mode group of M modes.  With LOOP a
+// workgroup loops over mode groups and stages the bands in LDS once; the loop
+// costs registers (fewer resident waves).  OCC is the minimum number of
+// resident waves per SIMD the register allocation must allow.  T is the storage type of S
+// (double, or float for the mixed-precision form); the arithmetic is FP64.
+// When msum is not NULL the (0,0) mode is solved from the FP64 plane sums
+// msum[k] instead of S, its solution times inv_nxnz goes to pbar[k] and its
+// S entries are set to zero.
+template <int MR, int P, int M, typename T>
+__device__ __forceinline__ void tri_load_rows(double (&d)[MR], const T* S, long mode, long plane, int ny, int s,
+                                              const double* __restrict__ msum) {
+    const bool valid = mode < plane;
+    const bool mean = msum != nullptr && mode == 0;
+#pragma unroll
+    for (int r = 0; r < MR; ++r) {
+        const int row = s + r;
+        if (mean) d[r] = row < ny ? msum[row] : 0.0;
+        else d[r] = (valid && row < ny) ? (double)S[(size_t)row * plane + mode] : 0.0;
+    }
+}
+
+template <int MR, int P, int M, typename T = double, bool LOOP = false, int OCC = 1>
+__global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __restrict__ ax,
                                                      const double* __restrict__ az,
                                                      const int* __restrict__ is0x,
                                                      const int* __restrict__ is0z,
                                                      const double* __restrict__ bands, int nx, int ny,
-                                                     long plane, double pcr_min) {
+                                                     long plane, double pcr_min, int ngroups,
+                                                     const double* __restrict__ msum = nullptr,
+                                                     double* __restrict__ pbar = nullptr,
+                                                     double inv_nxnz = 0.0) {
     constexpr int NT = M * P;
     constexpr int NR = 2 * P;  // reduced rows per mode
     __shared__ double ra[NR][M], rb[NR][M], rc[NR][M], rd[NR][M];  // mode fastest
     extern __shared__ double smb[];                                  // 6*ny band values
-    for (int e = (int)threadIdx.x; e < 6 * ny; e += NT) smb[e] = bands[e];
-    const int mi = (int)threadIdx.x % M;
     const int p = (int)threadIdx.x / M;
-    const long mode = (long)xcd_tile(blockIdx.x, gridDim.x) * M + mi;
-    const bool valid = mode < plane;
-    const long md = valid ? mode : 0;
-    const int i = (int)(md % nx);
-    const int j = (int)(md / nx);
-    const double lam = ax[i] + az[j];
     const int s = p * MR;
 
+    // Adjacent mode groups share cache lines, so they go to one XCD (each has
+    // its own L2; workgroups reach XCDs round-robin): without LOOP through
+    // xcd_tile, with LOOP as a contiguous range of groups per XCD, strided
+    // over that XCD's workgroups (gridDim.x >= 8).
+    const int mi = (int)threadIdx.x % M;
+    int grp, g_end, g_step;
+    if constexpr (LOOP) {
+        const int xcd = (int)blockIdx.x % 8;
+        g_step = (int)gridDim.x / 8 + (xcd < (int)gridDim.x % 8 ? 1 : 0);
+        grp = (int)((long)ngroups * xcd / 8) + (int)blockIdx.x / 8;
+        g_end = (int)((long)ngroups * (xcd + 1) / 8);
+    } else {
+        grp = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+        g_end = grp + 1;
+        g_step = 1;
+    }
+    // the (0,0)-mode band set is only needed by the workgroup holding mode 0
+    const int nband = (grp == 0 ? 6 : 3) * ny;
+    for (int e = (int)threadIdx.x; e < nband; e += NT) smb[e] = bands[e];
+    bool staged = false;
     double d[MR];
-#pragma unroll
-    for (int r = 0; r < MR; ++r) {
-        const int row = s + r;
-        d[r] = (valid && row < ny) ? S[(size_t)row * plane + md] : 0.0;
-    }
-    __syncthreads();
-    const double* A = smb + ((is0x[i] && is0z[j]) ? 3 * ny : 0);
-    const double* B = A + ny;
-    const double* C = B + ny;
-    auto band_a = [&](int row) { return row < ny ? A[row] : 0.0; };
-    auto band_c = [&](int row) { return row < ny ? C[row] : 0.0; };
-    auto beta = [&](int row) { return row < ny ? B[row] + lam : 1.0; };
+    if (grp < g_end) tri_load_rows<MR, P, M, T>(d, S, (long)grp * M + mi, plane, ny, s, msum);
+    for (; grp < g_end; grp += g_step) {
+        const long mode = (long)grp * M + mi;
+        const int nxt = grp + g_step;
 
-    double Ah[MR], Ch[MR];  // d[] is overwritten by D^ for interior rows
-    double Ae, Be, De;
-    {
-        double Ak = band_a(s + 1);
-        double Bk = beta(s + 1);
-        double Dk = d[1];
-        double cprev = band_c(s + 1);
-#pragma unroll
-        for (int l = 2; l < MR; ++l) {
-            const double rinv = rcp64(Bk);
-            d[l - 1] = Dk * rinv;
-            Ah[l - 1] = Ak * rinv;
-            Ch[l - 1] = cprev * rinv;
-            const double f = band_a(s + l) * rinv;
-            Ak = -f * Ak;
-            Bk = beta(s + l) - f * cprev;
-            Dk = d[l] - f * Dk;
-            cprev = band_c(s + l);
+        const bool valid = mode < plane;
+        const long md = valid ? mode : 0;
+        const int i = (int)(md % nx);
+        const int j = (int)(md / nx);
+        const double lam = ax[i] + az[j];
+        // bands staged.  Later groups need no barrier here: each thread's last
+        // reads of the reduced rows are its own slots, which only it rewrites.
+        if (!staged) {
+            __syncthreads();
+            staged = true;
         }
-        Ae = Ak;
-        Be = Bk;
-        De = Dk;
-    }
-    // upward pass: x_{s+1} = Pp - Qq x_s - Rr x_e
-    double Pp = 0.0, Qq = 0.0, Rr = -1.0;
+        const double* A = smb + ((is0x[i] && is0z[j]) ? 3 * ny : 0);
+        const double* B = A + ny;
+        const double* C = B + ny;
+        auto band_a = [&](int row) { return row < ny ? A[row] : 0.0; };
+        auto band_c = [&](int row) { return row < ny ? C[row] : 0.0; };
+        auto beta = [&](int row) { return row < ny ? B[row] + lam : 1.0; };
+
+        double Ah[MR], Ch[MR];  // d[] is overwritten by D^ for interior rows
+        double Ae, Be, De;
+        {
+            double Ak = band_a(s + 1);
+            double Bk = beta(s + 1);
+            double Dk = d[1];
+            double cprev = band_c(s + 1);
 #pragma unroll
-    for (int l = MR - 2; l >= 1; --l) {
-        Pp = d[l] - Ch[l] * Pp;
-        Qq = Ah[l] - Ch[l] * Qq;
-        Rr = -Ch[l] * Rr;
-    }
-    {
-        const double cs = band_c(s);
-        ra[2 * p][mi] = band_a(s);
-        rb[2 * p][mi] = beta(s) - cs * Qq;
-        rc[2 * p][mi] = -cs * Rr;
-        rd[2 * p][mi] = d[0] - cs * Pp;
-        ra[2 * p + 1][mi] = Ae;
-        rb[2 * p + 1][mi] = Be;
-        rc[2 * p + 1][mi] = band_c(s + MR - 1);
-        rd[2 * p + 1][mi] = De;
-    }
-    const bool serial = __syncthreads_or(valid && fabs(lam) < pcr_min) != 0;
-    if (serial) {
-        // one thread per mode: Thomas on the NR tip rows; x overwrites rd, rb = 1
-        if (p == 0) {
-            for (int q = 1; q < NR; ++q) {
-                const double m = ra[q][mi] / rb[q - 1][mi];
-                rb[q][mi] = rb[q][mi] - m * rc[q - 1][mi];
-                rd[q][mi] = rd[q][mi] - m * rd[q - 1][mi];
+            for (int l = 2; l < MR; ++l) {
+                const double rinv = rcp64(Bk);
+                d[l - 1] = Dk * rinv;
+                Ah[l - 1] = Ak * rinv;
+                Ch[l - 1] = cprev * rinv;
+                const double f = band_a(s + l) * rinv;
+                Ak = -f * Ak;
+                Bk = beta(s + l) - f * cprev;
+                Dk = d[l] - f * Dk;
+                cprev = band_c(s + l);
             }
-            double xq = rd[NR - 1][mi] / rb[NR - 1][mi];
-            rd[NR - 1][mi] = xq;
-            rb[NR - 1][mi] = 1.0;
-            for (int q = NR - 2; q >= 0; --q) {
-                xq = (rd[q][mi] - rc[q][mi] * xq) / rb[q][mi];
-                rd[q][mi] = xq;
-                rb[q][mi] = 1.0;
+            Ae = Ak;
+            Be = Bk;
+            De = Dk;
+        }
+        // upward pass: x_{s+1} = Pp - Qq x_s - Rr x_e
+        double Pp = 0.0, Qq = 0.0, Rr = -1.0;
+#pragma unroll
+        for (int l = MR - 2; l >= 1; --l) {
+            Pp = d[l] - Ch[l] * Pp;
+            Qq = Ah[l] - Ch[l] * Qq;
+            Rr = -Ch[l] * Rr;
+        }
+        {
+            const double cs = band_c(s);
+            ra[2 * p][mi] = band_a(s);
+            rb[2 * p][mi] = beta(s) - cs * Qq;
+            rc[2 * p][mi] = -cs * Rr;
+            rd[2 * p][mi] = d[0] - cs * Pp;
+            ra[2 * p + 1][mi] = Ae;
+            rb[2 * p + 1][mi] = Be;
+            rc[2 * p + 1][mi] = band_c(s + MR - 1);
+            rd[2 * p + 1][mi] = De;
+        }
+        const bool serial = __syncthreads_or(valid && fabs(lam) < pcr_min) != 0;
+        if (serial) {
+            // one thread per mode: Thomas on the NR tip rows; x overwrites rd, rb = 1
+            if (p == 0) {
+                for (int q = 1; q < NR; ++q) {
+                    const double m = ra[q][mi] / rb[q - 1][mi];
+                    rb[q][mi] = rb[q][mi] - m * rc[q - 1][mi];
+                    rd[q][mi] = rd[q][mi] - m * rd[q - 1][mi];
+                }
+                double xq = rd[NR - 1][mi] / rb[NR - 1][mi];
+                rd[NR - 1][mi] = xq;
+                rb[NR - 1][mi] = 1.0;
+                for (int q = NR - 2; q >= 0; --q) {
+                    xq = (rd[q][mi] - rc[q][mi] * xq) / rb[q][mi];
+                    rd[q][mi] = xq;
+                    rb[q][mi] = 1.0;
+                }
+            }
+            __syncthreads();
+        }
+        // parallel cyclic reduction on the NR tip rows of each mode
+        for (int st = 1; !serial && st < NR; st *= 2) {
+            double na[2], nb[2], nc[2], nd[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int e = (int)threadIdx.x + h * NT;
+                const int mm = e % M;
+                const int q = e / M;
+                const double aq = ra[q][mm], bq = rb[q][mm], cq = rc[q][mm], dq = rd[q][mm];
+                double a2 = 0.0, b2 = bq, c2 = 0.0, d2 = dq;
+                if (q - st >= 0) {
+                    const double al = -aq * rcp64(rb[q - st][mm]);
+                    a2 = al * ra[q - st][mm];
+                    b2 += al * rc[q - st][mm];
+                    d2 += al * rd[q - st][mm];
+                }
+                if (q + st < NR) {
+                    const double ga = -cq * rcp64(rb[q + st][mm]);
+                    c2 = ga * rc[q + st][mm];
+                    b2 += ga * ra[q + st][mm];
+                    d2 += ga * rd[q + st][mm];
+                }
+                na[h] = a2; nb[h] = b2; nc[h] = c2; nd[h] = d2;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int e = (int)threadIdx.x + h * NT;
+                const int mm = e % M;
+                const int q = e / M;
+                ra[q][mm] = na[h]; rb[q][mm] = nb[h]; rc[q][mm] = nc[h]; rd[q][mm] = nd[h];
+            }
+            __syncthreads();
+        }
+        const double xs = serial ? rd[2 * p][mi] : rd[2 * p][mi] * rcp64(rb[2 * p][mi]);
+        const double xe = serial ? rd[2 * p + 1][mi] : rd[2 * p + 1][mi] * rcp64(rb[2 * p + 1][mi]);
+        d[0] = xs;
+        double xn = xe;
+#pragma unroll
+        for (int l = MR - 2; l >= 1; --l) {
+            xn = d[l] - Ah[l] * xs - Ch[l] * xn;
+            d[l] = xn;
+        }
+        d[MR - 1] = xe;
+        if (valid) {
+            const bool mean = msum != nullptr && mode == 0;
+#pragma unroll
+            for (int r = 0; r < MR; ++r) {
+                const int row = s + r;
+                if (row < ny) {
+                    if (mean) pbar[row] = d[r] * inv_nxnz;
+                    S[(size_t)row * plane + md] = mean ? T(0) : (T)d[r];
+                }
             }
         }
-        __syncthreads();
-    }
-    // parallel cyclic reduction on the NR tip rows of each mode
-    for (int st = 1; !serial && st < NR; st *= 2) {
-        double na[2], nb[2], nc[2], nd[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int e = (int)threadIdx.x + h * NT;
-            const int mm = e % M;
-            const int q = e / M;
-            const double aq = ra[q][mm], bq = rb[q][mm], cq = rc[q][mm], dq = rd[q][mm];
-            double a2 = 0.0, b2 = bq, c2 = 0.0, d2 = dq;
-            if (q - st >= 0) {
-                const double al = -aq * rcp64(rb[q - st][mm]);
-                a2 = al * ra[q - st][mm];
-                b2 += al * rc[q - st][mm];
-                d2 += al * rd[q - st][mm];
-            }
-            if (q + st < NR) {
-                const double ga = -cq * rcp64(rb[q + st][mm]);
-                c2 = ga * rc[q + st][mm];
-                b2 += ga * ra[q + st][mm];
-                d2 += ga * rd[q + st][mm];
-            }
-            na[h] = a2; nb[h] = b2; nc[h] = c2; nd[h] = d2;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int e = (int)threadIdx.x + h * NT;
-            const int mm = e % M;
-            const int q = e / M;
-            ra[q][mm] = na[h]; rb[q][mm] = nb[h]; rc[q][mm] = nc[h]; rd[q][mm] = nd[h];
-        }
-        __syncthreads();
-    }
-    const double xs = serial ? rd[2 * p][mi] : rd[2 * p][mi] * rcp64(rb[2 * p][mi]);
-    const double xe = serial ? rd[2 * p + 1][mi] : rd[2 * p + 1][mi] * rcp64(rb[2 * p + 1][mi]);
-    d[0] = xs;
-    double xn = xe;
-#pragma unroll
-    for (int l = MR - 2; l >= 1; --l) {
-        xn = d[l] - Ah[l] * xs - Ch[l] * xn;
-        d[l] = xn;
-    }
-    d[MR - 1] = xe;
-    if (valid) {
-#pragma unroll
-        for (int r = 0; r < MR; ++r) {
-            const int row = s + r;
-            if (row < ny) S[(size_t)row * plane + md] = d[r];
+        if constexpr (!LOOP) {
+            break;
+        } else {
+            if (nxt < g_end) tri_load_rows<MR, P, M, T>(d, S, (long)nxt * M + mi, plane, ny, s, msum);
         }
     }
 }

@@ -1,6 +1,7 @@
-// Launchers of the compile-time-sized FFT kernels (poisson_ct_x.hip,
-// poisson_ct_z.hip).  variant in [0, kCtVariants) selects the (tile, threads)
-// pair of ct_var_tile / ct_var_threads.
+// Launchers of the compile-time-sized kernels (poisson_ct_x.hip,
+// poisson_ct_z.hip, poisson_tri.hip, poisson_mixed_x.hip).  FFT `variant` in
+// [0, kCtVariants) selects the (tile, threads) pair of ct_var_tile /
+// ct_var_threads.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,5 +12,42 @@ bool ct_fft_supported(int n);
 void ct_x_launch(int n, int variant, bool inv, const double* src, double* dst, long nlines, const double2* tw,
                  double scale, hipStream_t st);
 void ct_z_launch(int n, int variant, bool inv, double* S, int nx, int ny, const double2* tw, hipStream_t st);
+// FP32 z transforms; the forward one also turns the FP64 line sums lsum (if
+// not NULL) into the FP64 plane sums msum.
+void ct_z_launch(int n, int variant, bool inv, float* S, int nx, int ny, const float2* tw, const double* lsum,
+                 double* msum, hipStream_t st);
+
+// Partitioned tridiagonal solve: kPartChunks chunks of part_rows_for(ny) rows.
+// Variants: 0 = 16 modes per workgroup, 1 = 8 modes, one workgroup per mode
+// group; 2 = as 1 with registers capped for 4 resident waves per SIMD; 3 = 8
+// modes, persistent (as many workgroups as are resident at once, each looping
+// over mode groups).
+constexpr int kPartChunks = 32;
+constexpr int kPartVariants = 4;
+int part_rows_for(int ny);
+template <typename T>
+void part_launch(int variant, int mr, T* S, const double* ax, const double* az, const int* i0x, const int* i0z,
+                 const double* bands, int nx, int nz, int ny, double pcr_min, int ncu, const double* msum,
+                 double* pbar, double inv_nxnz, hipStream_t st);
+
+// Mixed-precision x kernels (poisson_mixed.h).
+enum MxKind { MX_FWD = 0, MX_RES_F32 = 1, MX_RES_F64 = 2, MX_BWD = 3, MX_ACC_F32 = 4, MX_ACC_F64 = 5 };
+struct MxArgs {
+    const void* src = nullptr;      // f (FWD), iterate p (RES), FP32 spectrum (BWD, ACC)
+    const double* f = nullptr;      // RES: right-hand side
+    const double* q = nullptr;      // RES: per-plane constants
+    const double* bands = nullptr;  // RES: plain a, b, c
+    const void* base = nullptr;     // ACC: FP32 field or FP64 iterate (NULL = 0)
+    const double* pbar0 = nullptr;  // ACC
+    const double* pbar1 = nullptr;  // ACC
+    void* dst = nullptr;            // FP32 spectrum (FWD, RES), FP32 field (BWD), FP64 P (ACC)
+    double* lsum = nullptr;         // FP64 line sums of the input (FWD), of r (RES), of the output
+    int nz = 0, ny = 0;
+    double ix2 = 0, iz2 = 0;
+    long nlines = 0;
+    const float2* tw = nullptr;
+    double scale = 1.0;
+};
+void mx_x_launch(int kind, int n, int variant, const MxArgs& a, hipStream_t st);
 
 }  // namespace dnsp

@@ -1,0 +1,61 @@
+// The Poisson solver handle, shared by poisson.hip (FP64 form, tuner) and
+// poisson_mixed.hip (mixed-precision form).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "fft_lds.h"
+
+struct dnsp_poisson {
+    int nx = 0, nz = 0, ny = 0;
+    double inv_dx = 0, inv_dz = 0;
+    int precision = 0;
+    int device = 0;
+    int ncu = 0;  // compute units of the device (persistent launches)
+    dnsp::FftPlan px{}, pz{};
+    int Gx = 1, TAz = 1, ncol = 0;
+    double* d_ax = nullptr;
+    double* d_az = nullptr;
+    int* d_is0x = nullptr;
+    int* d_is0z = nullptr;
+    double* d_bands = nullptr;  // a, b, c, a00, b00, c00 (ny each)
+    double2* d_twx = nullptr;
+    double2* d_twz = nullptr;
+    double* d_cp = nullptr;    // serial Thomas c' workspace, nx*nz*ny (allocated when used)
+    int fft_strategy = DNSP_FFT_AUTO;
+    int tri_strategy = DNSP_TRIDIAG_AUTO;
+    bool ct_x = false, ct_z = false;  // compile-time FFT kernels in use
+    int xvar = 0, zvar = 0;           // their (tile, threads) variants
+    int tvar = 0;                     // partitioned tridiagonal variant
+    int part_rows = 0;                // partitioned tridiagonal rows per chunk (0 = serial)
+    double* d_host = nullptr;  // staging for dnsp_poisson_run_host
+    double min_pivot = 0;
+    // modes with |lambda| below this solve their reduced system serially
+    double pcr_min = 0;
+
+    // mixed-precision form (precision == DNSP_PRECISION_MIXED)
+    int mxvar = 0, mzvar = 0, mtvar = 0;  // its FFT and tridiagonal variants
+    int mrows = 0;                         // its partitioned rows per chunk
+    float2* d_twx32 = nullptr;
+    float2* d_twz32 = nullptr;
+    float* d_A = nullptr;       // FP32 field: sweep-0 spectrum, then the sweep-0 field
+    float* d_B = nullptr;       // FP32 field: refinement spectra
+    double* d_lsA = nullptr;    // FP64 line sums (nz*ny)
+    double* d_lsB = nullptr;
+    double* d_small = nullptr;  // msum, pbar0, pbar1, q (ny each), residual partials
+    double* d_fcopy = nullptr;  // copy of an in-place right-hand side (allocated when used)
+};
+
+namespace dnsp {
+
+// Residual-norm reduction: fixed grid, two partial sums per workgroup.
+constexpr int kResBlocks = 1024;
+
+// Allocates and fills the mixed-precision state (poisson_mixed.hip).
+int mixed_setup(dnsp_poisson* h);
+void mixed_release(dnsp_poisson* h);
+// Times the mixed-precision kernel variants and keeps the fastest.
+int mixed_autotune(dnsp_poisson* h, const double* f, int reps, hipStream_t st, double* times_ms);
+
+}  // namespace dnsp

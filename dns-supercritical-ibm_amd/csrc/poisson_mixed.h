@@ -1,0 +1,331 @@
+// Kernels of the mixed-precision Poisson solve (FP32 transforms and FP32
+// spectra, FP64 tridiagonal arithmetic, FP64 residual refinement).
+//
+// Sweep 0 solves L p0 = f with p0 = pt + pbar0(k): pt an FP32 field whose
+// (0,0) mode is zero in the spectral solve, pbar0 the FP64 plane-mean profile
+// solved from the FP64 plane sums of f.  A refinement sweep forms the FP64
+// residual r = f - L p on the fly inside the forward x transform, solves
+// L d = r the same way and writes p + d in FP64.  The operator L is the one
+// the FP64 solve inverts (periodic second differences in x and z, the y bands,
+// and the (0,0)-mode band set on the plane means), so the iterates converge to
+// the FP64 solution; the plane means enter through per-plane constants q(k).
+//
+// Every kernel that produces a field also writes the FP64 sums of its x lines
+// (fixed summation order, so results are bit-reproducible run to run).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "poisson_ct.h"
+
+namespace dnsp {
+
+constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
+
+// Slots of the line-sum reduction of an x tile: pairs are reduced over groups
+// of cgcd(N/2, 64) lanes, which always lie in one line and one wavefront.
+template <int N, int G>
+constexpr int line_slots() {
+    return G * N / cgcd(N / 2, 64);
+}
+
+// FP64 sums of the 2G real lines of an x tile.  Thread pair e = tid + q*NT
+// covers elements 2e, 2e+1 of the tile (line 2e/N) and contributes ps[q]
+// (zero for pairs outside the tile or past nlines).  Contains a barrier.
+template <int N, int G, int NT, int NL>
+__device__ __forceinline__ void tile_line_sums(const double (&ps)[NL], double* slot, double* __restrict__ lsum,
+                                               long l0, long nlines) {
+    constexpr int S = N / 2;
+    constexpr int GW = cgcd(S, 64);
+    static_assert(N % 2 == 0, "even lengths only");
+    static_assert(2 * G <= NT, "one thread per line for the final sums");
+    static_assert(NT % 64 == 0, "whole wavefronts");
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        double s = ps[q];
+#pragma unroll
+        for (int o = GW / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (e % GW == 0 && e < G * N) slot[e / GW] = s;
+    }
+    __syncthreads();
+    const int t = (int)threadIdx.x;
+    if (t < 2 * G && l0 + t < nlines) {
+        double acc = 0.0;
+        for (int u = 0; u < S / GW; ++u) acc += slot[t * (S / GW) + u];
+        lsum[l0 + t] = acc;
+    }
+}
+
+// Halfcomplex output of a forward x tile (as kt_xfwd), FP32.
+template <int N, int G, int NT>
+__device__ __forceinline__ void xfwd_store_f32(const float2* sm, float* S, long l0, long nlines) {
+    constexpr int LP = ct_pitch(N);
+    constexpr int NL = (G * N + NT - 1) / NT;
+    float2* dst = reinterpret_cast<float2*>(S + l0 * N);
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int r = (2 * e) / N;
+        if ((G * N % NT == 0 || e < G * N) && l0 + r < nlines) {
+            const int t0 = 2 * e - r * N;
+            const int g = r >> 1;
+            float out[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int t = t0 + h;
+                const bool lo = 2 * t <= N;
+                const int u = lo ? t : N - t;
+                const float2 C = sm[g * LP + ct_pad(u)];
+                const float2 Cm = sm[g * LP + ct_pad(u == 0 ? 0 : N - u)];
+                if ((r & 1) == 0) out[h] = lo ? 0.5f * (C.x + Cm.x) : 0.5f * (C.y - Cm.y);
+                else              out[h] = lo ? 0.5f * (C.y + Cm.y) : 0.5f * (Cm.x - C.x);
+            }
+            dst[e] = make_float2(out[0], out[1]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sweep 0, forward x: FP64 right-hand side -> FP32 halfcomplex lines, plus the
+// FP64 sums of the input lines.
+// ---------------------------------------------------------------------------
+template <int N, int G, int NT>
+__global__ __launch_bounds__(NT) void kt_xfwd_mf(const double* __restrict__ f, float* __restrict__ S,
+                                                 double* __restrict__ lsum, long nlines,
+                                                 const float2* __restrict__ tw) {
+    constexpr int LP = ct_pitch(N);
+    constexpr int NL = (G * N + NT - 1) / NT;
+    __shared__ float2 sm[G * LP];
+    __shared__ double slot[line_slots<N, G>()];
+    float* smf = reinterpret_cast<float*>(sm);
+    const long l0 = (long)xcd_tile(blockIdx.x, gridDim.x) * 2 * G;
+    const double2* src = reinterpret_cast<const double2*>(f + l0 * N);
+    double2 v[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int r = (2 * e) / N;
+        v[q] = ((G * N % NT == 0 || e < G * N) && l0 + r < nlines) ? src[e] : make_double2(0.0, 0.0);
+    }
+    double ps[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        ps[q] = v[q].x + v[q].y;
+        if (G * N % NT == 0 || e < G * N) {
+            const int r = (2 * e) / N;
+            const int t = 2 * e - r * N;
+            const int g = r >> 1, c = r & 1;
+            smf[(g * LP + ct_pad(t)) * 2 + c] = (float)v[q].x;
+            smf[(g * LP + ct_pad(t + 1)) * 2 + c] = (float)v[q].y;
+        }
+    }
+    tile_line_sums<N, G, NT, NL>(ps, slot, lsum, l0, nlines);
+    fft_ct<N, G, NT, false>(sm, tw);
+    xfwd_store_f32<N, G, NT>(sm, S, l0, nlines);
+}
+
+// ---------------------------------------------------------------------------
+// Refinement, forward x: r = f - (L_xz + L_y) p - q(k) in FP64 for the tile's
+// lines (p of type TP: the FP32 sweep-0 field or an FP64 iterate), then the
+// FP32 halfcomplex transform of r and the FP64 sums of r's lines.
+// bands = a, b, c of the plain y band set (ny each).  Lines are (j, k) with
+// line = j + nz*k; x neighbours wrap, z neighbours wrap, y neighbours stop at
+// the walls (the operator of the FP64 solve).
+// ---------------------------------------------------------------------------
+template <int N, int G, int NT, typename TP>
+__global__ __launch_bounds__(NT) void kt_xfwd_mr(const TP* __restrict__ p, const double* __restrict__ f,
+                                                 const double* __restrict__ qk, const double* __restrict__ bands,
+                                                 float* __restrict__ S, double* __restrict__ lsum, int nz,
+                                                 int ny, double ix2, double iz2, long nlines,
+                                                 const float2* __restrict__ tw) {
+    using P2 = typename Cx<TP>::V;
+    constexpr int LP = ct_pitch(N);
+    constexpr int NL = (G * N + NT - 1) / NT;
+    __shared__ float2 sm[G * LP];
+    __shared__ double slot[line_slots<N, G>()];
+    float* smf = reinterpret_cast<float*>(sm);
+    const long l0 = (long)xcd_tile(blockIdx.x, gridDim.x) * 2 * G;
+    const int lane = (int)threadIdx.x & 63;
+    double ps[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int r = (2 * e) / N;
+        const long line = l0 + r;
+        const int t0 = 2 * e - r * N;
+        const bool in_tile = (G * N % NT == 0 || e < G * N);
+        const bool ok = in_tile && line < nlines;
+        P2 c;
+        c.x = TP(0);
+        c.y = TP(0);
+        if (ok) c = *reinterpret_cast<const P2*>(p + line * N + t0);
+        const double cx = (double)c.x, cy = (double)c.y;
+        // x neighbours from the adjacent lanes (pairs e-1, e+1 of the same line)
+        double lft = __shfl_up(cy, 1, 64);
+        double rgt = __shfl_down(cx, 1, 64);
+        double r0 = 0.0, r1 = 0.0;
+        if (ok) {
+            const TP* pl = p + line * N;
+            if (lane == 0 || t0 == 0) lft = (double)pl[t0 == 0 ? N - 1 : t0 - 1];
+            if (lane == 63 || t0 + 2 == N) rgt = (double)pl[t0 + 2 == N ? 0 : t0 + 2];
+            const int k = (int)(line / nz);
+            const int j = (int)(line - (long)k * nz);
+            const long lzm = line + (j == 0 ? nz - 1 : -1);
+            const long lzp = line + (j == nz - 1 ? 1 - nz : 1);
+            const P2 zm = *reinterpret_cast<const P2*>(p + lzm * N + t0);
+            const P2 zp = *reinterpret_cast<const P2*>(p + lzp * N + t0);
+            const double2 fv = *reinterpret_cast<const double2*>(f + line * N + t0);
+            const double bk = bands[ny + k];
+            double v0 = ix2 * (cy - 2 * cx + lft) + iz2 * ((double)zp.x - 2 * cx + (double)zm.x) + bk * cx;
+            double v1 = ix2 * (rgt - 2 * cy + cx) + iz2 * ((double)zp.y - 2 * cy + (double)zm.y) + bk * cy;
+            if (k > 0) {
+                const P2 yd = *reinterpret_cast<const P2*>(p + (line - nz) * N + t0);
+                const double ak = bands[k];
+                v0 += ak * (double)yd.x;
+                v1 += ak * (double)yd.y;
+            }
+            if (k < ny - 1) {
+                const P2 yu = *reinterpret_cast<const P2*>(p + (line + nz) * N + t0);
+                const double ck = bands[2 * ny + k];
+                v0 += ck * (double)yu.x;
+                v1 += ck * (double)yu.y;
+            }
+            const double qv = qk[k];
+            r0 = fv.x - (v0 + qv);
+            r1 = fv.y - (v1 + qv);
+        }
+        ps[q] = r0 + r1;
+        if (in_tile) {
+            const int g = r >> 1, cc = r & 1;
+            smf[(g * LP + ct_pad(t0)) * 2 + cc] = (float)r0;
+            smf[(g * LP + ct_pad(t0 + 1)) * 2 + cc] = (float)r1;
+        }
+    }
+    tile_line_sums<N, G, NT, NL>(ps, slot, lsum, l0, nlines);
+    fft_ct<N, G, NT, false>(sm, tw);
+    xfwd_store_f32<N, G, NT>(sm, S, l0, nlines);
+}
+
+// Halfcomplex pairs of lines -> LDS complex lines ready for the inverse FFT
+// (as kt_xbwd), FP32.
+template <int N, int G, int NT>
+__device__ __forceinline__ void xbwd_load_f32(float2* sm, const float* S, long l0, long nlines) {
+    constexpr int LP = ct_pitch(N);
+    constexpr int H = N / 2 + 1;
+    constexpr int NI = (G * H + NT - 1) / NT;
+    const float* src = S + l0 * N;
+    float xr[NI], xi[NI], yr[NI], yi[NI];
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int g = e / H;
+        const int t = e - g * H;
+        const bool ok = e < G * H;
+        const bool single = (t == 0) || (2 * t == N);
+        const bool va = ok && (l0 + 2 * g < nlines);
+        const bool vb = ok && (l0 + 2 * g + 1 < nlines);
+        const float* la = src + (2 * g) * N;
+        const float* lb = la + N;
+        xr[q] = va ? la[t] : 0.f;
+        xi[q] = (va && !single) ? la[N - t] : 0.f;
+        yr[q] = vb ? lb[t] : 0.f;
+        yi[q] = (vb && !single) ? lb[N - t] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        if (e < G * H) {
+            const int g = e / H;
+            const int t = e - g * H;
+            const bool single = (t == 0) || (2 * t == N);
+            sm[g * LP + ct_pad(t)] = make_float2(xr[q] - yi[q], xi[q] + yr[q]);
+            if (!single) sm[g * LP + ct_pad(N - t)] = make_float2(xr[q] + yi[q], yr[q] - xi[q]);
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Sweep 0, inverse x: FP32 halfcomplex lines -> FP32 field pt = scale * x-HC2R
+// (in place allowed: S == pt), plus the FP64 sums of the stored pt lines.
+// ---------------------------------------------------------------------------
+template <int N, int G, int NT>
+__global__ __launch_bounds__(NT) void kt_xbwd_mf(const float* S, float* pt, double* __restrict__ lsum,
+                                                 long nlines, const float2* __restrict__ tw, float scale) {
+    constexpr int LP = ct_pitch(N);
+    constexpr int NL = (G * N + NT - 1) / NT;
+    __shared__ float2 sm[G * LP];
+    __shared__ double slot[line_slots<N, G>()];
+    const long l0 = (long)xcd_tile(blockIdx.x, gridDim.x) * 2 * G;
+    xbwd_load_f32<N, G, NT>(sm, S, l0, nlines);
+    fft_ct<N, G, NT, true>(sm, tw);
+    const float* smf = reinterpret_cast<const float*>(sm);
+    float2* dst = reinterpret_cast<float2*>(pt + l0 * N);
+    double ps[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int r = (2 * e) / N;
+        ps[q] = 0.0;
+        if ((G * N % NT == 0 || e < G * N) && l0 + r < nlines) {
+            const int t = 2 * e - r * N;
+            const int g = r >> 1, c = r & 1;
+            const float2 o = make_float2(smf[(g * LP + ct_pad(t)) * 2 + c] * scale,
+                                         smf[(g * LP + ct_pad(t + 1)) * 2 + c] * scale);
+            dst[e] = o;
+            ps[q] = (double)o.x + (double)o.y;
+        }
+    }
+    tile_line_sums<N, G, NT, NL>(ps, slot, lsum, l0, nlines);
+}
+
+// ---------------------------------------------------------------------------
+// Refinement, inverse x with accumulation into FP64:
+//   P = base + (pbar0[k] + pbar1[k]) + scale * x-HC2R(S)
+// base is the FP32 sweep-0 field (with its profile pbar0), the FP64 iterate
+// itself (base == P, pbar0 == NULL) or NULL (zero); pbar0, pbar1 may be NULL.  lsum (optional) receives the FP64 sums
+// of P's lines for a following sweep or residual check.
+// ---------------------------------------------------------------------------
+template <int N, int G, int NT, typename TP>
+__global__ __launch_bounds__(NT) void kt_xbwd_ma(const float* __restrict__ S, const TP* base,
+                                                 const double* __restrict__ pbar0,
+                                                 const double* __restrict__ pbar1, double* P,
+                                                 double* __restrict__ lsum, int nz, long nlines,
+                                                 const float2* __restrict__ tw, double scale) {
+    using P2 = typename Cx<TP>::V;
+    constexpr int LP = ct_pitch(N);
+    constexpr int NL = (G * N + NT - 1) / NT;
+    __shared__ float2 sm[G * LP];
+    __shared__ double slot[line_slots<N, G>()];
+    const long l0 = (long)xcd_tile(blockIdx.x, gridDim.x) * 2 * G;
+    xbwd_load_f32<N, G, NT>(sm, S, l0, nlines);
+    fft_ct<N, G, NT, true>(sm, tw);
+    const float* smf = reinterpret_cast<const float*>(sm);
+    double ps[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int r = (2 * e) / N;
+        ps[q] = 0.0;
+        if ((G * N % NT == 0 || e < G * N) && l0 + r < nlines) {
+            const int t = 2 * e - r * N;
+            const int g = r >> 1, c = r & 1;
+            const long line = l0 + r;
+            const int k = (int)(line / nz);
+            const double add = (pbar0 ? pbar0[k] : 0.0) + (pbar1 ? pbar1[k] : 0.0);
+            P2 b;
+            b.x = TP(0);
+            b.y = TP(0);
+            if (base != nullptr) b = *reinterpret_cast<const P2*>(base + line * N + t);
+            const double o0 = (double)b.x + (add + scale * (double)smf[(g * LP + ct_pad(t)) * 2 + c]);
+            const double o1 = (double)b.y + (add + scale * (double)smf[(g * LP + ct_pad(t + 1)) * 2 + c]);
+            *reinterpret_cast<double2*>(P + line * N + t) = make_double2(o0, o1);
+            ps[q] = o0 + o1;
+        }
+    }
+    if (lsum != nullptr) tile_line_sums<N, G, NT, NL>(ps, slot, lsum, l0, nlines);
+}
+
+}  // namespace dnsp

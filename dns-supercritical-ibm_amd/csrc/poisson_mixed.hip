@@ -1,0 +1,453 @@
+// Mixed-precision Poisson solve (FP32 transforms and spectra, FP64
+// tridiagonal arithmetic and FP64 (0,0) mode, FP64 residual refinement) and
+// the FP64 residual norm.  Kernels of the sweeps are in poisson_mixed.h; the
+// operator is the one the FP64 solve of poisson.hip inverts
+// (poisson_solver_multigpu.f90:132-155 eigenvalues, :395-422 bands).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+#include "poisson_ct.h"
+#include "poisson_ct_launch.h"
+#include "poisson_internal.h"
+
+namespace dnsp {
+
+// Per-plane constants of a refinement residual: the iterate is p = pt + pbar(k)
+// with pt a field whose FP64 line sums are lsum.  With m(k) = pbar(k) +
+// (sum of plane k of pt)/(nx*nz), the plane means of p,
+//   q(k) = (L_y pbar)(k) + ((L00_y - L_y) m)(k)
+// (L_y, L00_y: the plain and the (0,0)-mode y band operators), so that
+// L p = (L_xz + L_y) pt + q(k).  pbar may be NULL (zero).  One wavefront per plane.
+__global__ __launch_bounds__(64) void k_mixed_q(double* __restrict__ q, const double* __restrict__ lsum,
+                                                const double* __restrict__ pbar, const double* __restrict__ bands,
+                                                int nz, int ny, double inv_nxnz) {
+    const int k = (int)blockIdx.x;
+    const int lane = (int)threadIdx.x;
+    double m[3];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        const int kk = k - 1 + u;
+        const bool in = kk >= 0 && kk < ny;
+        double acc = 0.0;
+        if (in)
+            for (int j = lane; j < nz; j += 64) acc += lsum[(size_t)kk * nz + j];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        m[u] = in ? acc * inv_nxnz + (pbar ? pbar[kk] : 0.0) : 0.0;
+    }
+    if (lane == 0) {
+        const double* a = bands;
+        const double* b = bands + ny;
+        const double* c = bands + 2 * ny;
+        const double* a0 = bands + 3 * ny;
+        const double* b0 = bands + 4 * ny;
+        const double* c0 = bands + 5 * ny;
+        double v = 0.0;
+        if (pbar) {
+            v += b[k] * pbar[k];
+            if (k > 0) v += a[k] * pbar[k - 1];
+            if (k < ny - 1) v += c[k] * pbar[k + 1];
+        }
+        v += (b0[k] - b[k]) * m[1];
+        if (k > 0) v += (a0[k] - a[k]) * m[0];
+        if (k < ny - 1) v += (c0[k] - c[k]) * m[2];
+        q[k] = v;
+    }
+}
+
+// FP64 sums of the x lines of P (one wavefront per line).
+__global__ __launch_bounds__(256) void k_line_sums(const double* __restrict__ P, double* __restrict__ lsum, int nx,
+                                                   long nlines) {
+    const long line = (long)blockIdx.x * 4 + (long)(threadIdx.x / 64);
+    const int lane = (int)threadIdx.x & 63;
+    if (line >= nlines) return;  // wavefront-uniform
+    double acc = 0.0;
+    for (int i = lane; i < nx; i += 64) acc += P[line * nx + i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) lsum[line] = acc;
+}
+
+// Partial sums of (f - L p)^2 and f^2: grid kResBlocks x 256, fixed order.
+// L p = periodic x, z second differences + plain y bands + q(k).
+__global__ __launch_bounds__(256) void k_resid_norm(const double* __restrict__ p, const double* __restrict__ f,
+                                                    const double* __restrict__ q, const double* __restrict__ bands,
+                                                    int nx, int nz, int ny, double ix2, double iz2,
+                                                    double* __restrict__ part) {
+    __shared__ double sr[4], sf[4];
+    const long plane = (long)nx * nz;
+    const long n = plane * ny;
+    double ar = 0.0, af = 0.0;
+    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (long)gridDim.x * 256) {
+        const int i = (int)(idx % nx);
+        const long line = idx / nx;
+        const int j = (int)(line % nz);
+        const int k = (int)(line / nz);
+        const double c = p[idx];
+        const double xm = p[idx + (i == 0 ? nx - 1 : -1)];
+        const double xp = p[idx + (i == nx - 1 ? 1 - nx : 1)];
+        const double zm = p[idx + (j == 0 ? (long)(nz - 1) * nx : -(long)nx)];
+        const double zp = p[idx + (j == nz - 1 ? -(long)(nz - 1) * nx : (long)nx)];
+        double v = ix2 * (xp - 2 * c + xm) + iz2 * (zp - 2 * c + zm) + bands[ny + k] * c;
+        if (k > 0) v += bands[k] * p[idx - plane];
+        if (k < ny - 1) v += bands[2 * ny + k] * p[idx + plane];
+        const double fv = f[idx];
+        const double r = fv - (v + q[k]);
+        ar += r * r;
+        af += fv * fv;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        ar += __shfl_xor(ar, o, 64);
+        af += __shfl_xor(af, o, 64);
+    }
+    const int w = (int)threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+        sr[w] = ar;
+        sf[w] = af;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = (sr[0] + sr[1]) + (sr[2] + sr[3]);
+        part[2 * blockIdx.x + 1] = (sf[0] + sf[1]) + (sf[2] + sf[3]);
+    }
+}
+
+static int make_twiddles32(int n, float2** dev) {
+    std::vector<float2> tw((size_t)n);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int t = 0; t < n; ++t) {
+        const long double ang = 2.0L * pi * (long double)t / (long double)n;
+        tw[(size_t)t] = make_float2((float)cosl(ang), (float)(-sinl(ang)));
+    }
+    DNSP_HIP_TRY(hipMalloc(dev, sizeof(float2) * (size_t)n));
+    DNSP_HIP_TRY(hipMemcpy(*dev, tw.data(), sizeof(float2) * (size_t)n, hipMemcpyHostToDevice));
+    return DNSP_OK;
+}
+
+static size_t small_len(const dnsp_poisson* h) { return 4 * (size_t)h->ny + 2 * (size_t)kResBlocks; }
+
+int mixed_setup(dnsp_poisson* h) {
+    const size_t nlines = (size_t)h->nz * h->ny;
+    if (!h->d_small) {
+        DNSP_HIP_TRY(hipMalloc(&h->d_small, sizeof(double) * small_len(h)));
+        DNSP_HIP_TRY(hipMemset(h->d_small, 0, sizeof(double) * small_len(h)));
+        DNSP_HIP_TRY(hipMalloc(&h->d_lsA, sizeof(double) * nlines));
+        DNSP_HIP_TRY(hipMalloc(&h->d_lsB, sizeof(double) * nlines));
+    }
+    if (h->precision != DNSP_PRECISION_MIXED) return DNSP_OK;
+    h->mrows = part_rows_for(h->ny);
+    if (!ct_fft_supported(h->nx) || !ct_fft_supported(h->nz) || h->mrows == 0) {
+        set_error("mixed-precision form: nx=%d and nz=%d must be in {32,64,96,128,192,256,384,512,768} "
+                  "and ny=%d <= %d",
+                  h->nx, h->nz, h->ny, 32 * kPartChunks);
+        return DNSP_ERR_UNSUPPORTED;
+    }
+    const size_t N = (size_t)h->nx * h->nz * h->ny;
+    int rc = make_twiddles32(h->nx, &h->d_twx32);
+    if (!rc) rc = make_twiddles32(h->nz, &h->d_twz32);
+    if (rc) return rc;
+    DNSP_HIP_TRY(hipMalloc(&h->d_A, sizeof(float) * N));
+    DNSP_HIP_TRY(hipMalloc(&h->d_B, sizeof(float) * N));
+    h->mtvar = 3;
+    return DNSP_OK;
+}
+
+void mixed_release(dnsp_poisson* h) {
+    (void)hipFree(h->d_twx32);
+    (void)hipFree(h->d_twz32);
+    (void)hipFree(h->d_A);
+    (void)hipFree(h->d_B);
+    (void)hipFree(h->d_lsA);
+    (void)hipFree(h->d_lsB);
+    (void)hipFree(h->d_small);
+    (void)hipFree(h->d_fcopy);
+}
+
+namespace {
+
+struct Small {
+    double *msum, *pbar0, *pbar1, *q, *part;
+};
+
+Small small_of(const dnsp_poisson* h) {
+    double* s = h->d_small;
+    const int ny = h->ny;
+    return {s, s + ny, s + 2 * ny, s + 3 * ny, s + 4 * ny};
+}
+
+double inv_nxnz_of(const dnsp_poisson* h) { return 1.0 / ((double)h->nx * (double)h->nz); }
+
+MxArgs base_args(const dnsp_poisson* h) {
+    MxArgs a;
+    a.nz = h->nz;
+    a.ny = h->ny;
+    a.ix2 = h->inv_dx * h->inv_dx;
+    a.iz2 = h->inv_dz * h->inv_dz;
+    a.nlines = (long)h->nz * h->ny;
+    a.tw = h->d_twx32;
+    a.scale = inv_nxnz_of(h);
+    return a;
+}
+
+// Sweep 0: f -> d_A holds the FP32 field pt (line sums in d_lsA) and pbar0 the
+// FP64 plane-mean profile; with `out` it writes out = pt + pbar0 in FP64 instead.
+void sweep0(dnsp_poisson* h, const double* f, double* out, hipStream_t st) {
+    const Small sm = small_of(h);
+    MxArgs a = base_args(h);
+    a.src = f;
+    a.dst = h->d_A;
+    a.lsum = h->d_lsA;
+    mx_x_launch(MX_FWD, h->nx, h->mxvar, a, st);
+    ct_z_launch(h->nz, h->mzvar, false, h->d_A, h->nx, h->ny, h->d_twz32, h->d_lsA, sm.msum, st);
+    part_launch<float>(h->mtvar, h->mrows, h->d_A, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands, h->nx,
+                       h->nz, h->ny, h->pcr_min, h->ncu, sm.msum, sm.pbar0, inv_nxnz_of(h), st);
+    ct_z_launch(h->nz, h->mzvar, true, h->d_A, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
+    MxArgs b = base_args(h);
+    b.src = h->d_A;
+    if (out) {
+        b.base = nullptr;
+        b.pbar0 = sm.pbar0;
+        b.dst = out;
+        mx_x_launch(MX_ACC_F32, h->nx, h->mxvar, b, st);
+    } else {
+        b.dst = h->d_A;
+        b.lsum = h->d_lsA;
+        mx_x_launch(MX_BWD, h->nx, h->mxvar, b, st);
+    }
+}
+
+// One refinement sweep.  first: the iterate is (d_A, pbar0) from sweep 0,
+// else the FP64 P with its line sums in d_lsA.  Writes P; with `sums` also
+// the line sums of P into d_lsA.
+void refine(dnsp_poisson* h, const double* f, double* P, bool first, bool sums, hipStream_t st) {
+    const Small sm = small_of(h);
+    k_mixed_q<<<h->ny, 64, 0, st>>>(sm.q, h->d_lsA, first ? sm.pbar0 : nullptr, h->d_bands, h->nz, h->ny,
+                                    inv_nxnz_of(h));
+    MxArgs a = base_args(h);
+    a.src = first ? static_cast<const void*>(h->d_A) : static_cast<const void*>(P);
+    a.f = f;
+    a.q = sm.q;
+    a.bands = h->d_bands;
+    a.dst = h->d_B;
+    a.lsum = h->d_lsB;
+    mx_x_launch(first ? MX_RES_F32 : MX_RES_F64, h->nx, h->mxvar, a, st);
+    ct_z_launch(h->nz, h->mzvar, false, h->d_B, h->nx, h->ny, h->d_twz32, h->d_lsB, sm.msum, st);
+    part_launch<float>(h->mtvar, h->mrows, h->d_B, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands, h->nx,
+                       h->nz, h->ny, h->pcr_min, h->ncu, sm.msum, sm.pbar1, inv_nxnz_of(h), st);
+    ct_z_launch(h->nz, h->mzvar, true, h->d_B, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
+    MxArgs b = base_args(h);
+    b.src = h->d_B;
+    b.base = first ? static_cast<const void*>(h->d_A) : static_cast<const void*>(P);
+    b.pbar0 = first ? sm.pbar0 : nullptr;
+    b.pbar1 = sm.pbar1;
+    b.dst = P;
+    b.lsum = sums ? h->d_lsA : nullptr;
+    mx_x_launch(first ? MX_ACC_F32 : MX_ACC_F64, h->nx, h->mxvar, b, st);
+}
+
+// rel = ||f - L P|| / ||f|| (||f - L P|| when f = 0), with P's line sums in
+// lsum.  Synchronises the stream.
+int resid_norm(dnsp_poisson* h, const double* f, const double* P, const double* lsum, double* rel, hipStream_t st) {
+    const Small sm = small_of(h);
+    double* q = sm.q;
+    k_mixed_q<<<h->ny, 64, 0, st>>>(q, lsum, nullptr, h->d_bands, h->nz, h->ny, inv_nxnz_of(h));
+    k_resid_norm<<<kResBlocks, 256, 0, st>>>(P, f, q, h->d_bands, h->nx, h->nz, h->ny, h->inv_dx * h->inv_dx,
+                                             h->inv_dz * h->inv_dz, sm.part);
+    DNSP_LAUNCH_CHECK();
+    std::vector<double> part(2 * (size_t)kResBlocks);
+    DNSP_HIP_TRY(hipMemcpyAsync(part.data(), sm.part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+    DNSP_HIP_TRY(hipStreamSynchronize(st));
+    long double sr = 0.0L, sf = 0.0L;
+    for (int b = 0; b < kResBlocks; ++b) {
+        sr += part[2 * (size_t)b];
+        sf += part[2 * (size_t)b + 1];
+    }
+    *rel = (double)(sf > 0 ? sqrtl(sr / sf) : sqrtl(sr));
+    return DNSP_OK;
+}
+
+int check_mixed(const dnsp_poisson* h) {
+    DNSP_ARG_CHECK(h, "NULL handle");
+    if (h->precision != DNSP_PRECISION_MIXED) {
+        set_error("handle was created with precision %d; the mixed-precision form needs DNSP_PRECISION_MIXED",
+                  h->precision);
+        return DNSP_ERR_ARG;
+    }
+    return DNSP_OK;
+}
+
+// The right-hand side the sweeps read: rhs itself, or a copy when the solution
+// overwrites it before a later sweep reads it again.
+int rhs_for(dnsp_poisson* h, const double* rhs, const double* P, bool reread, hipStream_t st, const double** f) {
+    *f = rhs;
+    if (rhs != P || !reread) return DNSP_OK;
+    const size_t bytes = sizeof(double) * (size_t)h->nx * h->nz * h->ny;
+    if (!h->d_fcopy) DNSP_HIP_TRY(hipMalloc(&h->d_fcopy, bytes));
+    DNSP_HIP_TRY(hipMemcpyAsync(h->d_fcopy, rhs, bytes, hipMemcpyDeviceToDevice, st));
+    *f = h->d_fcopy;
+    return DNSP_OK;
+}
+
+}  // namespace
+
+int mixed_autotune(dnsp_poisson* h, const double* f, int reps, hipStream_t st, double* times_ms) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        set_error("autotune: event creation failed");
+        return DNSP_ERR_HIP;
+    }
+    int rc = DNSP_OK;
+    int pick[3] = {h->mxvar, h->mzvar, h->mtvar};
+    double best[3] = {1e30, 1e30, 1e30};
+    for (int dir = 0; dir < 3 && rc == DNSP_OK; ++dir) {
+        for (int v = 0; v < 4 && rc == DNSP_OK; ++v) {
+            auto pair = [&]() {
+                if (dir == 0) {
+                    MxArgs a = base_args(h);
+                    a.src = f;
+                    a.dst = h->d_A;
+                    a.lsum = h->d_lsA;
+                    mx_x_launch(MX_FWD, h->nx, v, a, st);
+                    MxArgs b = base_args(h);
+                    b.src = h->d_A;
+                    b.dst = h->d_A;
+                    b.lsum = h->d_lsA;
+                    mx_x_launch(MX_BWD, h->nx, v, b, st);
+                } else if (dir == 1) {
+                    ct_z_launch(h->nz, v, false, h->d_A, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
+                    ct_z_launch(h->nz, v, true, h->d_A, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
+                } else {
+                    for (int r = 0; r < 2; ++r)
+                        part_launch<float>(v, h->mrows, h->d_A, h->d_ax, h->d_az, h->d_is0x, h->d_is0z,
+                                           h->d_bands, h->nx, h->nz, h->ny, h->pcr_min, h->ncu, nullptr, nullptr,
+                                           0.0, st);
+                }
+            };
+            pair();  // warm-up
+            (void)hipEventRecord(e0, st);
+            for (int r = 0; r < reps; ++r) pair();
+            (void)hipEventRecord(e1, st);
+            if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                set_error("mixed autotune: launch or synchronize failed");
+                rc = DNSP_ERR_HIP;
+                break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double per = (double)ms / reps;
+            if (times_ms) times_ms[dir * 4 + v] = per;
+            if (per < best[dir]) {
+                best[dir] = per;
+                pick[dir] = v;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    h->mxvar = pick[0];
+    h->mzvar = pick[1];
+    h->mtvar = pick[2];
+    return DNSP_OK;
+}
+
+}  // namespace dnsp
+
+using namespace dnsp;
+
+extern "C" {
+
+int dnsp_poisson_solve_mixed(dnsp_poisson* h, const double* rhs, double* P, int refinements, void* stream) {
+    clear_error();
+    int rc = check_mixed(h);
+    if (rc) return rc;
+    DNSP_ARG_CHECK(rhs && P, "NULL array");
+    DNSP_ARG_CHECK(refinements >= 0 && refinements <= DNSP_MAX_REFINEMENTS, "refinements=%d not in [0, %d]",
+                   refinements, DNSP_MAX_REFINEMENTS);
+    hipStream_t st = as_stream(stream);
+    const double* f = rhs;
+    rc = rhs_for(h, rhs, P, refinements > 1, st, &f);
+    if (rc) return rc;
+    sweep0(h, f, refinements == 0 ? P : nullptr, st);
+    for (int r = 0; r < refinements; ++r) refine(h, f, P, r == 0, r + 1 < refinements, st);
+    DNSP_LAUNCH_CHECK();
+    return DNSP_OK;
+}
+
+int dnsp_poisson_run_mixed(dnsp_poisson* h, double* P, int refinements, void* stream) {
+    return dnsp_poisson_solve_mixed(h, P, P, refinements, stream);
+}
+
+int dnsp_poisson_solve_refine(dnsp_poisson* h, const double* rhs, double* P, double tol, int max_refinements,
+                              int* refinements_done, double* rel_resid, void* stream) {
+    clear_error();
+    int rc = check_mixed(h);
+    if (rc) return rc;
+    DNSP_ARG_CHECK(rhs && P, "NULL array");
+    DNSP_ARG_CHECK(tol >= 0 && std::isfinite(tol), "tol must be finite and >= 0");
+    DNSP_ARG_CHECK(max_refinements >= 0 && max_refinements <= DNSP_MAX_REFINEMENTS,
+                   "max_refinements=%d not in [0, %d]", max_refinements, DNSP_MAX_REFINEMENTS);
+    hipStream_t st = as_stream(stream);
+    const double* f = rhs;
+    rc = rhs_for(h, rhs, P, true, st, &f);
+    if (rc) return rc;
+    double rel = 0.0;
+    int done = 0;
+    if (max_refinements == 0) {
+        sweep0(h, f, P, st);
+        k_line_sums<<<(int)(((long)h->nz * h->ny + 3) / 4), 256, 0, st>>>(P, h->d_lsA, h->nx, (long)h->nz * h->ny);
+        rc = resid_norm(h, f, P, h->d_lsA, &rel, st);
+    } else {
+        sweep0(h, f, nullptr, st);
+        for (int r = 0; r < max_refinements; ++r) {
+            refine(h, f, P, r == 0, true, st);
+            done = r + 1;
+            rc = resid_norm(h, f, P, h->d_lsA, &rel, st);
+            if (rc || rel <= tol) break;
+        }
+    }
+    if (rc) return rc;
+    if (refinements_done) *refinements_done = done;
+    if (rel_resid) *rel_resid = rel;
+    return DNSP_OK;
+}
+
+int dnsp_poisson_residual(dnsp_poisson* h, const double* rhs, const double* P, double* rel_resid, void* stream) {
+    clear_error();
+    DNSP_ARG_CHECK(h && rhs && P && rel_resid, "NULL handle, array or output");
+    hipStream_t st = as_stream(stream);
+    const long nlines = (long)h->nz * h->ny;
+    k_line_sums<<<(int)((nlines + 3) / 4), 256, 0, st>>>(P, h->d_lsB, h->nx, nlines);
+    return resid_norm(h, rhs, P, h->d_lsB, rel_resid, st);
+}
+
+int dnsp_poisson_set_mixed_variants(dnsp_poisson* h, int xvar, int zvar, int tvar) {
+    clear_error();
+    int rc = check_mixed(h);
+    if (rc) return rc;
+    DNSP_ARG_CHECK(xvar >= 0 && xvar < kCtVariants && zvar >= 0 && zvar < kCtVariants,
+                   "FFT kernel variants must be in [0, %d)", kCtVariants);
+    DNSP_ARG_CHECK(tvar >= 0 && tvar < kPartVariants, "tridiagonal variant must be in [0, %d)", kPartVariants);
+    h->mxvar = xvar;
+    h->mzvar = zvar;
+    h->mtvar = tvar;
+    return DNSP_OK;
+}
+
+int dnsp_poisson_get_mixed_variants(const dnsp_poisson* h, int* xvar, int* zvar, int* tvar) {
+    clear_error();
+    int rc = check_mixed(h);
+    if (rc) return rc;
+    DNSP_ARG_CHECK(xvar && zvar && tvar, "NULL argument");
+    *xvar = h->mxvar;
+    *zvar = h->mzvar;
+    *tvar = h->mtvar;
+    return DNSP_OK;
+}
+
+}  // extern "C"

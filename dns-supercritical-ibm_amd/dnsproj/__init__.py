@@ -29,6 +29,10 @@ DNSP_ERR_UNSUPPORTED = 5
 FFT_AUTO, FFT_GENERIC, FFT_FIXED = 0, 1, 2
 TRIDIAG_AUTO, TRIDIAG_SERIAL, TRIDIAG_PARTITION = 0, 1, 2
 
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+MAX_REFINEMENTS = 8
+TUNE_SLOTS = 24
+
 STAGE_X_FWD, STAGE_Z_FWD, STAGE_TRIDIAG, STAGE_Z_BWD, STAGE_X_BWD = range(5)
 STAGE_NAMES = ("x_fwd", "z_fwd", "tridiag", "z_bwd", "x_bwd")
 
@@ -65,7 +69,13 @@ _SIGNATURES = [
     ("dnsp_poisson_get_strategy", [_vp, _ip, _ip, _ip], ctypes.c_int),
     ("dnsp_poisson_set_variants", [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     ("dnsp_poisson_get_variants", [_vp, _ip, _ip, _ip], ctypes.c_int),
+    ("dnsp_poisson_set_mixed_variants", [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    ("dnsp_poisson_get_mixed_variants", [_vp, _ip, _ip, _ip], ctypes.c_int),
     ("dnsp_poisson_autotune", [_vp, _vp, ctypes.c_int, _vp, _dp], ctypes.c_int),
+    ("dnsp_poisson_solve_mixed", [_vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+    ("dnsp_poisson_run_mixed", [_vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+    ("dnsp_poisson_solve_refine", [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _ip, _dp, _vp], ctypes.c_int),
+    ("dnsp_poisson_residual", [_vp, _vp, _vp, _dp, _vp], ctypes.c_int),
     ("dnsp_poisson_min_pivot", [_vp, _dp], ctypes.c_int),
     ("dnsp_poisson_shape", [_vp, _ip, _ip, _ip], ctypes.c_int),
     ("dnsp_poisson_free", [_vp], ctypes.c_int),
@@ -147,10 +157,12 @@ def _device_ptr(P, numel):
 
 
 class PoissonSolver:
-    """One single-rank FP64 Poisson solver instance (the C handle).
+    """One single-rank Poisson solver instance (the C handle).
 
     ``bands`` = (P_band_a, P_band_b, P_band_c, P_band_a_00, P_band_b_00, P_band_c_00).
     Arrays are (ny, nz, nx) C-order tensors, i.e. the reference's x-fastest P.
+    ``precision=PRECISION_MIXED`` adds the mixed-precision form
+    (``solve_mixed``, ``solve_refine``) to the FP64 one.
     """
 
     def __init__(self, nx, nz, ny, inv_dx, inv_dz, bands, precision=0):
@@ -165,6 +177,7 @@ class PoissonSolver:
         self._bands = bs  # keep alive
         self.nx, self.nz, self.ny = int(nx), int(nz), int(ny)
         self.inv_dx, self.inv_dz = float(inv_dx), float(inv_dz)
+        self.precision = int(precision)
 
     @property
     def shape(self):
@@ -192,14 +205,56 @@ class PoissonSolver:
         _check(lib().dnsp_poisson_get_variants(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return (a.value, b.value, c.value)
 
+    def set_mixed_variants(self, xvar, zvar, tvar=0):
+        _check(lib().dnsp_poisson_set_mixed_variants(self._h, int(xvar), int(zvar), int(tvar)))
+
+    @property
+    def mixed_variants(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().dnsp_poisson_get_mixed_variants(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return (a.value, b.value, c.value)
+
     def autotune(self, work=None, reps=5, stream=None):
         """Picks the fastest kernel variants; returns the timing table (ms per pair)."""
-        times = np.zeros(10)
+        times = np.zeros(TUNE_SLOTS)
         wp = _vp(_device_ptr(work, self.numel)) if work is not None else _vp(None)
         _check(lib().dnsp_poisson_autotune(self._h, wp, int(reps), _vp(_stream_ptr(stream)),
                                            times.ctypes.data_as(_dp)))
-        return {"x_ms": times[:4].tolist(), "z_ms": times[4:8].tolist(), "tri_ms": times[8:].tolist(),
-                "picked": self.variants}
+        out = {"x_ms": times[:4].tolist(), "z_ms": times[4:8].tolist(), "tri_ms": times[8:12].tolist(),
+               "picked": self.variants}
+        if self.precision == PRECISION_MIXED:
+            out.update({"mixed_x_ms": times[12:16].tolist(), "mixed_z_ms": times[16:20].tolist(),
+                        "mixed_tri_ms": times[20:24].tolist(), "mixed_picked": self.mixed_variants})
+        return out
+
+    def solve_mixed(self, rhs, P, refinements=1, stream=None):
+        """Mixed-precision solve (FP32 transforms + FP64 refinement): rhs -> P (rhs may be P)."""
+        if self._h is None:
+            raise RuntimeError("solver has been freed")
+        _check(lib().dnsp_poisson_solve_mixed(self._h, _vp(_device_ptr(rhs, self.numel)),
+                                              _vp(_device_ptr(P, self.numel)), int(refinements),
+                                              _vp(_stream_ptr(stream))))
+        return P
+
+    def run_mixed(self, P, refinements=1, stream=None):
+        return self.solve_mixed(P, P, refinements, stream)
+
+    def solve_refine(self, rhs, P, tol=1e-10, max_refinements=4, stream=None):
+        """Refines until ||rhs - L P||/||rhs|| <= tol; returns (refinements done, relative residual)."""
+        done, rel = ctypes.c_int(), ctypes.c_double()
+        _check(lib().dnsp_poisson_solve_refine(self._h, _vp(_device_ptr(rhs, self.numel)),
+                                               _vp(_device_ptr(P, self.numel)), float(tol),
+                                               int(max_refinements), ctypes.byref(done), ctypes.byref(rel),
+                                               _vp(_stream_ptr(stream))))
+        return done.value, rel.value
+
+    def residual(self, rhs, P, stream=None):
+        """||rhs - L P|| / ||rhs|| of the discrete operator (FP64, on the device)."""
+        rel = ctypes.c_double()
+        _check(lib().dnsp_poisson_residual(self._h, _vp(_device_ptr(rhs, self.numel)),
+                                           _vp(_device_ptr(P, self.numel)), ctypes.byref(rel),
+                                           _vp(_stream_ptr(stream))))
+        return rel.value
 
     @property
     def min_pivot(self):

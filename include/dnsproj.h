@@ -47,11 +47,16 @@ int dnsp_stream_synchronize(void* stream);
  *   inv_dx, inv_dz   : 1/dx, 1/dz (the reference's inv_dx, inv_dz)
  *   P_band_*         : y bands (length ny) as read from geom_data/P_band_abc.dat,
  *   P_band_*_00      : and the (0,0)-mode set from P_band_abc_00.dat
- *   precision        : 0 = plain FP64 (the only form of this entry point)
+ *   precision        : DNSP_PRECISION_FP64 (0): the plain FP64 form, as the
+ *                      reference; DNSP_PRECISION_MIXED (1): the same, plus the
+ *                      state of the mixed-precision form (two FP32 fields;
+ *                      nx, nz in {32,64,96,128,192,256,384,512,768}, ny <= 1024)
  * The eigenvalues a_x, a_z (:132-155) and the FFT twiddles are computed on the
  * host; the reference's poi_C array (:162-179) is not stored.
  * On success *out owns device memory on the current HIP device.
  */
+#define DNSP_PRECISION_FP64   0
+#define DNSP_PRECISION_MIXED  1
 int dnsp_poisson_init(int nx_glob, int nz_glob, int ny, double inv_dx, double inv_dz,
                       const double* P_band_a, const double* P_band_b, const double* P_band_c,
                       const double* P_band_a_00, const double* P_band_b_00, const double* P_band_c_00,
@@ -122,19 +127,56 @@ int dnsp_poisson_get_strategy(const dnsp_poisson* h, int* fft_ct_x, int* fft_ct_
 /* Kernel variants.  xvar, zvar (0..3): the compile-time-sized FFT kernels'
  * (complex elements per workgroup tile, threads) = (3072,256), (3072,512),
  * (1536,256), (6144,512); ignored where a length has no such kernels.
- * tvar (0..1): partitioned tridiagonal workgroups of 16 or 8 modes (x 32
- * chunks).  No counterpart in the reference. */
+ * tvar (0..3): partitioned tridiagonal workgroups of 16 or 8 modes (x 32
+ * chunks), one per mode group (0, 1); 8 modes with registers capped for 4
+ * resident waves per SIMD (2); 8 modes, persistent (3: as many workgroups as
+ * are resident at once, each looping over mode groups).  The
+ * mixed-precision form has its own set.  No counterpart in the reference. */
 int dnsp_poisson_set_variants(dnsp_poisson* h, int xvar, int zvar, int tvar);
 int dnsp_poisson_get_variants(const dnsp_poisson* h, int* xvar, int* zvar, int* tvar);
+int dnsp_poisson_set_mixed_variants(dnsp_poisson* h, int xvar, int zvar, int tvar);
+int dnsp_poisson_get_mixed_variants(const dnsp_poisson* h, int* xvar, int* zvar, int* tvar);
 
 /* Autotuner: times `reps` forward+inverse pairs of every FFT kernel variant
  * of each direction, and pairs of tridiagonal solves of every partitioned
  * variant, on `work` (nx*nz*ny device doubles, overwritten; NULL = a
- * temporary allocation) and keeps the fastest in the handle.  times_ms
- * (optional, 10 doubles) receives the mean ms per pair: 4 x variants, 4 z
- * variants, 2 tridiagonal variants; -1 where not applicable.  Synchronises
- * `stream`. */
+ * temporary allocation) and keeps the fastest in the handle; for a
+ * DNSP_PRECISION_MIXED handle the same for the mixed-precision kernels.
+ * times_ms (optional, DNSP_TUNE_SLOTS doubles) receives the mean ms per pair:
+ * 4 x, 4 z and 4 tridiagonal variants of the FP64 form, then of the mixed
+ * form; -1 where not applicable.  Synchronises `stream`. */
+#define DNSP_TUNE_SLOTS 24
 int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream, double* times_ms);
+
+/*
+ * Mixed-precision form of the solve (no counterpart in the reference, whose
+ * solve is FP64 throughout; it solves the same discrete system).  Sweep 0:
+ * FP32 x and z transforms and FP32 spectra, FP64 arithmetic in the y
+ * tridiagonal solves, and the (0,0) mode (the mean profile, pinned by the
+ * P_band_*_00 set) solved entirely in FP64 from FP64 plane sums.  Each
+ * refinement sweep evaluates the FP64 residual r = rhs - L p inside the
+ * forward x transform, solves L d = r the same way and adds d in FP64.
+ * One refinement reaches a relative residual ||rhs - L p||/||rhs|| of about
+ * 1e-11 on channel grids; the handle must be DNSP_PRECISION_MIXED.
+ *   dnsp_poisson_solve_mixed: `refinements` sweeps (0..DNSP_MAX_REFINEMENTS)
+ *     after sweep 0; rhs may equal P (a copy of rhs is then kept when
+ *     refinements > 1).  Enqueued on `stream`, no synchronisation.
+ *   dnsp_poisson_run_mixed: in place, P = rhs on entry.
+ *   dnsp_poisson_solve_refine: refines until the relative residual is <= tol
+ *     or max_refinements sweeps are done; returns the sweeps done and the
+ *     residual (each check reads P and rhs once more).  Synchronises.
+ */
+#define DNSP_MAX_REFINEMENTS 8
+int dnsp_poisson_solve_mixed(dnsp_poisson* h, const double* rhs, double* P, int refinements, void* stream);
+int dnsp_poisson_run_mixed(dnsp_poisson* h, double* P, int refinements, void* stream);
+int dnsp_poisson_solve_refine(dnsp_poisson* h, const double* rhs, double* P, double tol, int max_refinements,
+                              int* refinements_done, double* rel_resid, void* stream);
+
+/* Relative residual ||rhs - L P|| / ||rhs|| (||rhs - L P|| if rhs = 0) of the
+ * discrete operator the solve inverts: periodic second differences in x and
+ * z, the plain y bands, and the (0,0)-mode bands on the xz-plane means.  FP64,
+ * any precision of handle.  Synchronises `stream`. */
+int dnsp_poisson_residual(dnsp_poisson* h, const double* rhs, const double* P, double* rel_resid, void* stream);
 
 /* min |pivot| over all modes and rows: the init diagnostic min_abs_div_bacp
  * (poisson_solver_multigpu.f90:181-222), computed on the device at init. */
