@@ -1,0 +1,747 @@
+// Batched Poisson solves of many independent instances of uneven sizes.
+//
+// Every stage of the solve is one launch per distinct transform length (x, z)
+// or rows-per-chunk (tridiagonal) covering the tiles of all instances of that
+// size: a workgroup decodes its task through a prefix table (instance, tile)
+// and runs the same tile bodies as the single-instance kernels.  A batch of
+// 48 instances over 5 x lengths, 5 z lengths and 4 row counts thus takes 24
+// launches per FP64 solve instead of 240.  The FP64 form and the
+// mixed-precision form (sweep 0 and one FP64 residual refinement) are
+// provided; the launch sequence allocates nothing and can be captured in a
+// graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <type_traits>
+#include <vector>
+
+#include "common.h"
+#include "poisson_ct.h"
+#include "poisson_ct_launch.h"
+#include "poisson_mixed.h"
+
+namespace dnsp {
+
+// Device description of one instance.
+struct BInst {
+    int nx, nz, ny, mr;
+    long nlines, plane;
+    double ix2, iz2, pcr_min, inv_nxnz;
+    const double* ax;
+    const double* az;
+    const int* i0x;
+    const int* i0z;
+    const double* bands;
+    const double2* twx;
+    const double2* twz;
+    const float2* twx32;
+    const float2* twz32;
+    const double* f;  // bound right-hand side
+    double* P;        // bound solution
+    float* A;         // mixed form: FP32 fields, line sums, msum | pbar0 | pbar1 | q
+    float* B;
+    double* lsA;
+    double* lsB;
+    double* small;
+};
+
+// Launch group: the instances of one size, their task offsets.
+struct BGroup {
+    int key = 0;  // transform length or rows per chunk
+    int cnt = 0;
+    long total = 0;
+    int* ids = nullptr;     // device, cnt
+    long* start = nullptr;  // device, cnt
+};
+
+__device__ __forceinline__ int find_inst(const long* __restrict__ start, int cnt, long task) {
+    int lo = 0, hi = cnt - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= task) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+enum BxKind { BX_F64_FWD = 0, BX_F64_BWD, BX_MX_FWD, BX_MX_RES, BX_MX_BWD, BX_MX_ACC, BX_MX_OUT0 };
+
+template <int N, int V, int KIND>
+__global__ __launch_bounds__(ct_var_threads(V)) void kb_x(const BInst* __restrict__ inst,
+                                                          const int* __restrict__ ids,
+                                                          const long* __restrict__ start, int cnt, int sums) {
+    constexpr int G = ct_lines<N, ct_var_tile(V)>();
+    constexpr int NT = ct_var_threads(V);
+    constexpr bool F64 = KIND == BX_F64_FWD || KIND == BX_F64_BWD;
+    using VT = typename std::conditional<F64, double2, float2>::type;
+    __shared__ VT sm[G * ct_pitch(N)];
+    __shared__ double slot[F64 ? 1 : line_slots<N, G>()];
+    const long task = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int u = find_inst(start, cnt, task);
+    const BInst& b = inst[ids[u]];
+    const long tile = task - start[u];
+    if constexpr (KIND == BX_F64_FWD) {
+        xfwd_body<N, G, NT>(b.f, b.P, b.nlines, b.twx, tile, sm);
+    } else if constexpr (KIND == BX_F64_BWD) {
+        xbwd_body<N, G, NT>(b.P, b.P, b.nlines, b.twx, b.inv_nxnz, tile, sm);
+    } else if constexpr (KIND == BX_MX_FWD) {
+        xfwd_mf_body<N, G, NT>(b.f, b.A, b.lsA, b.nlines, b.twx32, tile, sm, slot);
+    } else if constexpr (KIND == BX_MX_RES) {
+        xfwd_mr_body<N, G, NT, float>(b.A, b.f, b.small + 3 * b.ny, b.bands, b.B, b.lsB, b.nz, b.ny, b.ix2, b.iz2,
+                                      b.nlines, b.twx32, tile, sm, slot);
+    } else if constexpr (KIND == BX_MX_BWD) {
+        xbwd_mf_body<N, G, NT>(b.A, b.A, b.lsA, b.nlines, b.twx32, (float)b.inv_nxnz, tile, sm, slot);
+    } else if constexpr (KIND == BX_MX_ACC) {
+        xbwd_ma_body<N, G, NT, float>(b.B, b.A, b.small + b.ny, b.small + 2 * b.ny, b.P, sums ? b.lsA : nullptr,
+                                      b.nz, b.nlines, b.twx32, b.inv_nxnz, tile, sm, slot);
+    } else {  // BX_MX_OUT0: sweep 0 written straight to P (no refinement)
+        xbwd_ma_body<N, G, NT, float>(b.A, nullptr, b.small + b.ny, nullptr, b.P, nullptr, b.nz, b.nlines,
+                                      b.twx32, b.inv_nxnz, tile, sm, slot);
+    }
+}
+
+enum BzKind { BZ_F64_FWD = 0, BZ_F64_BWD, BZ_A_FWD, BZ_A_BWD, BZ_B_FWD, BZ_B_BWD };
+
+template <int N, int V, int KIND>
+__global__ __launch_bounds__(ct_var_threads(V)) void kb_z(const BInst* __restrict__ inst,
+                                                          const int* __restrict__ ids,
+                                                          const long* __restrict__ start, int cnt) {
+    constexpr int TA = ct_cols<N, ct_var_tile(V)>();
+    constexpr int NT = ct_var_threads(V);
+    constexpr bool F64 = KIND == BZ_F64_FWD || KIND == BZ_F64_BWD;
+    using T = typename std::conditional<F64, double, float>::type;
+    using VT = typename Cx<T>::V;
+    __shared__ VT sm[TA * ct_pitch(N)];
+    const long task = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int u = find_inst(start, cnt, task);
+    const BInst& b = inst[ids[u]];
+    const int tile = (int)(task - start[u]);
+    const int ntc = ((b.nx + 1) / 2 + TA - 1) / TA;
+    if constexpr (KIND == BZ_F64_FWD) zfwd_body<N, TA, NT, double>(b.P, b.nx, ntc, b.twz, nullptr, nullptr, tile, sm);
+    else if constexpr (KIND == BZ_F64_BWD) zbwd_body<N, TA, NT, double>(b.P, b.nx, ntc, b.twz, tile, sm);
+    else if constexpr (KIND == BZ_A_FWD) zfwd_body<N, TA, NT, float>(b.A, b.nx, ntc, b.twz32, b.lsA, b.small, tile, sm);
+    else if constexpr (KIND == BZ_A_BWD) zbwd_body<N, TA, NT, float>(b.A, b.nx, ntc, b.twz32, tile, sm);
+    else if constexpr (KIND == BZ_B_FWD) zfwd_body<N, TA, NT, float>(b.B, b.nx, ntc, b.twz32, b.lsB, b.small, tile, sm);
+    else zbwd_body<N, TA, NT, float>(b.B, b.nx, ntc, b.twz32, tile, sm);
+}
+
+// Tridiagonal: one mode group of 8 modes per task.  KIND 0: FP64 on P;
+// 1: FP32 on A with the (0,0) mode -> pbar0; 2: FP32 on B -> pbar1.
+constexpr int kBatchM = 8;
+
+template <int MR, int KIND, int OCC>
+__global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri(const BInst* __restrict__ inst,
+                                                                      const int* __restrict__ ids,
+                                                                      const long* __restrict__ start, int cnt) {
+    constexpr int M = kBatchM;
+    constexpr int P = kPartChunks;
+    constexpr int NT = M * P;
+    constexpr int NR = 2 * P;
+    using T = typename std::conditional<KIND == 0, double, float>::type;
+    __shared__ double ra[NR][M], rb[NR][M], rc[NR][M], rd[NR][M];
+    extern __shared__ double smb[];
+    const long task = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int u = find_inst(start, cnt, task);
+    const BInst& b = inst[ids[u]];
+    const long grp = task - start[u];
+    const int mi = (int)threadIdx.x % M;
+    const int s = ((int)threadIdx.x / M) * MR;
+    const int nband = (grp == 0 ? 6 : 3) * b.ny;
+    for (int e = (int)threadIdx.x; e < nband; e += NT) smb[e] = b.bands[e];
+    T* S = KIND == 0 ? reinterpret_cast<T*>(b.P) : reinterpret_cast<T*>(KIND == 1 ? b.A : b.B);
+    const double* msum = KIND == 0 ? nullptr : b.small;
+    double* pbar = KIND == 0 ? nullptr : b.small + (KIND == 1 ? 1 : 2) * b.ny;
+    double d[MR];
+    tri_load_rows<MR, P, M, T>(d, S, grp * M + mi, b.plane, b.ny, s, msum);
+    __syncthreads();
+    tri_group<MR, P, M, T>(d, S, b.ax, b.az, b.i0x, b.i0z, smb, b.nx, b.ny, b.plane, b.pcr_min, grp * M + mi, msum,
+                           pbar, b.inv_nxnz, ra, rb, rc, rd);
+}
+
+// q(k) of the first refinement for every plane of every instance (one
+// wavefront per plane; see k_mixed_q).
+__global__ __launch_bounds__(64) void kb_q(const BInst* __restrict__ inst, const int* __restrict__ ids,
+                                           const long* __restrict__ start, int cnt) {
+    const long task = blockIdx.x;
+    const int u = find_inst(start, cnt, task);
+    const BInst& b = inst[ids[u]];
+    const int k = (int)(task - start[u]);
+    const int ny = b.ny, nz = b.nz;
+    const double* pbar = b.small + ny;
+    const int lane = (int)threadIdx.x;
+    double m[3];
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+        const int kk = k - 1 + w;
+        const bool in = kk >= 0 && kk < ny;
+        double acc = 0.0;
+        if (in)
+            for (int j = lane; j < nz; j += 64) acc += b.lsA[(size_t)kk * nz + j];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        m[w] = in ? acc * b.inv_nxnz + pbar[kk] : 0.0;
+    }
+    if (lane == 0) {
+        const double* a = b.bands;
+        const double* bb = a + ny;
+        const double* c = bb + ny;
+        const double* a0 = c + ny;
+        const double* b0 = a0 + ny;
+        const double* c0 = b0 + ny;
+        double v = bb[k] * pbar[k];
+        if (k > 0) v += a[k] * pbar[k - 1];
+        if (k < ny - 1) v += c[k] * pbar[k + 1];
+        v += (b0[k] - bb[k]) * m[1];
+        if (k > 0) v += (a0[k] - a[k]) * m[0];
+        if (k < ny - 1) v += (c0[k] - c[k]) * m[2];
+        b.small[3 * ny + k] = v;
+    }
+}
+
+#define DNSP_CT_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768)
+#define DNSP_PART_ROWS(X) X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32)
+
+template <int N, int V>
+static void bx_launch_nv(int kind, const BInst* inst, const BGroup& g, int sums, hipStream_t st) {
+    const int nb = (int)g.total;
+    constexpr int NT = ct_var_threads(V);
+    switch (kind) {
+        case BX_F64_FWD: kb_x<N, V, BX_F64_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
+        case BX_F64_BWD: kb_x<N, V, BX_F64_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
+        case BX_MX_FWD: kb_x<N, V, BX_MX_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
+        case BX_MX_RES: kb_x<N, V, BX_MX_RES><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
+        case BX_MX_BWD: kb_x<N, V, BX_MX_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
+        case BX_MX_ACC: kb_x<N, V, BX_MX_ACC><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
+        default: kb_x<N, V, BX_MX_OUT0><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
+    }
+}
+
+template <int N>
+static void bx_launch_n(int kind, int v, const BInst* inst, const BGroup& g, int sums, hipStream_t st) {
+    switch (v) {
+        case 0: bx_launch_nv<N, 0>(kind, inst, g, sums, st); break;
+        case 1: bx_launch_nv<N, 1>(kind, inst, g, sums, st); break;
+        case 2: bx_launch_nv<N, 2>(kind, inst, g, sums, st); break;
+        default: bx_launch_nv<N, 3>(kind, inst, g, sums, st); break;
+    }
+}
+
+static void bx_launch(int kind, int v, const BInst* inst, const BGroup& g, int sums, hipStream_t st) {
+    if (g.total <= 0) return;
+    switch (g.key) {
+#define DNSP_CASE(N) case N: bx_launch_n<N>(kind, v, inst, g, sums, st); break;
+        DNSP_CT_SIZES(DNSP_CASE)
+#undef DNSP_CASE
+        default: break;
+    }
+}
+
+template <int N, int V>
+static void bz_launch_nv(int kind, const BInst* inst, const BGroup& g, hipStream_t st) {
+    const int nb = (int)g.total;
+    constexpr int NT = ct_var_threads(V);
+    switch (kind) {
+        case BZ_F64_FWD: kb_z<N, V, BZ_F64_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
+        case BZ_F64_BWD: kb_z<N, V, BZ_F64_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
+        case BZ_A_FWD: kb_z<N, V, BZ_A_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
+        case BZ_A_BWD: kb_z<N, V, BZ_A_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
+        case BZ_B_FWD: kb_z<N, V, BZ_B_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
+        default: kb_z<N, V, BZ_B_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
+    }
+}
+
+template <int N>
+static void bz_launch_n(int kind, int v, const BInst* inst, const BGroup& g, hipStream_t st) {
+    switch (v) {
+        case 0: bz_launch_nv<N, 0>(kind, inst, g, st); break;
+        case 1: bz_launch_nv<N, 1>(kind, inst, g, st); break;
+        case 2: bz_launch_nv<N, 2>(kind, inst, g, st); break;
+        default: bz_launch_nv<N, 3>(kind, inst, g, st); break;
+    }
+}
+
+static void bz_launch(int kind, int v, const BInst* inst, const BGroup& g, hipStream_t st) {
+    if (g.total <= 0) return;
+    switch (g.key) {
+#define DNSP_CASE(N) case N: bz_launch_n<N>(kind, v, inst, g, st); break;
+        DNSP_CT_SIZES(DNSP_CASE)
+#undef DNSP_CASE
+        default: break;
+    }
+}
+
+template <int MR, int OCC>
+static void bt_launch_m(int kind, const BInst* inst, const BGroup& g, size_t lds, hipStream_t st) {
+    const int nb = (int)g.total;
+    constexpr int NT = kBatchM * kPartChunks;
+    if (kind == 0) kb_tri<MR, 0, OCC><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
+    else if (kind == 1) kb_tri<MR, 1, OCC><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
+    else kb_tri<MR, 2, OCC><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
+}
+
+static void bt_launch(int kind, int v, const BInst* inst, const BGroup& g, size_t lds, hipStream_t st) {
+    if (g.total <= 0) return;
+    switch (g.key) {
+#define DNSP_CASE(MR)                                            \
+    case MR:                                                     \
+        if (v == 0) bt_launch_m<MR, 1>(kind, inst, g, lds, st);  \
+        else bt_launch_m<MR, 4>(kind, inst, g, lds, st);         \
+        break;
+        DNSP_PART_ROWS(DNSP_CASE)
+#undef DNSP_CASE
+        default: break;
+    }
+}
+
+}  // namespace dnsp
+
+using namespace dnsp;
+
+struct dnsp_batch {
+    int n = 0;
+    int precision = 0;
+    int device = 0;
+    std::vector<int> nx, nz, ny, mr;
+    std::vector<double> pcr_min;
+    BInst* d_inst = nullptr;
+    std::vector<BInst> h_inst;
+    std::vector<void*> allocs;  // everything hipMalloc'd, freed at the end
+    // launch groups: x and z per FFT variant (tile sizes differ), tridiagonal, planes
+    std::vector<BGroup> gx[kCtVariants], gz[kCtVariants], gt;
+    BGroup gq;
+    int max_ny = 0;
+    int xvar = 0, zvar = 0, tvar = 1;     // FP64 form
+    int mxvar = 0, mzvar = 0, mtvar = 1;  // mixed form
+    bool bound = false;
+};
+
+namespace {
+
+int dev_alloc(dnsp_batch* b, void** p, size_t bytes) {
+    DNSP_HIP_TRY(hipMalloc(p, bytes));
+    b->allocs.push_back(*p);
+    return DNSP_OK;
+}
+
+template <typename T>
+int dev_copy(dnsp_batch* b, T** p, const T* src, size_t count) {
+    int rc = dev_alloc(b, reinterpret_cast<void**>(p), sizeof(T) * (count ? count : 1));
+    if (rc) return rc;
+    if (count) DNSP_HIP_TRY(hipMemcpy(*p, src, sizeof(T) * count, hipMemcpyHostToDevice));
+    return DNSP_OK;
+}
+
+void release(dnsp_batch* b) {
+    if (!b) return;
+    int cur = 0;
+    if (hipGetDevice(&cur) == hipSuccess && cur != b->device) (void)hipSetDevice(b->device);
+    for (void* p : b->allocs) (void)hipFree(p);
+    if (cur != b->device) (void)hipSetDevice(cur);
+    delete b;
+}
+
+int hc_wave(int i, int n) {
+    if (i <= n / 2) return i;
+    const int ii = (n + 1) / 2 - 1 - (i - (n / 2 + 1));
+    return ii > 1 ? ii : 1;
+}
+
+// poisson_solver_multigpu.f90:139-152
+void eig(int n, double inv_d, std::vector<double>& lam, std::vector<int>& is00) {
+    const double pi = 4.0 * std::atan(1.0);
+    lam.resize((size_t)n);
+    is00.resize((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const int ii = hc_wave(i, n);
+        const double sv = std::sin(ii * pi / n);
+        lam[(size_t)i] = -4 * (inv_d * inv_d) * (sv * sv);
+        is00[(size_t)i] = ii == 0 ? 1 : 0;
+    }
+}
+
+template <typename V>
+int twiddles(dnsp_batch* b, int n, V** out) {
+    std::vector<V> tw((size_t)n);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int t = 0; t < n; ++t) {
+        const long double ang = 2.0L * pi * (long double)t / (long double)n;
+        tw[(size_t)t].x = (decltype(tw[0].x))cosl(ang);
+        tw[(size_t)t].y = (decltype(tw[0].x))(-sinl(ang));
+    }
+    return dev_copy(b, out, tw.data(), (size_t)n);
+}
+
+// Groups instances by key; per instance `tasks` work items.
+int make_groups(dnsp_batch* b, const std::vector<int>& key, const std::vector<long>& tasks, std::vector<BGroup>& out) {
+    std::map<int, std::vector<int>> by;
+    for (int i = 0; i < b->n; ++i) by[key[(size_t)i]].push_back(i);
+    out.clear();
+    for (auto& kv : by) {
+        BGroup g;
+        g.key = kv.first;
+        g.cnt = (int)kv.second.size();
+        std::vector<long> st((size_t)g.cnt);
+        long tot = 0;
+        for (int u = 0; u < g.cnt; ++u) {
+            st[(size_t)u] = tot;
+            tot += tasks[(size_t)kv.second[(size_t)u]];
+        }
+        g.total = tot;
+        if (tot > 0x7fffffffL) {
+            set_error("batch launch group too large (%ld tasks)", tot);
+            return DNSP_ERR_ARG;
+        }
+        int rc = dev_copy(b, &g.ids, kv.second.data(), kv.second.size());
+        if (!rc) rc = dev_copy(b, &g.start, st.data(), st.size());
+        if (rc) return rc;
+        out.push_back(g);
+    }
+    return DNSP_OK;
+}
+
+int variant_tasks(int n, int v, bool xdir) {
+    const int tile = ct_var_tile(v);
+    if (xdir) {
+        const int G = std::max(1, tile / n);
+        return G;
+    }
+    return std::min(16, std::max(1, tile / n));
+}
+
+int setup(dnsp_batch* b, const double* bands_cat) {
+    DNSP_HIP_TRY(hipGetDevice(&b->device));
+    const int n = b->n;
+    std::map<int, double2*> tw64;
+    std::map<int, float2*> tw32;
+    b->h_inst.resize((size_t)n);
+    size_t boff = 0;
+    const bool mixed = b->precision == DNSP_PRECISION_MIXED;
+    for (int i = 0; i < n; ++i) {
+        BInst& I = b->h_inst[(size_t)i];
+        std::memset(&I, 0, sizeof(I));
+        const int nx = b->nx[(size_t)i], nz = b->nz[(size_t)i], ny = b->ny[(size_t)i];
+        I.nx = nx;
+        I.nz = nz;
+        I.ny = ny;
+        I.mr = b->mr[(size_t)i];
+        I.nlines = (long)nz * ny;
+        I.plane = (long)nx * nz;
+        I.inv_nxnz = 1.0 / ((double)nx * (double)nz);
+        const double* bi = bands_cat + boff;
+        boff += 6 * (size_t)ny;
+        double mn = 1e300;
+        const int k0 = ny > 2 ? 1 : 0, k1 = ny > 2 ? ny - 1 : ny;
+        for (int k = k0; k < k1; ++k) mn = std::min(mn, std::fabs(bi[k]) + std::fabs(bi[2 * ny + k]));
+        I.pcr_min = mn;
+        int rc = dev_copy(b, const_cast<double**>(&I.bands), bi, 6 * (size_t)ny);
+        if (rc) return rc;
+        for (int dir = 0; dir < 2; ++dir) {
+            const int len = dir == 0 ? nx : nz;
+            if (!tw64.count(len)) {
+                double2* t = nullptr;
+                rc = twiddles(b, len, &t);
+                if (rc) return rc;
+                tw64[len] = t;
+            }
+            if (mixed && !tw32.count(len)) {
+                float2* t = nullptr;
+                rc = twiddles(b, len, &t);
+                if (rc) return rc;
+                tw32[len] = t;
+            }
+        }
+        I.twx = tw64[nx];
+        I.twz = tw64[nz];
+        I.twx32 = mixed ? tw32[nx] : nullptr;
+        I.twz32 = mixed ? tw32[nz] : nullptr;
+        b->max_ny = std::max(b->max_ny, ny);
+        if (mixed) {
+            const size_t N = (size_t)nx * nz * ny;
+            rc = dev_alloc(b, reinterpret_cast<void**>(&I.A), sizeof(float) * N);
+            if (!rc) rc = dev_alloc(b, reinterpret_cast<void**>(&I.B), sizeof(float) * N);
+            if (!rc) rc = dev_alloc(b, reinterpret_cast<void**>(&I.lsA), sizeof(double) * I.nlines);
+            if (!rc) rc = dev_alloc(b, reinterpret_cast<void**>(&I.lsB), sizeof(double) * I.nlines);
+            if (!rc) rc = dev_alloc(b, reinterpret_cast<void**>(&I.small), sizeof(double) * 4 * ny);
+            if (rc) return rc;
+        }
+    }
+    return DNSP_OK;
+}
+
+int setup_eig(dnsp_batch* b, const double* inv_dx, const double* inv_dz) {
+    for (int i = 0; i < b->n; ++i) {
+        BInst& I = b->h_inst[(size_t)i];
+        std::vector<double> ax, az;
+        std::vector<int> ix, iz;
+        eig(I.nx, inv_dx[i], ax, ix);
+        eig(I.nz, inv_dz[i], az, iz);
+        I.ix2 = inv_dx[i] * inv_dx[i];
+        I.iz2 = inv_dz[i] * inv_dz[i];
+        int rc = dev_copy(b, const_cast<double**>(&I.ax), ax.data(), ax.size());
+        if (!rc) rc = dev_copy(b, const_cast<double**>(&I.az), az.data(), az.size());
+        if (!rc) rc = dev_copy(b, const_cast<int**>(&I.i0x), ix.data(), ix.size());
+        if (!rc) rc = dev_copy(b, const_cast<int**>(&I.i0z), iz.data(), iz.size());
+        if (rc) return rc;
+    }
+    return DNSP_OK;
+}
+
+int setup_groups(dnsp_batch* b) {
+    const int n = b->n;
+    for (int v = 0; v < kCtVariants; ++v) {
+        std::vector<long> tx((size_t)n), tz((size_t)n);
+        for (int i = 0; i < n; ++i) {
+            const BInst& I = b->h_inst[(size_t)i];
+            const int G = variant_tasks(I.nx, v, true);
+            tx[(size_t)i] = (I.nlines + 2 * G - 1) / (2 * G);
+            const int TA = variant_tasks(I.nz, v, false);
+            const long ntc = ((I.nx + 1) / 2 + TA - 1) / TA;
+            tz[(size_t)i] = ntc * I.ny;
+        }
+        int rc = make_groups(b, b->nx, tx, b->gx[v]);
+        if (!rc) rc = make_groups(b, b->nz, tz, b->gz[v]);
+        if (rc) return rc;
+    }
+    std::vector<long> tt((size_t)n), tq((size_t)n);
+    std::vector<int> zero((size_t)n, 0);
+    for (int i = 0; i < n; ++i) {
+        const BInst& I = b->h_inst[(size_t)i];
+        tt[(size_t)i] = (I.plane + kBatchM - 1) / kBatchM;
+        tq[(size_t)i] = I.ny;
+    }
+    int rc = make_groups(b, b->mr, tt, b->gt);
+    if (rc) return rc;
+    std::vector<BGroup> gq;
+    rc = make_groups(b, zero, tq, gq);
+    if (rc) return rc;
+    b->gq = gq.empty() ? BGroup() : gq[0];
+    return DNSP_OK;
+}
+
+int upload_inst(dnsp_batch* b) {
+    if (!b->d_inst) {
+        int rc = dev_alloc(b, reinterpret_cast<void**>(&b->d_inst), sizeof(BInst) * (size_t)b->n);
+        if (rc) return rc;
+    }
+    DNSP_HIP_TRY(hipMemcpy(b->d_inst, b->h_inst.data(), sizeof(BInst) * (size_t)b->n, hipMemcpyHostToDevice));
+    return DNSP_OK;
+}
+
+size_t tri_lds(const dnsp_batch* b) { return 6 * sizeof(double) * (size_t)b->max_ny; }
+
+void run_fp64(dnsp_batch* b, int xv, int zv, int tv, hipStream_t st) {
+    for (const BGroup& g : b->gx[xv]) bx_launch(BX_F64_FWD, xv, b->d_inst, g, 0, st);
+    for (const BGroup& g : b->gz[zv]) bz_launch(BZ_F64_FWD, zv, b->d_inst, g, st);
+    for (const BGroup& g : b->gt) bt_launch(0, tv, b->d_inst, g, tri_lds(b), st);
+    for (const BGroup& g : b->gz[zv]) bz_launch(BZ_F64_BWD, zv, b->d_inst, g, st);
+    for (const BGroup& g : b->gx[xv]) bx_launch(BX_F64_BWD, xv, b->d_inst, g, 0, st);
+}
+
+void run_mixed(dnsp_batch* b, int refinements, int xv, int zv, int tv, hipStream_t st) {
+    for (const BGroup& g : b->gx[xv]) bx_launch(BX_MX_FWD, xv, b->d_inst, g, 0, st);
+    for (const BGroup& g : b->gz[zv]) bz_launch(BZ_A_FWD, zv, b->d_inst, g, st);
+    for (const BGroup& g : b->gt) bt_launch(1, tv, b->d_inst, g, tri_lds(b), st);
+    for (const BGroup& g : b->gz[zv]) bz_launch(BZ_A_BWD, zv, b->d_inst, g, st);
+    if (refinements == 0) {
+        for (const BGroup& g : b->gx[xv]) bx_launch(BX_MX_OUT0, xv, b->d_inst, g, 0, st);
+        return;
+    }
+    for (const BGroup& g : b->gx[xv]) bx_launch(BX_MX_BWD, xv, b->d_inst, g, 0, st);
+    if (b->gq.total > 0) kb_q<<<(int)b->gq.total, 64, 0, st>>>(b->d_inst, b->gq.ids, b->gq.start, b->gq.cnt);
+    for (const BGroup& g : b->gx[xv]) bx_launch(BX_MX_RES, xv, b->d_inst, g, 0, st);
+    for (const BGroup& g : b->gz[zv]) bz_launch(BZ_B_FWD, zv, b->d_inst, g, st);
+    for (const BGroup& g : b->gt) bt_launch(2, tv, b->d_inst, g, tri_lds(b), st);
+    for (const BGroup& g : b->gz[zv]) bz_launch(BZ_B_BWD, zv, b->d_inst, g, st);
+    for (const BGroup& g : b->gx[xv]) bx_launch(BX_MX_ACC, xv, b->d_inst, g, 0, st);
+}
+
+int check_bound(const dnsp_batch* b) {
+    DNSP_ARG_CHECK(b, "NULL batch");
+    DNSP_ARG_CHECK(b->bound, "no arrays bound to the batch (dnsp_batch_bind)");
+    return DNSP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dnsp_batch_create(int n, const int* nx, const int* nz, const int* ny, const double* inv_dx, const double* inv_dz,
+                      const double* bands, int precision, dnsp_batch** out) {
+    clear_error();
+    DNSP_ARG_CHECK(out, "out is NULL");
+    *out = nullptr;
+    DNSP_ARG_CHECK(n >= 1 && n <= (1 << 20), "n=%d instances", n);
+    DNSP_ARG_CHECK(nx && nz && ny && inv_dx && inv_dz && bands, "NULL argument");
+    DNSP_ARG_CHECK(precision == DNSP_PRECISION_FP64 || precision == DNSP_PRECISION_MIXED, "precision=%d",
+                   precision);
+    size_t nb = 0;
+    for (int i = 0; i < n; ++i) {
+        DNSP_ARG_CHECK(ct_fft_supported(nx[i]) && ct_fft_supported(nz[i]),
+                       "instance %d: nx=%d, nz=%d must be in {32,64,96,128,192,256,384,512,768}", i, nx[i], nz[i]);
+        DNSP_ARG_CHECK(ny[i] >= 2 && part_rows_for(ny[i]) > 0, "instance %d: ny=%d not in [2, %d]", i, ny[i],
+                       32 * kPartChunks);
+        DNSP_ARG_CHECK(std::isfinite(inv_dx[i]) && std::isfinite(inv_dz[i]), "instance %d: spacing not finite", i);
+        nb += 6 * (size_t)ny[i];
+    }
+    for (size_t e = 0; e < nb; ++e) DNSP_ARG_CHECK(std::isfinite(bands[e]), "band value %zu is not finite", e);
+    dnsp_batch* b = new (std::nothrow) dnsp_batch();
+    if (!b) {
+        set_error("host allocation failed");
+        return DNSP_ERR_NOMEM;
+    }
+    b->n = n;
+    b->precision = precision;
+    b->nx.assign(nx, nx + n);
+    b->nz.assign(nz, nz + n);
+    b->ny.assign(ny, ny + n);
+    b->mr.resize((size_t)n);
+    for (int i = 0; i < n; ++i) b->mr[(size_t)i] = part_rows_for(ny[i]);
+    int rc = setup(b, bands);
+    if (!rc) rc = setup_eig(b, inv_dx, inv_dz);
+    if (!rc) rc = setup_groups(b);
+    if (!rc) rc = upload_inst(b);
+    if (rc) {
+        release(b);
+        return rc;
+    }
+    *out = b;
+    return DNSP_OK;
+}
+
+int dnsp_batch_bind(dnsp_batch* b, const double* const* rhs, double* const* P) {
+    clear_error();
+    DNSP_ARG_CHECK(b && rhs && P, "NULL argument");
+    for (int i = 0; i < b->n; ++i) {
+        DNSP_ARG_CHECK(rhs[i] && P[i], "instance %d: NULL array", i);
+        DNSP_ARG_CHECK(rhs[i] != P[i], "instance %d: rhs and P must be distinct arrays", i);
+    }
+    for (int i = 0; i < b->n; ++i) {
+        b->h_inst[(size_t)i].f = rhs[i];
+        b->h_inst[(size_t)i].P = P[i];
+    }
+    int rc = upload_inst(b);
+    if (rc) return rc;
+    b->bound = true;
+    return DNSP_OK;
+}
+
+int dnsp_batch_run(dnsp_batch* b, void* stream) {
+    clear_error();
+    int rc = check_bound(b);
+    if (rc) return rc;
+    run_fp64(b, b->xvar, b->zvar, b->tvar, as_stream(stream));
+    DNSP_LAUNCH_CHECK();
+    return DNSP_OK;
+}
+
+int dnsp_batch_run_mixed(dnsp_batch* b, int refinements, void* stream) {
+    clear_error();
+    int rc = check_bound(b);
+    if (rc) return rc;
+    DNSP_ARG_CHECK(b->precision == DNSP_PRECISION_MIXED, "batch was created with precision %d", b->precision);
+    DNSP_ARG_CHECK(refinements == 0 || refinements == 1, "refinements=%d: the batched form does 0 or 1",
+                   refinements);
+    run_mixed(b, refinements, b->mxvar, b->mzvar, b->mtvar, as_stream(stream));
+    DNSP_LAUNCH_CHECK();
+    return DNSP_OK;
+}
+
+int dnsp_batch_set_variants(dnsp_batch* b, int xvar, int zvar, int tvar, int mxvar, int mzvar, int mtvar) {
+    clear_error();
+    DNSP_ARG_CHECK(b, "NULL batch");
+    for (int v : {xvar, zvar, mxvar, mzvar}) DNSP_ARG_CHECK(v >= 0 && v < kCtVariants, "FFT variant %d", v);
+    for (int v : {tvar, mtvar}) DNSP_ARG_CHECK(v == 0 || v == 1, "tridiagonal variant %d", v);
+    b->xvar = xvar;
+    b->zvar = zvar;
+    b->tvar = tvar;
+    b->mxvar = mxvar;
+    b->mzvar = mzvar;
+    b->mtvar = mtvar;
+    return DNSP_OK;
+}
+
+int dnsp_batch_get_variants(const dnsp_batch* b, int* v6) {
+    clear_error();
+    DNSP_ARG_CHECK(b && v6, "NULL argument");
+    const int v[6] = {b->xvar, b->zvar, b->tvar, b->mxvar, b->mzvar, b->mtvar};
+    std::memcpy(v6, v, sizeof(v));
+    return DNSP_OK;
+}
+
+// Tunes the variants of the whole batched solve: each FP64 (and, for a mixed
+// batch, mixed) solve is timed with one knob changed at a time, x then z then
+// tridiagonal; overwrites the bound solution arrays.  times_ms (optional,
+// DNSP_TUNE_SLOTS): ms per solve, FP64 x, z, tridiagonal variants (4 slots
+// each), then the mixed ones; -1 where not applicable.
+int dnsp_batch_autotune(dnsp_batch* b, int reps, void* stream, double* times_ms) {
+    clear_error();
+    int rc = check_bound(b);
+    if (rc) return rc;
+    DNSP_ARG_CHECK(reps >= 1 && reps <= 1000, "reps=%d", reps);
+    hipStream_t st = as_stream(stream);
+    if (times_ms)
+        for (int i = 0; i < DNSP_TUNE_SLOTS; ++i) times_ms[i] = -1.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DNSP_HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        set_error("event creation failed");
+        return DNSP_ERR_HIP;
+    }
+    for (int form = 0; form < 2 && rc == DNSP_OK; ++form) {
+        if (form == 1 && b->precision != DNSP_PRECISION_MIXED) break;
+        int* var[3] = {form ? &b->mxvar : &b->xvar, form ? &b->mzvar : &b->zvar, form ? &b->mtvar : &b->tvar};
+        for (int knob = 0; knob < 3 && rc == DNSP_OK; ++knob) {
+            const int nv = knob == 2 ? 2 : kCtVariants;
+            double best = 1e30;
+            int pick = *var[knob];
+            for (int v = 0; v < nv; ++v) {
+                *var[knob] = v;
+                auto once = [&]() {
+                    if (form == 0) run_fp64(b, b->xvar, b->zvar, b->tvar, st);
+                    else run_mixed(b, 1, b->mxvar, b->mzvar, b->mtvar, st);
+                };
+                once();
+                (void)hipEventRecord(e0, st);
+                for (int r = 0; r < reps; ++r) once();
+                (void)hipEventRecord(e1, st);
+                if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                    set_error("batch autotune: launch or synchronize failed");
+                    rc = DNSP_ERR_HIP;
+                    break;
+                }
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                const double per = (double)ms / reps;
+                if (times_ms) times_ms[form * 12 + knob * 4 + v] = per;
+                if (per < best) {
+                    best = per;
+                    pick = v;
+                }
+            }
+            *var[knob] = pick;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int dnsp_batch_size(const dnsp_batch* b, int* n) {
+    clear_error();
+    DNSP_ARG_CHECK(b && n, "NULL argument");
+    *n = b->n;
+    return DNSP_OK;
+}
+
+int dnsp_batch_free(dnsp_batch* b) {
+    clear_error();
+    release(b);
+    return DNSP_OK;
+}
+
+}  // extern "C"

@@ -27,14 +27,16 @@ constexpr int ct_cols() {  // complex columns per z workgroup, at most 16
 // ---------------------------------------------------------------------------
 // x forward: 2G real lines (one contiguous block of memory) per workgroup
 // ---------------------------------------------------------------------------
+// Bodies take the tile index and the LDS tile (G * ct_pitch(N) complex
+// values); the __global__ wrappers pick the tile with xcd_tile.  All threads
+// of the workgroup call a body.
 template <int N, int G, int NT>
-__global__ __launch_bounds__(NT) void kt_xfwd(const double* P, double* S, long nlines,
-                                                      const double2* __restrict__ tw) {
+__device__ __forceinline__ void xfwd_body(const double* P, double* S, long nlines, const double2* __restrict__ tw,
+                                          long tile, double2* sm) {
     constexpr int LP = ct_pitch(N);
     constexpr int NL = (G * N + NT - 1) / NT;  // double2 loads per thread
-    __shared__ double2 sm[G * LP];
     double* smd = reinterpret_cast<double*>(sm);
-    const long l0 = (long)xcd_tile(blockIdx.x, gridDim.x) * 2 * G;
+    const long l0 = tile * 2 * G;
     const double2* src = reinterpret_cast<const double2*>(P + l0 * N);
     double2 v[NL];
 #pragma unroll
@@ -80,18 +82,24 @@ __global__ __launch_bounds__(NT) void kt_xfwd(const double* P, double* S, long n
     }
 }
 
+template <int N, int G, int NT>
+__global__ __launch_bounds__(NT) void kt_xfwd(const double* P, double* S, long nlines,
+                                              const double2* __restrict__ tw) {
+    __shared__ double2 sm[G * ct_pitch(N)];
+    xfwd_body<N, G, NT>(P, S, nlines, tw, xcd_tile(blockIdx.x, gridDim.x), sm);
+}
+
 // ---------------------------------------------------------------------------
 // x backward (+ scale): halfcomplex pairs of lines -> real lines
 // ---------------------------------------------------------------------------
 template <int N, int G, int NT>
-__global__ __launch_bounds__(NT) void kt_xbwd(const double* S, double* P, long nlines,
-                                                      const double2* __restrict__ tw, double scale) {
+__device__ __forceinline__ void xbwd_body(const double* S, double* P, long nlines, const double2* __restrict__ tw,
+                                          double scale, long tile, double2* sm) {
     constexpr int LP = ct_pitch(N);
     constexpr int H = N / 2 + 1;
     constexpr int NI = (G * H + NT - 1) / NT;
     constexpr int NL = (G * N + NT - 1) / NT;
-    __shared__ double2 sm[G * LP];
-    const long l0 = (long)xcd_tile(blockIdx.x, gridDim.x) * 2 * G;
+    const long l0 = tile * 2 * G;
     const double* src = S + l0 * N;
     double xr[NI], xi[NI], yr[NI], yi[NI];
 #pragma unroll
@@ -138,6 +146,13 @@ __global__ __launch_bounds__(NT) void kt_xbwd(const double* S, double* P, long n
     }
 }
 
+template <int N, int G, int NT>
+__global__ __launch_bounds__(NT) void kt_xbwd(const double* S, double* P, long nlines,
+                                              const double2* __restrict__ tw, double scale) {
+    __shared__ double2 sm[G * ct_pitch(N)];
+    xbwd_body<N, G, NT>(S, P, nlines, tw, scale, xcd_tile(blockIdx.x, gridDim.x), sm);
+}
+
 // ---------------------------------------------------------------------------
 // z: TA complex columns of one plane per workgroup.  Complex column c pairs
 // the adjacent halfcomplex x rows 2c (real part) and 2c+1 (imaginary part),
@@ -148,19 +163,16 @@ __global__ __launch_bounds__(NT) void kt_xbwd(const double* S, double* P, long n
 // T is the storage (and arithmetic) type, double or float.  When lsum is not
 // NULL, the workgroup holding x row 0 of plane k also writes
 // msum[k] = sum_j lsum[k*N + j] (FP64 line sums -> plane sum, fixed order).
-template <int N, int TA, int NT, typename T = double>
-__global__ __launch_bounds__(NT) void kt_zfwd(T* S, int nx, int ncol, int ntc,
-                                              const typename Cx<T>::V* __restrict__ tw,
-                                              const double* __restrict__ lsum = nullptr,
-                                              double* __restrict__ msum = nullptr) {
+template <int N, int TA, int NT, typename T>
+__device__ __forceinline__ void zfwd_body(T* S, int nx, int ntc, const typename Cx<T>::V* __restrict__ tw,
+                                          const double* __restrict__ lsum, double* __restrict__ msum, int tile,
+                                          typename Cx<T>::V* sm) {
     using V = typename Cx<T>::V;
     constexpr int LP = ct_pitch(N);
     constexpr int W = 2 * TA;  // real rows per tile
     constexpr int TOT = W * N;
     constexpr int NL = (TOT + NT - 1) / NT;
-    __shared__ V sm[TA * LP];
     T* smd = reinterpret_cast<T*>(sm);
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int k = tile / ntc;
     const int r0 = (tile - k * ntc) * W;  // first real row of the tile
     T* Sk = S + (size_t)k * nx * N;
@@ -212,8 +224,17 @@ __global__ __launch_bounds__(NT) void kt_zfwd(T* S, int nx, int ncol, int ntc,
 }
 
 template <int N, int TA, int NT, typename T = double>
-__global__ __launch_bounds__(NT) void kt_zbwd(T* S, int nx, int ncol, int ntc,
-                                              const typename Cx<T>::V* __restrict__ tw) {
+__global__ __launch_bounds__(NT) void kt_zfwd(T* S, int nx, int ncol, int ntc,
+                                              const typename Cx<T>::V* __restrict__ tw,
+                                              const double* __restrict__ lsum = nullptr,
+                                              double* __restrict__ msum = nullptr) {
+    __shared__ typename Cx<T>::V sm[TA * ct_pitch(N)];
+    zfwd_body<N, TA, NT, T>(S, nx, ntc, tw, lsum, msum, xcd_tile(blockIdx.x, gridDim.x), sm);
+}
+
+template <int N, int TA, int NT, typename T>
+__device__ __forceinline__ void zbwd_body(T* S, int nx, int ntc, const typename Cx<T>::V* __restrict__ tw,
+                                          int tile, typename Cx<T>::V* sm) {
     using V = typename Cx<T>::V;
     constexpr int LP = ct_pitch(N);
     constexpr int W = 2 * TA;
@@ -221,8 +242,6 @@ __global__ __launch_bounds__(NT) void kt_zbwd(T* S, int nx, int ncol, int ntc,
     constexpr int NI = (H * TA + NT - 1) / NT;
     constexpr int TOT = W * N;
     constexpr int NL = (TOT + NT - 1) / NT;
-    __shared__ V sm[TA * LP];
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int k = tile / ntc;
     const int r0 = (tile - k * ntc) * W;
     T* Sk = S + (size_t)k * nx * N;
@@ -263,6 +282,13 @@ __global__ __launch_bounds__(NT) void kt_zbwd(T* S, int nx, int ncol, int ntc,
         if ((TOT % NT == 0 || e < TOT) && r0 + r < nx)
             Sk[(size_t)j * nx + r0 + r] = smd[((r >> 1) * LP + ct_pad(j)) * 2 + (r & 1)];
     }
+}
+
+template <int N, int TA, int NT, typename T = double>
+__global__ __launch_bounds__(NT) void kt_zbwd(T* S, int nx, int ncol, int ntc,
+                                              const typename Cx<T>::V* __restrict__ tw) {
+    __shared__ typename Cx<T>::V sm[TA * ct_pitch(N)];
+    zbwd_body<N, TA, NT, T>(S, nx, ntc, tw, xcd_tile(blockIdx.x, gridDim.x), sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -314,28 +340,174 @@ __device__ __forceinline__ void tri_load_rows(double (&d)[MR], const T* S, long 
     }
 }
 
-template <int MR, int P, int M, typename T = double, bool LOOP = false, int OCC = 1>
-__global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __restrict__ ax,
-                                                     const double* __restrict__ az,
-                                                     const int* __restrict__ is0x,
-                                                     const int* __restrict__ is0z,
-                                                     const double* __restrict__ bands, int nx, int ny,
-                                                     long plane, double pcr_min, int ngroups,
-                                                     const double* __restrict__ msum = nullptr,
-                                                     double* __restrict__ pbar = nullptr,
-                                                     double inv_nxnz = 0.0) {
+// One mode group: thread (mi, p) holds rows s..s+MR-1 of mode `mode` in d[]
+// and writes its solution rows.  smb: the staged bands (plain set, then the
+// (0,0)-mode set where needed); ra..rd: the LDS reduced system.  Contains
+// workgroup barriers; all threads call it with the same group.
+template <int MR, int P, int M, typename T>
+__device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* __restrict__ ax,
+                                          const double* __restrict__ az, const int* __restrict__ is0x,
+                                          const int* __restrict__ is0z, const double* smb, int nx, int ny,
+                                          long plane, double pcr_min, long mode, const double* __restrict__ msum,
+                                          double* __restrict__ pbar, double inv_nxnz, double (*ra)[M],
+                                          double (*rb)[M], double (*rc)[M], double (*rd)[M]) {
     constexpr int NT = M * P;
     constexpr int NR = 2 * P;  // reduced rows per mode
-    __shared__ double ra[NR][M], rb[NR][M], rc[NR][M], rd[NR][M];  // mode fastest
-    extern __shared__ double smb[];                                  // 6*ny band values
+    const int mi = (int)threadIdx.x % M;
     const int p = (int)threadIdx.x / M;
     const int s = p * MR;
+    const bool valid = mode < plane;
+    const long md = valid ? mode : 0;
+    const int i = (int)(md % nx);
+    const int j = (int)(md / nx);
+    const double lam = ax[i] + az[j];
+    const double* A = smb + ((is0x[i] && is0z[j]) ? 3 * ny : 0);
+    const double* B = A + ny;
+    const double* C = B + ny;
+    auto band_a = [&](int row) { return row < ny ? A[row] : 0.0; };
+    auto band_c = [&](int row) { return row < ny ? C[row] : 0.0; };
+    auto beta = [&](int row) { return row < ny ? B[row] + lam : 1.0; };
+
+    double Ah[MR], Ch[MR];  // d[] is overwritten by D^ for interior rows
+    double Ae, Be, De;
+    {
+        double Ak = band_a(s + 1);
+        double Bk = beta(s + 1);
+        double Dk = d[1];
+        double cprev = band_c(s + 1);
+#pragma unroll
+        for (int l = 2; l < MR; ++l) {
+            const double rinv = rcp64(Bk);
+            d[l - 1] = Dk * rinv;
+            Ah[l - 1] = Ak * rinv;
+            Ch[l - 1] = cprev * rinv;
+            const double f = band_a(s + l) * rinv;
+            Ak = -f * Ak;
+            Bk = beta(s + l) - f * cprev;
+            Dk = d[l] - f * Dk;
+            cprev = band_c(s + l);
+        }
+        Ae = Ak;
+        Be = Bk;
+        De = Dk;
+    }
+    // upward pass: x_{s+1} = Pp - Qq x_s - Rr x_e
+    double Pp = 0.0, Qq = 0.0, Rr = -1.0;
+#pragma unroll
+    for (int l = MR - 2; l >= 1; --l) {
+        Pp = d[l] - Ch[l] * Pp;
+        Qq = Ah[l] - Ch[l] * Qq;
+        Rr = -Ch[l] * Rr;
+    }
+    {
+        const double cs = band_c(s);
+        ra[2 * p][mi] = band_a(s);
+        rb[2 * p][mi] = beta(s) - cs * Qq;
+        rc[2 * p][mi] = -cs * Rr;
+        rd[2 * p][mi] = d[0] - cs * Pp;
+        ra[2 * p + 1][mi] = Ae;
+        rb[2 * p + 1][mi] = Be;
+        rc[2 * p + 1][mi] = band_c(s + MR - 1);
+        rd[2 * p + 1][mi] = De;
+    }
+    const bool serial = __syncthreads_or(valid && fabs(lam) < pcr_min) != 0;
+    if (serial) {
+        // one thread per mode: Thomas on the NR tip rows; x overwrites rd, rb = 1
+        if (p == 0) {
+            for (int q = 1; q < NR; ++q) {
+                const double m = ra[q][mi] / rb[q - 1][mi];
+                rb[q][mi] = rb[q][mi] - m * rc[q - 1][mi];
+                rd[q][mi] = rd[q][mi] - m * rd[q - 1][mi];
+            }
+            double xq = rd[NR - 1][mi] / rb[NR - 1][mi];
+            rd[NR - 1][mi] = xq;
+            rb[NR - 1][mi] = 1.0;
+            for (int q = NR - 2; q >= 0; --q) {
+                xq = (rd[q][mi] - rc[q][mi] * xq) / rb[q][mi];
+                rd[q][mi] = xq;
+                rb[q][mi] = 1.0;
+            }
+        }
+        __syncthreads();
+    }
+    // parallel cyclic reduction on the NR tip rows of each mode
+    for (int st = 1; !serial && st < NR; st *= 2) {
+        double na[2], nb[2], nc[2], nd[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = (int)threadIdx.x + h * NT;
+            const int mm = e % M;
+            const int q = e / M;
+            const double aq = ra[q][mm], bq = rb[q][mm], cq = rc[q][mm], dq = rd[q][mm];
+            double a2 = 0.0, b2 = bq, c2 = 0.0, d2 = dq;
+            if (q - st >= 0) {
+                const double al = -aq * rcp64(rb[q - st][mm]);
+                a2 = al * ra[q - st][mm];
+                b2 += al * rc[q - st][mm];
+                d2 += al * rd[q - st][mm];
+            }
+            if (q + st < NR) {
+                const double ga = -cq * rcp64(rb[q + st][mm]);
+                c2 = ga * rc[q + st][mm];
+                b2 += ga * ra[q + st][mm];
+                d2 += ga * rd[q + st][mm];
+            }
+            na[h] = a2; nb[h] = b2; nc[h] = c2; nd[h] = d2;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = (int)threadIdx.x + h * NT;
+            const int mm = e % M;
+            const int q = e / M;
+            ra[q][mm] = na[h]; rb[q][mm] = nb[h]; rc[q][mm] = nc[h]; rd[q][mm] = nd[h];
+        }
+        __syncthreads();
+    }
+    const double xs = serial ? rd[2 * p][mi] : rd[2 * p][mi] * rcp64(rb[2 * p][mi]);
+    const double xe = serial ? rd[2 * p + 1][mi] : rd[2 * p + 1][mi] * rcp64(rb[2 * p + 1][mi]);
+    d[0] = xs;
+    double xn = xe;
+#pragma unroll
+    for (int l = MR - 2; l >= 1; --l) {
+        xn = d[l] - Ah[l] * xs - Ch[l] * xn;
+        d[l] = xn;
+    }
+    d[MR - 1] = xe;
+    if (valid) {
+        const bool mean = msum != nullptr && mode == 0;
+#pragma unroll
+        for (int r = 0; r < MR; ++r) {
+            const int row = s + r;
+            if (row < ny) {
+                if (mean) pbar[row] = d[r] * inv_nxnz;
+                S[(size_t)row * plane + md] = mean ? T(0) : (T)d[r];
+            }
+        }
+    }
+}
+
+template <int MR, int P, int M, typename T = double, bool LOOP = false, int OCC = 1>
+__global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __restrict__ ax,
+                                                          const double* __restrict__ az,
+                                                          const int* __restrict__ is0x,
+                                                          const int* __restrict__ is0z,
+                                                          const double* __restrict__ bands, int nx, int ny,
+                                                          long plane, double pcr_min, int ngroups,
+                                                          const double* __restrict__ msum = nullptr,
+                                                          double* __restrict__ pbar = nullptr,
+                                                          double inv_nxnz = 0.0) {
+    constexpr int NT = M * P;
+    constexpr int NR = 2 * P;
+    __shared__ double ra[NR][M], rb[NR][M], rc[NR][M], rd[NR][M];  // mode fastest
+    extern __shared__ double smb[];                                  // 6*ny band values
+    const int mi = (int)threadIdx.x % M;
+    const int s = ((int)threadIdx.x / M) * MR;
 
     // Adjacent mode groups share cache lines, so they go to one XCD (each has
     // its own L2; workgroups reach XCDs round-robin): without LOOP through
     // xcd_tile, with LOOP as a contiguous range of groups per XCD, strided
     // over that XCD's workgroups (gridDim.x >= 8).
-    const int mi = (int)threadIdx.x % M;
     int grp, g_end, g_step;
     if constexpr (LOOP) {
         const int xcd = (int)blockIdx.x % 8;
@@ -347,156 +519,21 @@ __global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __
         g_end = grp + 1;
         g_step = 1;
     }
+    if (grp >= g_end) return;  // workgroup-uniform
     // the (0,0)-mode band set is only needed by the workgroup holding mode 0
     const int nband = (grp == 0 ? 6 : 3) * ny;
     for (int e = (int)threadIdx.x; e < nband; e += NT) smb[e] = bands[e];
-    bool staged = false;
     double d[MR];
-    if (grp < g_end) tri_load_rows<MR, P, M, T>(d, S, (long)grp * M + mi, plane, ny, s, msum);
+    tri_load_rows<MR, P, M, T>(d, S, (long)grp * M + mi, plane, ny, s, msum);
+    __syncthreads();  // bands staged.  Later groups need no barrier before
+                      // their rows: each thread's last reads of the reduced
+                      // rows are its own slots, which only it rewrites.
     for (; grp < g_end; grp += g_step) {
-        const long mode = (long)grp * M + mi;
+        tri_group<MR, P, M, T>(d, S, ax, az, is0x, is0z, smb, nx, ny, plane, pcr_min, (long)grp * M + mi, msum,
+                               pbar, inv_nxnz, ra, rb, rc, rd);
+        if constexpr (!LOOP) break;
         const int nxt = grp + g_step;
-
-        const bool valid = mode < plane;
-        const long md = valid ? mode : 0;
-        const int i = (int)(md % nx);
-        const int j = (int)(md / nx);
-        const double lam = ax[i] + az[j];
-        // bands staged.  Later groups need no barrier here: each thread's last
-        // reads of the reduced rows are its own slots, which only it rewrites.
-        if (!staged) {
-            __syncthreads();
-            staged = true;
-        }
-        const double* A = smb + ((is0x[i] && is0z[j]) ? 3 * ny : 0);
-        const double* B = A + ny;
-        const double* C = B + ny;
-        auto band_a = [&](int row) { return row < ny ? A[row] : 0.0; };
-        auto band_c = [&](int row) { return row < ny ? C[row] : 0.0; };
-        auto beta = [&](int row) { return row < ny ? B[row] + lam : 1.0; };
-
-        double Ah[MR], Ch[MR];  // d[] is overwritten by D^ for interior rows
-        double Ae, Be, De;
-        {
-            double Ak = band_a(s + 1);
-            double Bk = beta(s + 1);
-            double Dk = d[1];
-            double cprev = band_c(s + 1);
-#pragma unroll
-            for (int l = 2; l < MR; ++l) {
-                const double rinv = rcp64(Bk);
-                d[l - 1] = Dk * rinv;
-                Ah[l - 1] = Ak * rinv;
-                Ch[l - 1] = cprev * rinv;
-                const double f = band_a(s + l) * rinv;
-                Ak = -f * Ak;
-                Bk = beta(s + l) - f * cprev;
-                Dk = d[l] - f * Dk;
-                cprev = band_c(s + l);
-            }
-            Ae = Ak;
-            Be = Bk;
-            De = Dk;
-        }
-        // upward pass: x_{s+1} = Pp - Qq x_s - Rr x_e
-        double Pp = 0.0, Qq = 0.0, Rr = -1.0;
-#pragma unroll
-        for (int l = MR - 2; l >= 1; --l) {
-            Pp = d[l] - Ch[l] * Pp;
-            Qq = Ah[l] - Ch[l] * Qq;
-            Rr = -Ch[l] * Rr;
-        }
-        {
-            const double cs = band_c(s);
-            ra[2 * p][mi] = band_a(s);
-            rb[2 * p][mi] = beta(s) - cs * Qq;
-            rc[2 * p][mi] = -cs * Rr;
-            rd[2 * p][mi] = d[0] - cs * Pp;
-            ra[2 * p + 1][mi] = Ae;
-            rb[2 * p + 1][mi] = Be;
-            rc[2 * p + 1][mi] = band_c(s + MR - 1);
-            rd[2 * p + 1][mi] = De;
-        }
-        const bool serial = __syncthreads_or(valid && fabs(lam) < pcr_min) != 0;
-        if (serial) {
-            // one thread per mode: Thomas on the NR tip rows; x overwrites rd, rb = 1
-            if (p == 0) {
-                for (int q = 1; q < NR; ++q) {
-                    const double m = ra[q][mi] / rb[q - 1][mi];
-                    rb[q][mi] = rb[q][mi] - m * rc[q - 1][mi];
-                    rd[q][mi] = rd[q][mi] - m * rd[q - 1][mi];
-                }
-                double xq = rd[NR - 1][mi] / rb[NR - 1][mi];
-                rd[NR - 1][mi] = xq;
-                rb[NR - 1][mi] = 1.0;
-                for (int q = NR - 2; q >= 0; --q) {
-                    xq = (rd[q][mi] - rc[q][mi] * xq) / rb[q][mi];
-                    rd[q][mi] = xq;
-                    rb[q][mi] = 1.0;
-                }
-            }
-            __syncthreads();
-        }
-        // parallel cyclic reduction on the NR tip rows of each mode
-        for (int st = 1; !serial && st < NR; st *= 2) {
-            double na[2], nb[2], nc[2], nd[2];
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int e = (int)threadIdx.x + h * NT;
-                const int mm = e % M;
-                const int q = e / M;
-                const double aq = ra[q][mm], bq = rb[q][mm], cq = rc[q][mm], dq = rd[q][mm];
-                double a2 = 0.0, b2 = bq, c2 = 0.0, d2 = dq;
-                if (q - st >= 0) {
-                    const double al = -aq * rcp64(rb[q - st][mm]);
-                    a2 = al * ra[q - st][mm];
-                    b2 += al * rc[q - st][mm];
-                    d2 += al * rd[q - st][mm];
-                }
-                if (q + st < NR) {
-                    const double ga = -cq * rcp64(rb[q + st][mm]);
-                    c2 = ga * rc[q + st][mm];
-                    b2 += ga * ra[q + st][mm];
-                    d2 += ga * rd[q + st][mm];
-                }
-                na[h] = a2; nb[h] = b2; nc[h] = c2; nd[h] = d2;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int e = (int)threadIdx.x + h * NT;
-                const int mm = e % M;
-                const int q = e / M;
-                ra[q][mm] = na[h]; rb[q][mm] = nb[h]; rc[q][mm] = nc[h]; rd[q][mm] = nd[h];
-            }
-            __syncthreads();
-        }
-        const double xs = serial ? rd[2 * p][mi] : rd[2 * p][mi] * rcp64(rb[2 * p][mi]);
-        const double xe = serial ? rd[2 * p + 1][mi] : rd[2 * p + 1][mi] * rcp64(rb[2 * p + 1][mi]);
-        d[0] = xs;
-        double xn = xe;
-#pragma unroll
-        for (int l = MR - 2; l >= 1; --l) {
-            xn = d[l] - Ah[l] * xs - Ch[l] * xn;
-            d[l] = xn;
-        }
-        d[MR - 1] = xe;
-        if (valid) {
-            const bool mean = msum != nullptr && mode == 0;
-#pragma unroll
-            for (int r = 0; r < MR; ++r) {
-                const int row = s + r;
-                if (row < ny) {
-                    if (mean) pbar[row] = d[r] * inv_nxnz;
-                    S[(size_t)row * plane + md] = mean ? T(0) : (T)d[r];
-                }
-            }
-        }
-        if constexpr (!LOOP) {
-            break;
-        } else {
-            if (nxt < g_end) tri_load_rows<MR, P, M, T>(d, S, (long)nxt * M + mi, plane, ny, s, msum);
-        }
+        if (nxt < g_end) tri_load_rows<MR, P, M, T>(d, S, (long)nxt * M + mi, plane, ny, s, msum);
     }
 }
 

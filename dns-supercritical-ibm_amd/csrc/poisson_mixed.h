@@ -90,16 +90,17 @@ __device__ __forceinline__ void xfwd_store_f32(const float2* sm, float* S, long 
 // Sweep 0, forward x: FP64 right-hand side -> FP32 halfcomplex lines, plus the
 // FP64 sums of the input lines.
 // ---------------------------------------------------------------------------
+// Bodies take the tile index, the LDS tile (G * ct_pitch(N) float2) and the
+// line-sum slots (line_slots<N, G>() doubles); the __global__ wrappers pick
+// the tile with xcd_tile.
 template <int N, int G, int NT>
-__global__ __launch_bounds__(NT) void kt_xfwd_mf(const double* __restrict__ f, float* __restrict__ S,
-                                                 double* __restrict__ lsum, long nlines,
-                                                 const float2* __restrict__ tw) {
+__device__ __forceinline__ void xfwd_mf_body(const double* __restrict__ f, float* __restrict__ S,
+                                             double* __restrict__ lsum, long nlines, const float2* __restrict__ tw,
+                                             long tile, float2* sm, double* slot) {
     constexpr int LP = ct_pitch(N);
     constexpr int NL = (G * N + NT - 1) / NT;
-    __shared__ float2 sm[G * LP];
-    __shared__ double slot[line_slots<N, G>()];
     float* smf = reinterpret_cast<float*>(sm);
-    const long l0 = (long)xcd_tile(blockIdx.x, gridDim.x) * 2 * G;
+    const long l0 = tile * 2 * G;
     const double2* src = reinterpret_cast<const double2*>(f + l0 * N);
     double2 v[NL];
 #pragma unroll
@@ -126,6 +127,15 @@ __global__ __launch_bounds__(NT) void kt_xfwd_mf(const double* __restrict__ f, f
     xfwd_store_f32<N, G, NT>(sm, S, l0, nlines);
 }
 
+template <int N, int G, int NT>
+__global__ __launch_bounds__(NT) void kt_xfwd_mf(const double* __restrict__ f, float* __restrict__ S,
+                                                 double* __restrict__ lsum, long nlines,
+                                                 const float2* __restrict__ tw) {
+    __shared__ float2 sm[G * ct_pitch(N)];
+    __shared__ double slot[line_slots<N, G>()];
+    xfwd_mf_body<N, G, NT>(f, S, lsum, nlines, tw, xcd_tile(blockIdx.x, gridDim.x), sm, slot);
+}
+
 // ---------------------------------------------------------------------------
 // Refinement, forward x: r = f - (L_xz + L_y) p - q(k) in FP64 for the tile's
 // lines (p of type TP: the FP32 sweep-0 field or an FP64 iterate), then the
@@ -135,18 +145,16 @@ __global__ __launch_bounds__(NT) void kt_xfwd_mf(const double* __restrict__ f, f
 // the walls (the operator of the FP64 solve).
 // ---------------------------------------------------------------------------
 template <int N, int G, int NT, typename TP>
-__global__ __launch_bounds__(NT) void kt_xfwd_mr(const TP* __restrict__ p, const double* __restrict__ f,
-                                                 const double* __restrict__ qk, const double* __restrict__ bands,
-                                                 float* __restrict__ S, double* __restrict__ lsum, int nz,
-                                                 int ny, double ix2, double iz2, long nlines,
-                                                 const float2* __restrict__ tw) {
+__device__ __forceinline__ void xfwd_mr_body(const TP* __restrict__ p, const double* __restrict__ f,
+                                             const double* __restrict__ qk, const double* __restrict__ bands,
+                                             float* __restrict__ S, double* __restrict__ lsum, int nz, int ny,
+                                             double ix2, double iz2, long nlines, const float2* __restrict__ tw,
+                                             long tile, float2* sm, double* slot) {
     using P2 = typename Cx<TP>::V;
     constexpr int LP = ct_pitch(N);
     constexpr int NL = (G * N + NT - 1) / NT;
-    __shared__ float2 sm[G * LP];
-    __shared__ double slot[line_slots<N, G>()];
     float* smf = reinterpret_cast<float*>(sm);
-    const long l0 = (long)xcd_tile(blockIdx.x, gridDim.x) * 2 * G;
+    const long l0 = tile * 2 * G;
     const int lane = (int)threadIdx.x & 63;
     double ps[NL];
 #pragma unroll
@@ -208,6 +216,18 @@ __global__ __launch_bounds__(NT) void kt_xfwd_mr(const TP* __restrict__ p, const
     xfwd_store_f32<N, G, NT>(sm, S, l0, nlines);
 }
 
+template <int N, int G, int NT, typename TP>
+__global__ __launch_bounds__(NT) void kt_xfwd_mr(const TP* __restrict__ p, const double* __restrict__ f,
+                                                 const double* __restrict__ qk, const double* __restrict__ bands,
+                                                 float* __restrict__ S, double* __restrict__ lsum, int nz,
+                                                 int ny, double ix2, double iz2, long nlines,
+                                                 const float2* __restrict__ tw) {
+    __shared__ float2 sm[G * ct_pitch(N)];
+    __shared__ double slot[line_slots<N, G>()];
+    xfwd_mr_body<N, G, NT, TP>(p, f, qk, bands, S, lsum, nz, ny, ix2, iz2, nlines, tw,
+                               xcd_tile(blockIdx.x, gridDim.x), sm, slot);
+}
+
 // Halfcomplex pairs of lines -> LDS complex lines ready for the inverse FFT
 // (as kt_xbwd), FP32.
 template <int N, int G, int NT>
@@ -252,13 +272,12 @@ __device__ __forceinline__ void xbwd_load_f32(float2* sm, const float* S, long l
 // (in place allowed: S == pt), plus the FP64 sums of the stored pt lines.
 // ---------------------------------------------------------------------------
 template <int N, int G, int NT>
-__global__ __launch_bounds__(NT) void kt_xbwd_mf(const float* S, float* pt, double* __restrict__ lsum,
-                                                 long nlines, const float2* __restrict__ tw, float scale) {
+__device__ __forceinline__ void xbwd_mf_body(const float* S, float* pt, double* __restrict__ lsum, long nlines,
+                                             const float2* __restrict__ tw, float scale, long tile, float2* sm,
+                                             double* slot) {
     constexpr int LP = ct_pitch(N);
     constexpr int NL = (G * N + NT - 1) / NT;
-    __shared__ float2 sm[G * LP];
-    __shared__ double slot[line_slots<N, G>()];
-    const long l0 = (long)xcd_tile(blockIdx.x, gridDim.x) * 2 * G;
+    const long l0 = tile * 2 * G;
     xbwd_load_f32<N, G, NT>(sm, S, l0, nlines);
     fft_ct<N, G, NT, true>(sm, tw);
     const float* smf = reinterpret_cast<const float*>(sm);
@@ -281,6 +300,14 @@ __global__ __launch_bounds__(NT) void kt_xbwd_mf(const float* S, float* pt, doub
     tile_line_sums<N, G, NT, NL>(ps, slot, lsum, l0, nlines);
 }
 
+template <int N, int G, int NT>
+__global__ __launch_bounds__(NT) void kt_xbwd_mf(const float* S, float* pt, double* __restrict__ lsum,
+                                                 long nlines, const float2* __restrict__ tw, float scale) {
+    __shared__ float2 sm[G * ct_pitch(N)];
+    __shared__ double slot[line_slots<N, G>()];
+    xbwd_mf_body<N, G, NT>(S, pt, lsum, nlines, tw, scale, xcd_tile(blockIdx.x, gridDim.x), sm, slot);
+}
+
 // ---------------------------------------------------------------------------
 // Refinement, inverse x with accumulation into FP64:
 //   P = base + (pbar0[k] + pbar1[k]) + scale * x-HC2R(S)
@@ -289,17 +316,15 @@ __global__ __launch_bounds__(NT) void kt_xbwd_mf(const float* S, float* pt, doub
 // of P's lines for a following sweep or residual check.
 // ---------------------------------------------------------------------------
 template <int N, int G, int NT, typename TP>
-__global__ __launch_bounds__(NT) void kt_xbwd_ma(const float* __restrict__ S, const TP* base,
-                                                 const double* __restrict__ pbar0,
-                                                 const double* __restrict__ pbar1, double* P,
-                                                 double* __restrict__ lsum, int nz, long nlines,
-                                                 const float2* __restrict__ tw, double scale) {
+__device__ __forceinline__ void xbwd_ma_body(const float* __restrict__ S, const TP* base,
+                                             const double* __restrict__ pbar0, const double* __restrict__ pbar1,
+                                             double* P, double* __restrict__ lsum, int nz, long nlines,
+                                             const float2* __restrict__ tw, double scale, long tile, float2* sm,
+                                             double* slot) {
     using P2 = typename Cx<TP>::V;
     constexpr int LP = ct_pitch(N);
     constexpr int NL = (G * N + NT - 1) / NT;
-    __shared__ float2 sm[G * LP];
-    __shared__ double slot[line_slots<N, G>()];
-    const long l0 = (long)xcd_tile(blockIdx.x, gridDim.x) * 2 * G;
+    const long l0 = tile * 2 * G;
     xbwd_load_f32<N, G, NT>(sm, S, l0, nlines);
     fft_ct<N, G, NT, true>(sm, tw);
     const float* smf = reinterpret_cast<const float*>(sm);
@@ -326,6 +351,18 @@ __global__ __launch_bounds__(NT) void kt_xbwd_ma(const float* __restrict__ S, co
         }
     }
     if (lsum != nullptr) tile_line_sums<N, G, NT, NL>(ps, slot, lsum, l0, nlines);
+}
+
+template <int N, int G, int NT, typename TP>
+__global__ __launch_bounds__(NT) void kt_xbwd_ma(const float* __restrict__ S, const TP* base,
+                                                 const double* __restrict__ pbar0,
+                                                 const double* __restrict__ pbar1, double* P,
+                                                 double* __restrict__ lsum, int nz, long nlines,
+                                                 const float2* __restrict__ tw, double scale) {
+    __shared__ float2 sm[G * ct_pitch(N)];
+    __shared__ double slot[line_slots<N, G>()];
+    xbwd_ma_body<N, G, NT, TP>(S, base, pbar0, pbar1, P, lsum, nz, nlines, tw, scale,
+                               xcd_tile(blockIdx.x, gridDim.x), sm, slot);
 }
 
 }  // namespace dnsp

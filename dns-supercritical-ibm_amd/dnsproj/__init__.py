@@ -76,6 +76,16 @@ _SIGNATURES = [
     ("dnsp_poisson_run_mixed", [_vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     ("dnsp_poisson_solve_refine", [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _ip, _dp, _vp], ctypes.c_int),
     ("dnsp_poisson_residual", [_vp, _vp, _vp, _dp, _vp], ctypes.c_int),
+    ("dnsp_batch_create", [ctypes.c_int, _ip, _ip, _ip, _dp, _dp, _dp, ctypes.c_int, ctypes.POINTER(_vp)],
+     ctypes.c_int),
+    ("dnsp_batch_bind", [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], ctypes.c_int),
+    ("dnsp_batch_run", [_vp, _vp], ctypes.c_int),
+    ("dnsp_batch_run_mixed", [_vp, ctypes.c_int, _vp], ctypes.c_int),
+    ("dnsp_batch_autotune", [_vp, ctypes.c_int, _vp, _dp], ctypes.c_int),
+    ("dnsp_batch_set_variants", [_vp] + [ctypes.c_int] * 6, ctypes.c_int),
+    ("dnsp_batch_get_variants", [_vp, _ip], ctypes.c_int),
+    ("dnsp_batch_size", [_vp, _ip], ctypes.c_int),
+    ("dnsp_batch_free", [_vp], ctypes.c_int),
     ("dnsp_poisson_min_pivot", [_vp, _dp], ctypes.c_int),
     ("dnsp_poisson_shape", [_vp, _ip, _ip, _ip], ctypes.c_int),
     ("dnsp_poisson_free", [_vp], ctypes.c_int),
@@ -312,6 +322,86 @@ class PoissonSolver:
     def close(self):
         if getattr(self, "_h", None) is not None:
             lib().dnsp_poisson_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PoissonBatch:
+    """Many independent solves of uneven sizes in one batched launch sequence.
+
+    ``cases``: sequence of dicts with nx, nz, ny, inv_dx, inv_dz and bands (the
+    six band arrays).  ``bind`` fixes the device arrays (torch float64
+    tensors, (ny, nz, nx) each); ``run`` / ``run_mixed`` then only enqueue
+    kernels, so they can be captured in a graph.
+    """
+
+    def __init__(self, cases, precision=PRECISION_FP64):
+        L = lib()
+        n = len(cases)
+        if n < 1:
+            raise ValueError("at least one instance")
+        self.shapes = [(int(c["ny"]), int(c["nz"]), int(c["nx"])) for c in cases]
+        ar = lambda key, dt: np.ascontiguousarray([c[key] for c in cases], dtype=dt)
+        nx, nz, ny = ar("nx", np.int32), ar("nz", np.int32), ar("ny", np.int32)
+        idx, idz = ar("inv_dx", np.float64), ar("inv_dz", np.float64)
+        bands = []
+        for c in cases:
+            bs = [np.ascontiguousarray(b, dtype=np.float64) for b in c["bands"]]
+            if len(bs) != 6 or any(b.size != int(c["ny"]) for b in bs):
+                raise ValueError("six band arrays of length ny per instance")
+            bands.extend(bs)
+        bcat = np.ascontiguousarray(np.concatenate(bands))
+        h = _vp()
+        _check(L.dnsp_batch_create(n, nx.ctypes.data_as(_ip), nz.ctypes.data_as(_ip), ny.ctypes.data_as(_ip),
+                                   idx.ctypes.data_as(_dp), idz.ctypes.data_as(_dp), bcat.ctypes.data_as(_dp),
+                                   int(precision), ctypes.byref(h)))
+        self._h = h
+        self.n = n
+        self.precision = int(precision)
+        self._bound = None
+
+    def bind(self, rhs, P):
+        if len(rhs) != self.n or len(P) != self.n:
+            raise ValueError(f"{self.n} rhs and P arrays required")
+        pr = (_vp * self.n)(*[_device_ptr(r, int(np.prod(s))) for r, s in zip(rhs, self.shapes)])
+        pp = (_vp * self.n)(*[_device_ptr(p, int(np.prod(s))) for p, s in zip(P, self.shapes)])
+        _check(lib().dnsp_batch_bind(self._h, pr, pp))
+        self._bound = (list(rhs), list(P))  # keep the tensors alive
+
+    def run(self, stream=None):
+        _check(lib().dnsp_batch_run(self._h, _vp(_stream_ptr(stream))))
+
+    def run_mixed(self, refinements=1, stream=None):
+        _check(lib().dnsp_batch_run_mixed(self._h, int(refinements), _vp(_stream_ptr(stream))))
+
+    def autotune(self, reps=3, stream=None):
+        times = np.zeros(TUNE_SLOTS)
+        _check(lib().dnsp_batch_autotune(self._h, int(reps), _vp(_stream_ptr(stream)), times.ctypes.data_as(_dp)))
+        out = {"x_ms": times[:4].tolist(), "z_ms": times[4:8].tolist(), "tri_ms": times[8:10].tolist()}
+        if self.precision == PRECISION_MIXED:
+            out.update({"mixed_x_ms": times[12:16].tolist(), "mixed_z_ms": times[16:20].tolist(),
+                        "mixed_tri_ms": times[20:22].tolist()})
+        out["picked"] = self.variants
+        return out
+
+    @property
+    def variants(self):
+        v = np.zeros(6, dtype=np.int32)
+        _check(lib().dnsp_batch_get_variants(self._h, v.ctypes.data_as(_ip)))
+        return tuple(int(x) for x in v)
+
+    def set_variants(self, xvar, zvar, tvar, mxvar=0, mzvar=0, mtvar=1):
+        _check(lib().dnsp_batch_set_variants(self._h, int(xvar), int(zvar), int(tvar), int(mxvar), int(mzvar),
+                                             int(mtvar)))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().dnsp_batch_free(self._h)
             self._h = None
 
     def __del__(self):

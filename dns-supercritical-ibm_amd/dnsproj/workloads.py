@@ -131,3 +131,29 @@ def reference_style_trials(n_trials=10, seed=0):
         yu = np.concatenate([[0.0], np.cumsum(rng.uniform(0.1, 0.2, size=ny))]) + rng.uniform(-0.2, 0.2)
         out.append(dict(nx=nx, nz=nz, ny=ny, inv_dx=1.0 / dx, inv_dz=1.0 / dz, yu=yu, rhs=f))
     return out
+
+
+CONFIG3_NX = (128, 192, 256, 384, 512)
+CONFIG3_NY = (96, 128, 192, 256)
+CONFIG3_NZ = (64, 96, 128, 192, 256)
+
+
+def config3(n=48, seed=2, with_rhs=True):
+    """Batched uneven instances: (nx, ny, nz) drawn from default_rng(seed) over
+    CONFIG3_NX x CONFIG3_NY x CONFIG3_NZ, tanh faces with stretch U(1.2, 2.2),
+    RHS U(-1,1) (w-weighted mean removed), instance i's RHS from seed
+    (seed, i) so any subset can be regenerated alone."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nx = int(rng.choice(CONFIG3_NX))
+        ny = int(rng.choice(CONFIG3_NY))
+        nz = int(rng.choice(CONFIG3_NZ))
+        gamma = float(rng.uniform(1.2, 2.2))
+        yu = tanh_faces(ny, gamma)
+        c = dict(nx=nx, nz=nz, ny=ny, inv_dx=nx / LX, inv_dz=nz / LZ, yu=yu, gamma=gamma)
+        if with_rhs:
+            f = np.random.default_rng([seed, i]).uniform(-1.0, 1.0, size=(ny, nz, nx))
+            c["rhs"] = remove_weighted_mean(f, yu)
+        out.append(c)
+    return out

@@ -172,6 +172,42 @@ int dnsp_poisson_run_mixed(dnsp_poisson* h, double* P, int refinements, void* st
 int dnsp_poisson_solve_refine(dnsp_poisson* h, const double* rhs, double* P, double tol, int max_refinements,
                               int* refinements_done, double* rel_resid, void* stream);
 
+/*
+ * Batched form: many independent solves of uneven sizes (no counterpart in
+ * the reference, which solves one field per rank).  Each stage of the solve
+ * is one launch per distinct transform length (x, z) or rows-per-chunk
+ * (tridiagonal) over the tiles of all instances of that size.
+ *   dnsp_batch_create: n instances; nx[i], nz[i] in {32,64,96,128,192,256,384,
+ *     512,768}, 2 <= ny[i] <= 1024; bands = the instances' six band arrays
+ *     (a, b, c, a00, b00, c00, ny[i] each) concatenated in instance order;
+ *     precision as for dnsp_poisson_init (MIXED adds two FP32 fields per
+ *     instance).
+ *   dnsp_batch_bind: the device arrays of every instance (host arrays of n
+ *     device pointers; rhs[i] != P[i]); synchronous, done once.
+ *   dnsp_batch_run: FP64 solves P[i] = L_i^{-1} rhs[i] of all instances.
+ *   dnsp_batch_run_mixed: mixed-precision solves, `refinements` 0 or 1.
+ *   Both enqueue on `stream` only (no allocation, no synchronisation), so a
+ *   caller may capture them in a graph.
+ *   dnsp_batch_autotune: times the whole batched solve per variant (x, z:
+ *     0..3 as dnsp_poisson_set_variants; tridiagonal: 0 = 8-mode groups,
+ *     1 = the same with registers capped for 4 waves per SIMD), keeps the
+ *     fastest; overwrites the bound P arrays; times_ms as for
+ *     dnsp_poisson_autotune.  Synchronises.
+ *   dnsp_batch_set_variants / get_variants: FP64 x, z, tridiagonal, then the
+ *     mixed form's (6 ints).
+ */
+typedef struct dnsp_batch dnsp_batch;
+int dnsp_batch_create(int n, const int* nx, const int* nz, const int* ny, const double* inv_dx,
+                      const double* inv_dz, const double* bands, int precision, dnsp_batch** out);
+int dnsp_batch_bind(dnsp_batch* b, const double* const* rhs, double* const* P);
+int dnsp_batch_run(dnsp_batch* b, void* stream);
+int dnsp_batch_run_mixed(dnsp_batch* b, int refinements, void* stream);
+int dnsp_batch_autotune(dnsp_batch* b, int reps, void* stream, double* times_ms);
+int dnsp_batch_set_variants(dnsp_batch* b, int xvar, int zvar, int tvar, int mxvar, int mzvar, int mtvar);
+int dnsp_batch_get_variants(const dnsp_batch* b, int* v6);
+int dnsp_batch_size(const dnsp_batch* b, int* n);
+int dnsp_batch_free(dnsp_batch* b);
+
 /* Relative residual ||rhs - L P|| / ||rhs|| (||rhs - L P|| if rhs = 0) of the
  * discrete operator the solve inverts: periodic second differences in x and
  * z, the plain y bands, and the (0,0)-mode bands on the xz-plane means.  FP64,

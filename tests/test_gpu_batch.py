@@ -1,0 +1,198 @@
+"""GPU checks of the batched form (many uneven instances per launch) against
+the single-instance solver and the CPU oracle.
+
+Every instance of a batched FP64 solve must equal the single-instance solve
+with the same kernel variants bit for bit (the batched launches run the same
+tile and mode-group computations), and the oracle within 1e-12 (mean-removed
+max-abs, relative).  The mixed-precision batch must equal the single-instance
+mixed solve bit for bit and reach a relative residual <= 1e-10 with one
+refinement.  A graph capturing the batched solve must reproduce the eager
+result on every replay.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import dnsproj as D
+from dnsproj import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def to_dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to("cuda:0")
+
+
+def to_host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def rel_mean_removed(got, ref):
+    g = got - got.mean()
+    r = ref - ref.mean()
+    return np.abs(g - r).max() / max(np.abs(r).max(), 1e-300)
+
+
+def small_cases(seed, n=7):
+    """Uneven small instances over the compile-time FFT lengths."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nx = int(rng.choice([32, 64, 96, 128, 192]))
+        nz = int(rng.choice([32, 64, 96, 128]))
+        ny = int(rng.choice([2, 5, 17, 40, 64, 96, 130]))
+        yu = W.tanh_faces(ny, float(rng.uniform(1.2, 2.2)))
+        bands = D.mk_P_abc(yu, False) + D.mk_P_abc(yu, True)
+        rhs = W.rhs_uniform((ny, nz, nx), 100 * seed + i, yu)
+        out.append(dict(nx=nx, nz=nz, ny=ny, inv_dx=nx / W.LX, inv_dz=nz / W.LZ, yu=yu, bands=bands, rhs=rhs))
+    return out
+
+
+def bind(batch, cases):
+    import torch
+    f = [to_dev(c["rhs"]) for c in cases]
+    P = [torch.zeros_like(t) for t in f]
+    batch.bind(f, P)
+    return f, P
+
+
+@pytest.mark.parametrize("tvar", [0, 1])
+def test_batch_fp64_equals_single_and_oracle(cuda, tvar):
+    say("batch fp64", tvar)
+    cases = small_cases(1)
+    b = D.PoissonBatch(cases)
+    f, P = bind(b, cases)
+    for xv, zv in ((0, 0), (2, 1), (3, 3)):
+        b.set_variants(xv, zv, tvar)
+        b.run()
+        for c, fi, Pi in zip(cases, f, P):
+            s = D.PoissonSolver(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], c["bands"])
+            # same kernels, tile decomposition and 8-mode groups as the batch
+            s.set_strategy(D.FFT_FIXED, D.TRIDIAG_PARTITION)
+            s.set_variants(xv, zv, 1 if tvar == 0 else 2)
+            Q = to_dev(np.zeros_like(c["rhs"]))
+            s.solve(fi, Q)
+            got = to_host(Pi)
+            assert np.array_equal(got, to_host(Q)), (c["nx"], c["nz"], c["ny"], xv, zv)
+            s.close()
+    for c, Pi in zip(cases, P):
+        ref = O.solve(c["rhs"], c["inv_dx"], c["inv_dz"], c["bands"])
+        assert rel_mean_removed(to_host(Pi), ref) <= 1e-12, (c["nx"], c["nz"], c["ny"])
+    b.close()
+
+
+def test_batch_mixed_equals_single_and_reaches_target(cuda):
+    say("batch mixed")
+    cases = small_cases(2)
+    b = D.PoissonBatch(cases, precision=D.PRECISION_MIXED)
+    f, P = bind(b, cases)
+    b.set_variants(0, 0, 1, 2, 1, 1)
+    for refinements in (0, 1):
+        b.run_mixed(refinements)
+        for c, fi, Pi in zip(cases, f, P):
+            s = D.PoissonSolver(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], c["bands"],
+                                precision=D.PRECISION_MIXED)
+            s.set_mixed_variants(2, 1, 2)
+            Q = to_dev(np.zeros_like(c["rhs"]))
+            s.solve_mixed(fi, Q, refinements)
+            got = to_host(Pi)
+            assert np.array_equal(got, to_host(Q)), (c["nx"], c["nz"], c["ny"], refinements)
+            r = O.apply_operator(got, c["inv_dx"], c["inv_dz"], c["bands"]) - c["rhs"]
+            rel = np.linalg.norm(r) / np.linalg.norm(c["rhs"])
+            assert rel <= (1e-10 if refinements else 1e-3), (c["nx"], c["nz"], c["ny"], refinements, rel)
+            s.close()
+    b.close()
+
+
+def test_batch_autotune_and_graph_replay(cuda):
+    import torch
+    say("batch tune + graph")
+    cases = small_cases(3, n=5)
+    b = D.PoissonBatch(cases, precision=D.PRECISION_MIXED)
+    f, P = bind(b, cases)
+    table = b.autotune(reps=2)
+    assert all(t > 0 for t in table["x_ms"] + table["z_ms"] + table["tri_ms"] + table["mixed_x_ms"])
+    for form in ("fp64", "mixed"):
+        run = b.run if form == "fp64" else (lambda stream=None: b.run_mixed(1, stream))
+        run()
+        eager = [to_host(p) for p in P]
+        for p in P:
+            p.zero_()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                run(stream=s)
+        torch.cuda.synchronize()
+        for rep in range(3):
+            for p in P:
+                p.fill_(float(rep))  # eager work between replays
+            g.replay()
+            torch.cuda.synchronize()
+            for e, p in zip(eager, P):
+                assert np.array_equal(e, to_host(p)), (form, rep)
+        del g
+    b.close()
+
+
+def test_batch_errors(cuda):
+    say("batch errors")
+    cases = small_cases(4, n=2)
+    with pytest.raises(D.DnspError):
+        D.PoissonBatch([dict(cases[0], nx=7)])
+    b = D.PoissonBatch(cases)
+    with pytest.raises(D.DnspError):
+        b.run()  # nothing bound
+    f = [to_dev(c["rhs"]) for c in cases]
+    with pytest.raises(D.DnspError):
+        b.bind(f, f)  # rhs and P must differ
+    with pytest.raises(D.DnspError):
+        b.run_mixed(1)  # FP64 batch
+    b.close()
+
+
+@pytest.mark.slow
+def test_config3_batch(cuda):
+    """The 48-instance batch: every instance's FP64 solution against the
+    single-instance solver, the mixed form's residual on the device."""
+    import torch
+    say("config3 build")
+    cases = W.config3()
+    for c in cases:
+        c["bands"] = D.mk_P_abc(c["yu"], False) + D.mk_P_abc(c["yu"], True)
+    b = D.PoissonBatch(cases, precision=D.PRECISION_MIXED)
+    f = [to_dev(c["rhs"]) for c in cases]
+    P = [torch.zeros_like(t) for t in f]
+    b.bind(f, P)
+    say("config3 autotune")
+    b.autotune(reps=1)
+    v = b.variants
+    b.run()
+    for c, fi, Pi in zip(cases, f, P):
+        s = D.PoissonSolver(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], c["bands"],
+                            precision=D.PRECISION_MIXED)
+        s.set_strategy(D.FFT_FIXED, D.TRIDIAG_PARTITION)
+        s.set_variants(v[0], v[1], 1 if v[2] == 0 else 2)
+        Q = torch.zeros_like(fi)
+        s.solve(fi, Q)
+        assert torch.equal(Q, Pi)
+        assert s.residual(fi, Pi) <= 1e-10
+        s.close()
+    say("config3 mixed")
+    b.run_mixed(1)
+    for c, fi, Pi in zip(cases, f, P):
+        s = D.PoissonSolver(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], c["bands"])
+        assert s.residual(fi, Pi) <= 1e-10, (c["nx"], c["nz"], c["ny"])
+        s.close()
+    b.close()
