@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dns-supercritical-ibm_amd"))
 
@@ -38,7 +40,115 @@ def parse():
                     help="rocprofv3 FETCH/WRITE summary to report as roofline.traffic")
     ap.add_argument("--mode", default="bench", choices=["bench", "counters"],
                     help="counters: one 906 MB device copy (calibration) then 3 solves, for rocprofv3 --pmc")
+    ap.add_argument("--extras", default="batch,step,serve",
+                    help="comma list of extra workloads to measure: batch (48 uneven instances), step "
+                         "(384x256x192 projection step with IBM), serve (15-case request trace); '' = none")
+    ap.add_argument("--serve-requests", type=int, default=400)
     return ap.parse_args()
+
+
+def bench_batch(torch, D, W):
+    """48 uneven instances (nx in 128..512, ny in 96..256, nz in 64..256, seed 2),
+    FP64 and mixed (one refinement), autotuned; solves/s = instances / time."""
+    cases = W.config3()
+    for c in cases:
+        c["bands"] = D.mk_P_abc(c["yu"], False) + D.mk_P_abc(c["yu"], True)
+    f = [torch.from_numpy(c.pop("rhs")).to("cuda:0") for c in cases]
+    P = [torch.zeros_like(t) for t in f]
+    b = D.PoissonBatch(cases, precision=D.PRECISION_MIXED)
+    b.bind(f, P)
+    b.autotune(reps=2)
+
+    def timed(fn, K=10):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(K):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / K
+
+    ms64 = timed(b.run)
+    msm = timed(lambda: b.run_mixed(1))
+    worst = 0.0  # of the mixed results just timed
+    for c, fi, Pi in zip(cases, f, P):
+        s = D.PoissonSolver(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], c["bands"])
+        worst = max(worst, s.residual(fi, Pi))
+        s.close()
+    n = len(cases)
+    out = {"workload": "48 independent solves, (nx,ny,nz) from {128..512}x{96..256}x{64..256} seed 2, "
+                       "tanh faces U(1.2,2.2), RHS U(-1,1)", "instances": n,
+           "points": int(sum(t.numel() for t in f)),
+           "fp64_ms": round(ms64, 4), "fp64_solves_per_s": round(n / ms64 * 1e3, 1),
+           "mixed_ms": round(msm, 4), "mixed_solves_per_s": round(n / msm * 1e3, 1),
+           "mixed_worst_rel_residual": worst, "variants": list(b.variants)}
+    b.close()
+    del f, P
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_step(torch, D, W):
+    """384x256x192 variable-property projection step with a k/delta = 1/6 rough
+    wall, captured in one graph: FP64 vs mixed."""
+    from dnsproj import projection as DP
+    c = W.config4()
+    res = {}
+    outs = {}
+    for name, prec, ref in (("fp64", 0, -1), ("mixed", D.PRECISION_MIXED, 1)):
+        s = DP.ProjectionStep.from_case(c, precision=prec)
+        s.load(c["fields"])
+        st = torch.cuda.Stream()
+        s.capture(ref, stream=st)
+        for _ in range(3):
+            s.launch(stream=st)
+        st.synchronize()
+        K = 20
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(K):
+            s.launch(stream=st)
+        e1.record(st)
+        st.synchronize()
+        res[name + "_graph_ms"] = round(e0.elapsed_time(e1) / K, 4)
+        s.load(c["fields"])
+        s.launch(stream=st)
+        st.synchronize()
+        outs[name] = s.read("U")
+        s.close()
+    d = (outs["mixed"] - outs["fp64"]).abs().max() / outs["fp64"].abs().max()
+    res.update({"workload": "variable-property projection step nx=384 ny=256 nz=192, rho ratio ~10, rough "
+                            "bottom wall k/delta=1/6 (64 Fourier modes, seed 3), one graph",
+                "mixed_U_err_vs_fp64": float(d)})
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_serve(torch, D, W, n_req):
+    """15-case ensemble, Poisson arrivals 200/s (seed 4), mixed solves (one
+    refinement), 8 slots, all cases within the budget after a warm pass."""
+    from dnsproj import serving as SV
+    cases = W.config5_cases()
+    srv = SV.Server(200 << 30, max_slots=8, precision=D.PRECISION_MIXED, refinements=1)
+    for n, c in enumerate(cases):
+        srv.add_case(c, seed=5000 + n)
+    ids = np.arange(len(cases), dtype=np.int32)
+    srv.run(ids, np.zeros(len(cases)), np.arange(len(cases), dtype=np.uint64))  # admissions
+    t, cid, sd = W.poisson_arrivals(n_req, 200.0, 4, len(cases))
+    out = srv.run(cid, t, sd + 1000)
+    lat = out["latency_ms"]
+    st = out["stats"]
+    srv.close()
+    torch.cuda.empty_cache()
+    return {"workload": "15 channel cases Ri_b x k/delta, grids 256x192x128..768x384x384, projection-step "
+                        "requests, Poisson arrivals 200/s seed 4, 8 slots",
+            "requests": int(n_req), "p50_ms": round(float(np.percentile(lat, 50)), 4),
+            "p99_ms": round(float(np.percentile(lat, 99)), 4), "mean_ms": round(float(lat.mean()), 4),
+            "solves_per_s": round(n_req / st["wall_s"], 2), "offered_per_s": 200.0,
+            "peak_resident_GB": round(st["peak_resident_bytes"] / 1e9, 3)}
 
 
 def main():
@@ -184,6 +294,18 @@ def main():
     hw = wall if head == "fp64" else wall_m
     hl = lat if head == "fp64" else lat_m
 
+    extras = {}
+    for name in [e for e in args.extras.split(",") if e]:
+        try:
+            if name == "batch":
+                extras["batched"] = bench_batch(torch, D, W)
+            elif name == "step":
+                extras["projection_step"] = bench_step(torch, D, W)
+            elif name == "serve":
+                extras["serving"] = bench_serve(torch, D, W, args.serve_requests)
+        except Exception as e:  # reported, never hides the headline
+            extras[name + "_error"] = repr(e)
+
     out = {
         "metric": METRIC,
         "value": forms[head]["solves_per_s"],
@@ -212,6 +334,7 @@ def main():
         "solve_roofline": solve_roof,
         "cpu_baseline": cpu,
     }
+    out.update(extras)
     print(json.dumps(out), flush=True)
 
 

@@ -18,10 +18,10 @@ void ct_z_launch(int n, int variant, bool inv, float* S, int nx, int ny, const f
                  double* msum, hipStream_t st);
 
 // Partitioned tridiagonal solve: kPartChunks chunks of part_rows_for(ny) rows.
-// Variants: 0 = 16 modes per workgroup, 1 = 8 modes, one workgroup per mode
-// group; 2 = as 1 with registers capped for 4 resident waves per SIMD; 3 = 8
-// modes, persistent (as many workgroups as are resident at once, each looping
-// over mode groups).
+// Variants: 0 = 16 modes per workgroup with registers capped for 4 resident
+// waves per SIMD, 1 = 8 modes, one workgroup per mode group; 2 = as 1 with
+// registers capped for 4 resident waves per SIMD; 3 = 8 modes, persistent (as
+// many workgroups as are resident at once, each looping over mode groups).
 constexpr int kPartChunks = 32;
 constexpr int kPartVariants = 4;
 int part_rows_for(int ny);

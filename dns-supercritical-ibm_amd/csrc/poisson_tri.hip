@@ -84,7 +84,7 @@ void part_launch(int variant, int mr, T* S, const double* ax, const double* az, 
                  double* pbar, double inv_nxnz, hipStream_t st) {
     switch (variant) {
         case 0:
-            part_launch_m<16, T, false, 1>(mr, S, ax, az, i0x, i0z, bands, nx, nz, ny, pcr_min, ncu, msum, pbar,
+            part_launch_m<16, T, false, 4>(mr, S, ax, az, i0x, i0z, bands, nx, nz, ny, pcr_min, ncu, msum, pbar,
                                         inv_nxnz, st);
             break;
         case 1:

@@ -222,3 +222,136 @@ def ibm_part2(A, bot, top, kmax, wall_bot, wall_top, stream=None):
 
 
 __all__ = ["DnspError"] + [n for n in dir() if not n.startswith("_")]
+
+
+# ---------------------------------------------------------------------------
+# projection step object (include/dnsproj.h: dnsp_step_*)
+# ---------------------------------------------------------------------------
+STEP_FIELDS = ("U_hat", "V_hat", "W_hat", "U", "V", "W", "rho", "P_now", "P_old", "div_now", "P_per", "P")
+
+_STEP_SIGS = [
+    ("dnsp_step_create", [_i, _i, _i, _dp, _d, _d, _d, _i, _i, ctypes.POINTER(_vp)]),
+    ("dnsp_step_field", [_vp, _i, ctypes.POINTER(_vp)]),
+    ("dnsp_step_write", [_vp, _i, _vp, _vp]),
+    ("dnsp_step_read", [_vp, _i, _vp, _vp]),
+    ("dnsp_step_set_walls", [_vp, _dp]),
+    ("dnsp_step_set_ibm", [_vp, _i, _vp]),
+    ("dnsp_step_set_columns", [_vp, _ip, _ip]),
+    ("dnsp_step_poisson", [_vp, ctypes.POINTER(_vp)]),
+    ("dnsp_step_footprint", [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    ("dnsp_step_enqueue", [_vp, _i, _vp]),
+    ("dnsp_step_capture", [_vp, _i, _vp]),
+    ("dnsp_step_launch", [_vp, _vp]),
+    ("dnsp_step_free", [_vp]),
+]
+_SIGS.extend(_STEP_SIGS)
+
+
+class IbmForcingSet:
+    """IBM handles of U, V, W for one case (records from workloads.rough_wall_ibm)."""
+
+    def __init__(self, nx, nz, ny, records):
+        self.handles = {}
+        for comp in ("U", "V", "W"):
+            ijk, c = records[comp]
+            self.handles[comp] = IbmForcing(nx, nz, ny, 1, 0, ijk, c)
+
+    def close(self):
+        for h in self.handles.values():
+            h.close()
+        self.handles = {}
+
+
+class ProjectionStep:
+    """Resident fields of one channel case and its projection step on the device."""
+
+    def __init__(self, nx, nz, ny, yu, inv_dx, inv_dz, dtmax, incompressible=False, precision=0):
+        L = _lib()
+        yu = np.ascontiguousarray(yu, dtype=np.float64)
+        if yu.size != ny + 1:
+            raise ValueError("yu must hold ny+1 faces")
+        h = _vp()
+        _check(L.dnsp_step_create(int(nx), int(nz), int(ny), yu.ctypes.data_as(_dp), float(inv_dx), float(inv_dz),
+                                  float(dtmax), int(bool(incompressible)), int(precision), ctypes.byref(h)))
+        self._h = h
+        self.nx, self.nz, self.ny = int(nx), int(nz), int(ny)
+        self._ibm = None
+
+    @classmethod
+    def from_case(cls, c, precision=0, with_ibm=True):
+        s = cls(c["nx"], c["nz"], c["ny"], c["yu"], c["inv_dx"], c["inv_dz"], c["dtmax"], precision=precision)
+        s.set_walls(c["walls"])
+        if with_ibm and c.get("ibm") is not None:
+            s.set_ibm(IbmForcingSet(c["nx"], c["nz"], c["ny"], c["ibm"]))
+            s.set_columns(c["bot"], c["top"])
+        return s
+
+    def shape(self, name):
+        if name == "P":
+            return (self.ny, self.nz, self.nx)
+        return (self.ny + 2, self.nz + 2, self.nx + 2)
+
+    def write(self, name, t, stream=None):
+        _check(_lib().dnsp_step_write(self._h, STEP_FIELDS.index(name), _dev(t, self.shape(name)),
+                                      _vp(_stream_ptr(stream))))
+
+    def read(self, name, stream=None):
+        import torch
+        out = torch.empty(self.shape(name), dtype=torch.float64, device="cuda")
+        _check(_lib().dnsp_step_read(self._h, STEP_FIELDS.index(name), _vp(out.data_ptr()), _vp(_stream_ptr(stream))))
+        return out
+
+    def load(self, fields, stream=None):
+        """Writes host arrays (dict name -> numpy array) into the resident fields."""
+        import torch
+        for name, a in fields.items():
+            self.write(name, torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to("cuda"), stream)
+
+    def set_walls(self, walls):
+        w = np.ascontiguousarray(walls, dtype=np.float64)
+        _check(_lib().dnsp_step_set_walls(self._h, w.ctypes.data_as(_dp)))
+
+    def set_ibm(self, ibm_set):
+        self._ibm = ibm_set  # the handles must outlive the step
+        for k, comp in enumerate(("U", "V", "W")):
+            h = ibm_set.handles.get(comp) if ibm_set else None
+            _check(_lib().dnsp_step_set_ibm(self._h, k, h._h if h is not None else _vp(None)))
+
+    def set_columns(self, bot, top):
+        b = np.ascontiguousarray(bot, dtype=np.int32)
+        t = np.ascontiguousarray(top, dtype=np.int32)
+        _check(_lib().dnsp_step_set_columns(self._h, b.ctypes.data_as(_ip), t.ctypes.data_as(_ip)))
+
+    @property
+    def footprint(self):
+        v = ctypes.c_size_t()
+        _check(_lib().dnsp_step_footprint(self._h, ctypes.byref(v)))
+        return v.value
+
+    def poisson_handle(self):
+        v = _vp()
+        _check(_lib().dnsp_step_poisson(self._h, ctypes.byref(v)))
+        return v
+
+    def enqueue(self, refinements=-1, stream=None):
+        _check(_lib().dnsp_step_enqueue(self._h, int(refinements), _vp(_stream_ptr(stream))))
+
+    def capture(self, refinements=-1, stream=None):
+        _check(_lib().dnsp_step_capture(self._h, int(refinements), _vp(_stream_ptr(stream))))
+
+    def launch(self, stream=None):
+        _check(_lib().dnsp_step_launch(self._h, _vp(_stream_ptr(stream))))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            _lib().dnsp_step_free(self._h)
+            self._h = None
+        if self._ibm is not None:
+            self._ibm.close()
+            self._ibm = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -157,3 +157,133 @@ def config3(n=48, seed=2, with_rhs=True):
             c["rhs"] = remove_weighted_mean(f, yu)
         out.append(c)
     return out
+
+
+def rough_height_map(nx, nz, k, seed, nmodes=64, lx=LX, lz=LZ, kmax=8):
+    """Rough-wall height on the (x, z) cell centres: a sum of `nmodes` random
+    Fourier modes a cos(th) + b sin(th), th = 2 pi (m x/lx + n z/lz), m, n in
+    [1, kmax] from default_rng(seed), shifted and scaled to [0, k]."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) * lx / nx
+    z = (np.arange(nz) + 0.5) * lz / nz
+    h = np.zeros((nz, nx))
+    for _ in range(nmodes):
+        m, n = (int(v) for v in rng.integers(1, kmax + 1, size=2))
+        a, b = rng.standard_normal(2)
+        th = 2 * np.pi * (m * x[None, :] / lx + n * z[:, None] / lz)
+        h += a * np.cos(th) + b * np.sin(th)
+    h -= h.min()
+    return h * (k / h.max())
+
+
+def rough_wall_ibm(nx, nz, ny, yu, h):
+    """Ghost-cell forcing records for a bottom rough wall of height h(z, x):
+    per velocity component (U at x faces, W at z faces, V at y faces, the
+    others at cell centres) the highest cell below the surface whose upper
+    neighbour is fluid is a target; its value is the linear extrapolation
+    through zero at the surface from one fluid neighbour above (the nearer
+    one unless it sits within a quarter cell of the surface).  Records use
+    the reference layout (target, source 1, source 2 as (i, j, k); c = (cw,
+    c1, c2)).  Cells below every component's target are wall columns (bot);
+    there is no top roughness (top = ny + 1)."""
+    yu = np.asarray(yu, dtype=np.float64)
+    yp = 0.5 * (yu[:-1] + yu[1:])
+    hx = 0.5 * (h + np.roll(h, 1, axis=1))   # at x faces (i - 1/2)
+    hz = 0.5 * (h + np.roll(h, 1, axis=0))   # at z faces (j - 1/2)
+    comps = {"U": (hx, yp), "V": (h, yu[:ny + 1]), "W": (hz, yp)}
+    out, kg_min = {}, np.full((nz, nx), ny, dtype=np.int64)
+    jj, ii = np.meshgrid(np.arange(nz), np.arange(nx), indexing="ij")
+    for name, (hh, ys) in comps.items():
+        kg = np.searchsorted(ys, hh, side="left") - 1   # highest k with ys[k] < h
+        sel = kg >= 0
+        if np.any(kg[sel] + 2 >= min(len(ys), ny)):
+            raise ValueError("roughness too tall for the grid")
+        kg_s, hv, i_s, j_s = kg[sel], hh[sel], ii[sel], jj[sel]
+        yg, y1, y2 = ys[kg_s], ys[kg_s + 1], ys[kg_s + 2]
+        near = (y1 - hv) >= 0.25 * (y1 - yg)
+        c1 = np.where(near, -(hv - yg) / (y1 - hv), 0.0)
+        c2 = np.where(near, 0.0, -(hv - yg) / (y2 - hv))
+        ijk = np.stack([i_s, j_s, kg_s, i_s, j_s, kg_s + 1, i_s, j_s, kg_s + 2], axis=1).astype(np.int32)
+        cc = np.stack([np.zeros_like(c1), c1, c2], axis=1)
+        out[name] = (np.ascontiguousarray(ijk), np.ascontiguousarray(cc))
+        kg_min[sel] = np.minimum(kg_min[sel], kg_s)
+    bot = np.where(kg_min < ny, kg_min - 1, -1).astype(np.int32)
+    top = np.full((nz, nx), ny + 1, dtype=np.int32)
+    return out, bot, top
+
+
+def rho_of_T(T):
+    """rho(T) = 1 - 0.45 (1 + tanh((T - 0.5)/0.05)): density ratio ~10."""
+    return 1.0 - 0.45 * (1.0 + np.tanh((T - 0.5) / 0.05))
+
+
+def projection_case(nx, nz, ny, seed, gamma=1.8, k_rough=1.0 / 6.0, dtmax=1e-3):
+    """Inputs of one variable-property projection step: grid, halo fields
+    (ny+2, nz+2, nx+2) for the predictor velocities, rho from a wall-normal
+    temperature profile (0 at the bottom wall, 1 at the top) with 5% random
+    perturbations, previous pressures and divergence; a bottom rough wall of
+    height k_rough (sum of 64 Fourier modes, seed) as IBM records and column
+    slabs (none when k_rough == 0)."""
+    rng = np.random.default_rng(seed)
+    yu = tanh_faces(ny, gamma)
+    yp = 0.5 * (yu[:-1] + yu[1:])
+    sh = (ny + 2, nz + 2, nx + 2)
+    ypad = np.concatenate([[2 * yu[0] - yp[0]], yp, [2 * yu[-1] - yp[-1]]])
+    T = (ypad / LY)[:, None, None] + 0.05 * rng.standard_normal(sh)
+    U_prof = 1.5 * (1.0 - (ypad - 1.0) ** 2)
+    f = {
+        "U_hat": U_prof[:, None, None] + 0.1 * rng.standard_normal(sh),
+        "V_hat": 0.1 * rng.standard_normal(sh),
+        "W_hat": 0.1 * rng.standard_normal(sh),
+        "rho": rho_of_T(T),
+        "P_now": 0.01 * rng.standard_normal(sh),
+        "P_old": 0.01 * rng.standard_normal(sh),
+        "div_now": 0.01 * rng.standard_normal(sh),
+    }
+    c = dict(nx=nx, nz=nz, ny=ny, yu=yu, inv_dx=nx / LX, inv_dz=nz / LZ, dtmax=dtmax, fields=f,
+             walls=np.zeros(6), k_rough=k_rough)
+    if k_rough > 0:
+        h = rough_height_map(nx, nz, k_rough, seed)
+        c["ibm"], c["bot"], c["top"] = rough_wall_ibm(nx, nz, ny, yu, h)
+    return c
+
+
+def config4(seed=3):
+    """384 x 256 x 192 (nx, ny, nz) variable-property projection step with a
+    rough bottom wall, k/delta = 1/6 (tanh faces, stretch 1.8)."""
+    return projection_case(384, 192, 256, seed)
+
+
+CONFIG5_RI = (-1.0, -0.25, 0.0, 0.25, 1.0)
+CONFIG5_K = (0.0, 1.0 / 12.0, 1.0 / 6.0)
+CONFIG5_GRIDS = ((256, 128, 192), (384, 192, 256), (512, 256, 256), (768, 384, 384), (384, 128, 192),
+                 (512, 192, 384))  # (nx, nz, ny), from 256x192x128 to 768x384x384 (nx x ny x nz)
+
+
+def config5_cases(seed=4):
+    """The 15-case ensemble Ri_b x k/delta; each case's grid drawn from
+    CONFIG5_GRIDS (default_rng(seed)), tanh faces (stretch 1.8), a rough
+    bottom wall of height k (k = 0: smooth, no IBM).  Ri_b only enters the
+    momentum equation, outside the projection step; it is kept as a label."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for ri in CONFIG5_RI:
+        for k in CONFIG5_K:
+            nx, nz, ny = CONFIG5_GRIDS[int(rng.integers(len(CONFIG5_GRIDS)))]
+            yu = tanh_faces(ny, 1.8)
+            c = dict(nx=nx, nz=nz, ny=ny, yu=yu, inv_dx=nx / LX, inv_dz=nz / LZ, dtmax=1e-3,
+                     walls=np.zeros(6), Ri_b=ri, k_rough=k, ibm=None, bot=None, top=None)
+            if k > 0:
+                h = rough_height_map(nx, nz, k, int(rng.integers(1 << 30)))
+                c["ibm"], c["bot"], c["top"] = rough_wall_ibm(nx, nz, ny, yu, h)
+            out.append(c)
+    return out
+
+
+def poisson_arrivals(n, rate, seed, n_cases):
+    """n requests: exponential inter-arrival times of mean 1/rate (s), case
+    uniform over n_cases, request seed = index, from default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    cases = rng.integers(0, n_cases, size=n)
+    return t - t[0], cases.astype(np.int32), np.arange(n, dtype=np.uint64)

@@ -14,6 +14,8 @@
 #ifndef DNSPROJ_H
 #define DNSPROJ_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -127,10 +129,11 @@ int dnsp_poisson_get_strategy(const dnsp_poisson* h, int* fft_ct_x, int* fft_ct_
 /* Kernel variants.  xvar, zvar (0..3): the compile-time-sized FFT kernels'
  * (complex elements per workgroup tile, threads) = (3072,256), (3072,512),
  * (1536,256), (6144,512); ignored where a length has no such kernels.
- * tvar (0..3): partitioned tridiagonal workgroups of 16 or 8 modes (x 32
- * chunks), one per mode group (0, 1); 8 modes with registers capped for 4
- * resident waves per SIMD (2); 8 modes, persistent (3: as many workgroups as
- * are resident at once, each looping over mode groups).  The
+ * tvar (0..3): partitioned tridiagonal workgroups (x 32 chunks) of 16 modes
+ * with registers capped for 4 resident waves per SIMD (0), of 8 modes (1),
+ * of 8 modes capped the same way (2), one per mode group; 8 modes,
+ * persistent (3: as many workgroups as are resident at once, each looping
+ * over mode groups).  The
  * mixed-precision form has its own set.  No counterpart in the reference. */
 int dnsp_poisson_set_variants(dnsp_poisson* h, int xvar, int zvar, int tvar);
 int dnsp_poisson_get_variants(const dnsp_poisson* h, int* xvar, int* zvar, int* tvar);
@@ -343,6 +346,108 @@ int dnsp_ibm_read_slab(const char* path, int nx, int nz, int ny, int k_first, in
  * kmax = ny-1 for U, W, T and ny for V.  bot, top: device nx*nz ints. */
 int dnsp_ibm_part2(double* A, const int* bot, const int* top, int nx, int nz, int ny, int kmax, double wall_bot,
                    double wall_top, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Projection step (no single counterpart: the reference's time loop,       */
+/* src/main.f90, calls these routines one by one).  The object owns the      */
+/* fields of one channel case on the device and enqueues, on one stream:     */
+/* rho_min + the variable-density right-hand side (or the incompressible     */
+/* one), the Poisson solve (FP64, or the mixed form with 0 or 1 refinement), */
+/* update_vars (kmin_Vhat = 1), the IBM forcing of U, V, W (where a handle   */
+/* is set) and the rough-wall columns (where slabs are set).                 */
+/* ------------------------------------------------------------------------ */
+typedef struct dnsp_step dnsp_step;
+
+/* Field indices for dnsp_step_field: halo arrays (nx+2)(nz+2)(ny+2) except
+ * DNSP_STEP_P, the (nx, nz, ny) pressure (right-hand side, then solution). */
+#define DNSP_STEP_U_HAT    0
+#define DNSP_STEP_V_HAT    1
+#define DNSP_STEP_W_HAT    2
+#define DNSP_STEP_U        3
+#define DNSP_STEP_V        4
+#define DNSP_STEP_W        5
+#define DNSP_STEP_RHO      6
+#define DNSP_STEP_P_NOW    7
+#define DNSP_STEP_P_OLD    8
+#define DNSP_STEP_DIV_NOW  9
+#define DNSP_STEP_P_PER    10
+#define DNSP_STEP_P        11
+#define DNSP_STEP_NFIELDS  12
+
+/* yu: the ny+1 wall-normal faces (bands and metric coefficients follow from
+ * them); precision as for dnsp_poisson_init; fields start at zero. */
+int dnsp_step_create(int nx, int nz, int ny, const double* yu, double inv_dx, double inv_dz, double dtmax,
+                     int use_incompressible, int precision, dnsp_step** out);
+/* Device address of a field (the caller fills the inputs). */
+int dnsp_step_field(dnsp_step* s, int which, double** ptr);
+/* Device-to-device copy of a whole field from src / to dst, on `stream`. */
+int dnsp_step_write(dnsp_step* s, int which, const double* src, void* stream);
+int dnsp_step_read(dnsp_step* s, int which, double* dst, void* stream);
+/* Wall values U_bot, V_bot, W_bot, U_top, V_top, W_top (host array of 6). */
+int dnsp_step_set_walls(dnsp_step* s, const double* wall_bc);
+/* IBM forcing of component 0 (U), 1 (V), 2 (W): a handle created with
+ * nhalo = 1, nskip = 0 (borrowed, must outlive the step); NULL removes it. */
+int dnsp_step_set_ibm(dnsp_step* s, int component, dnsp_ibm* ibm);
+/* Rough-wall column slabs (host nx*nz ints each, as dnsp_ibm_part2); NULL
+ * removes them. */
+int dnsp_step_set_columns(dnsp_step* s, const int* bot, const int* top);
+/* The embedded Poisson solver (for tuning; owned by the step). */
+int dnsp_step_poisson(dnsp_step* s, dnsp_poisson** h);
+/* Device bytes the step holds (fields, geometry, solver work arrays). */
+int dnsp_step_footprint(const dnsp_step* s, size_t* bytes);
+/* refinements: -1 = FP64 solve, 0 or 1 = mixed form (precision MIXED). */
+int dnsp_step_enqueue(dnsp_step* s, int refinements, void* stream);
+/* Records the same sequence into a graph (stream capture on a non-null
+ * stream), kept in the step; dnsp_step_launch replays it. */
+int dnsp_step_capture(dnsp_step* s, int refinements, void* stream);
+int dnsp_step_launch(dnsp_step* s, void* stream);
+int dnsp_step_free(dnsp_step* s);
+
+/* ------------------------------------------------------------------------ */
+/* Serving engine (no counterpart in the reference): projection-step         */
+/* requests for an ensemble of channel cases, continuously batched.          */
+/* ------------------------------------------------------------------------ */
+typedef struct dnsp_server dnsp_server;
+
+/* One channel case.  Host arrays are copied.  IBM records per component
+ * (U, V, W) as dnsp_ibm_create (n_ibm[k] = 0: none); bot/top: rough-wall
+ * column slabs (nx*nz ints) or NULL.  A request's inputs are generated on
+ * the device from (seed, request seed). */
+typedef struct {
+    int nx, nz, ny;
+    const double* yu; /* ny+1 faces */
+    double inv_dx, inv_dz, dtmax;
+    int incompressible;
+    double walls[6];
+    int n_ibm[3];
+    const int* ibm_ijk[3];
+    const double* ibm_c[3];
+    const int* bot;
+    const int* top;
+    unsigned long long seed;
+} dnsp_case_desc;
+
+/* hbm_budget: device bytes the resident cases may hold; max_slots (1..16):
+ * requests in flight at once (one stream each); precision and refinements
+ * as for dnsp_step_enqueue. */
+int dnsp_server_create(size_t hbm_budget, int max_slots, int precision, int refinements, dnsp_server** out);
+/* Registers a case (its device state is created on first admission). */
+int dnsp_server_add_case(dnsp_server* sv, const dnsp_case_desc* d, int* case_id);
+int dnsp_server_case_footprint(const dnsp_server* sv, int case_id, size_t* bytes);
+/* Replays n requests (case_id[i], seed[i]) arriving at arrival_s[i] seconds
+ * after the call starts (non-decreasing), against the wall clock: waiting
+ * requests are admitted into free slots in arrival order, skipping those
+ * whose case is busy or does not fit the budget yet; a case is made
+ * resident on first admission (least-recently-used idle cases evicted when
+ * needed) and stays resident otherwise.  Outputs per request: latency_ms
+ * (completion - arrival), start_ms (admission, may be NULL), checksum[4]
+ * (may be NULL: sum and index-weighted sum of P, then of U).  stats (may be
+ * NULL, 4 doubles): admissions, evictions, peak resident bytes, wall s.
+ * Synchronous. */
+int dnsp_server_run(dnsp_server* sv, int n, const int* case_id, const double* arrival_s,
+                    const unsigned long long* seed, double* latency_ms, double* start_ms, double* checksum,
+                    double* stats);
+int dnsp_server_free(dnsp_server* sv);
 
 #ifdef __cplusplus
 }

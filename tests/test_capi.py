@@ -27,7 +27,9 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
     import dnsproj.projection as DP
-    assert set(names) == set(D.exported_names()) | set(DP.exported_names())
+    import dnsproj.serving as SV
+    bound = set(D.exported_names()) | set(DP.exported_names()) | set(SV.exported_names())
+    assert set(names) == bound, (set(names) - bound, bound - set(names))
 
 
 def test_version_and_error_strings():
