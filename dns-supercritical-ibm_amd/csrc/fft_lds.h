@@ -1,4 +1,5 @@
-// In-LDS batched complex FFT engine (FP64) for gfx950.
+// In-LDS batched complex FFT engine (FP64) for gfx950: radices 2, 3, 4, 5, 7,
+// 8, 11, 13.
 //
 // A workgroup holds G complex lines of length n in LDS, line g at
 // buf[g*n .. g*n+n).  The engine runs a mixed-radix Stockham autosort
@@ -241,6 +242,8 @@ __device__ void fft_lds(double2* buf, int G, const FftPlan& pl) {
             case 4: fft_stage<4, INV>(buf, G, pl.n, Ns, pl.tw); break;
             case 5: fft_stage<5, INV>(buf, G, pl.n, Ns, pl.tw); break;
             case 7: fft_stage<7, INV>(buf, G, pl.n, Ns, pl.tw); break;
+            case 11: fft_stage<11, INV>(buf, G, pl.n, Ns, pl.tw); break;
+            case 13: fft_stage<13, INV>(buf, G, pl.n, Ns, pl.tw); break;
             case 8: fft_stage<8, INV>(buf, G, pl.n, Ns, pl.tw); break;
             default: break;  // rejected by the host planner
         }

@@ -291,7 +291,7 @@ static bool fft_factorize(int n, FftPlan& pl) {
     while (r % 8 == 0 && r != 16) if (!push(8)) return false;
     while (r % 4 == 0) if (!push(4)) return false;
     while (r % 2 == 0) if (!push(2)) return false;
-    for (int p : {3, 5, 7})
+    for (int p : {3, 5, 7, 11, 13})
         while (r % p == 0) if (!push(p)) return false;
     return r == 1;
 }
@@ -366,7 +366,7 @@ static int poisson_setup(dnsp_poisson* h, const double* const bands[6]) {
     DNSP_HIP_TRY(hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device));
     const int nx = h->nx, nz = h->nz, ny = h->ny;
     if (!fft_factorize(nx, h->px) || !fft_factorize(nz, h->pz)) {
-        set_error("nx=%d / nz=%d: FFT lengths must factor into 2, 3, 5 and 7", nx, nz);
+        set_error("nx=%d / nz=%d: FFT lengths must factor into 2, 3, 5, 7, 11 and 13", nx, nz);
         return DNSP_ERR_UNSUPPORTED;
     }
     const int limx = fft_radix_limit(h->px), limz = fft_radix_limit(h->pz);

@@ -25,7 +25,7 @@ extern "C" {
 #define DNSP_ERR_HIP          2  /* a HIP runtime call failed */
 #define DNSP_ERR_NOMEM        3  /* device allocation failed */
 #define DNSP_ERR_SINGULAR     4  /* a zero or non-finite tridiagonal pivot */
-#define DNSP_ERR_UNSUPPORTED  5  /* FFT length with a prime factor the kernels do not handle */
+#define DNSP_ERR_UNSUPPORTED  5  /* FFT length with a prime factor above 13, or a form's size limits */
 
 typedef struct dnsp_poisson dnsp_poisson;
 
@@ -44,7 +44,7 @@ int dnsp_stream_synchronize(void* stream);
  * (src/poisson_solver_mgpu_dtdma/poisson_solver_multigpu.f90:33-326), single
  * y-rank branch (mpi_divs_y == 1, :162-222), and the size checks of
  * alloc_poisson_solver_multigpu.f90:3-38.
- *   nx_glob, nz_glob : periodic sizes (any size >= 1 whose prime factors are in {2,3,5,7})
+ *   nx_glob, nz_glob : periodic sizes (any size >= 1 whose prime factors are in {2,3,5,7,11,13})
  *   ny               : wall-normal size (>= 2)
  *   inv_dx, inv_dz   : 1/dx, 1/dz (the reference's inv_dx, inv_dz)
  *   P_band_*         : y bands (length ny) as read from geom_data/P_band_abc.dat,
@@ -346,6 +346,27 @@ int dnsp_ibm_read_slab(const char* path, int nx, int nz, int ny, int k_first, in
  * kmax = ny-1 for U, W, T and ny for V.  bot, top: device nx*nz ints. */
 int dnsp_ibm_part2(double* A, const int* bot, const int* top, int nx, int nz, int ny, int kmax, double wall_bot,
                    double wall_top, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* On-disk formats (host only).                                              */
+/* ------------------------------------------------------------------------ */
+/* Field files, cfd_read_arr / cfd_write_arr (src/mod_io.f90:7-79, :81-150):
+ * int32 nx, nz, ny then nx*nz*ny FP64 values, x fastest.  halo = 0: A is
+ * (nx, nz, ny); halo = 1: A is a (nx+2)(nz+2)(ny+2) halo array and only its
+ * interior is read or written.  A header that differs from (nx, nz, ny) is
+ * DNSP_ERR_ARG ("Mismatch in grid params", the reference's error stop). */
+int dnsp_io_field_shape(const char* path, int* nx, int* nz, int* ny);
+int dnsp_io_read_field(const char* path, int nx, int nz, int ny, int halo, double* A);
+int dnsp_io_write_field(const char* path, int nx, int nz, int ny, int halo, const double* A);
+/* Coefficient text files: a count line, then rows of numbers
+ * (calc_bands.py:358-364).  read_column: read_column_r8
+ * (src/mod_utils.f90:14-35): out[0..n) = column icol (0..3) of rows
+ * n_prev .. n_prev+n-1.  write_table: nrows x ncols, row-major. */
+int dnsp_io_read_column(const char* path, int icol, int n_prev, int n, double* out);
+int dnsp_io_write_table(const char* path, int nrows, int ncols, const double* rows);
+/* yu.txt, one face per line (calc_bands.py:211): *ny = faces - 1; yu (may be
+ * NULL to query the size) receives ny+1 values. */
+int dnsp_io_read_faces(const char* path, int* ny, double* yu);
 
 /* ------------------------------------------------------------------------ */
 /* Projection step (no single counterpart: the reference's time loop,       */

@@ -28,7 +28,9 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, n), n
     import dnsproj.projection as DP
     import dnsproj.serving as SV
-    bound = set(D.exported_names()) | set(DP.exported_names()) | set(SV.exported_names())
+    import dnsproj.io as IO
+    bound = (set(D.exported_names()) | set(DP.exported_names()) | set(SV.exported_names()) |
+             set(IO.exported_names()))
     assert set(names) == bound, (set(names) - bound, bound - set(names))
 
 

@@ -60,7 +60,7 @@ def case_grid(nx, nz, ny, seed):
 STAGE_CASES = [(64, 32, 64), (5, 7, 3), (6, 10, 9), (1, 1, 2), (2, 3, 4), (15, 14, 5), (768, 6, 3),
                (12, 384, 3), (96, 96, 17), (35, 49, 6), (32, 768, 5), (768, 32, 7), (512, 96, 4),
                (192, 256, 3), (128, 64, 70), (384, 128, 40), (7, 64, 5), (1, 96, 3), (3, 384, 4),
-               (45, 128, 6), (2, 32, 3)]
+               (45, 128, 6), (2, 32, 3), (13, 11, 5), (624, 312, 3), (143, 26, 4), (44, 169, 3)]
 
 
 @pytest.mark.parametrize("nx,nz,ny", STAGE_CASES)
@@ -217,7 +217,7 @@ def test_run_host_matches_device_run(cuda):
 
 def test_unsupported_length_is_an_error(cuda):
     say("unsupported")
-    c = case_grid(11, 8, 4, 0)
+    c = case_grid(17, 8, 4, 0)  # prime factor 17 > 13
     with pytest.raises(D.DnspError) as e:
         make_solver(c)
     assert e.value.code == D.DNSP_ERR_UNSUPPORTED
