@@ -695,7 +695,19 @@ int dnsp_poisson_backward(dnsp_poisson* h, double* S, void* stream) {
     return rc ? rc : dnsp_poisson_stage(h, S, S, DNSP_STAGE_X_BWD, stream);
 }
 
+int dnsp_poisson_set_form(dnsp_poisson* h, int refinements) {
+    clear_error();
+    DNSP_ARG_CHECK(h, "NULL handle");
+    DNSP_ARG_CHECK(refinements >= -1 && refinements <= DNSP_MAX_REFINEMENTS, "refinements=%d not in [-1, %d]",
+                   refinements, DNSP_MAX_REFINEMENTS);
+    DNSP_ARG_CHECK(refinements < 0 || h->precision == DNSP_PRECISION_MIXED,
+                   "the mixed form needs a handle created with DNSP_PRECISION_MIXED");
+    h->form = refinements;
+    return DNSP_OK;
+}
+
 int dnsp_poisson_run(dnsp_poisson* h, double* P, void* stream) {
+    if (h && h->form >= 0) return dnsp_poisson_run_mixed(h, P, h->form, stream);
     int rc = dnsp_poisson_forward(h, P, stream);
     if (rc) return rc;
     rc = dnsp_poisson_tridiag(h, P, stream);

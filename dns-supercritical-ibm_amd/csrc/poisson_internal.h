@@ -34,6 +34,8 @@ struct dnsp_poisson {
     // modes with |lambda| below this solve their reduced system serially
     double pcr_min = 0;
 
+    // form dnsp_poisson_run uses: -1 = FP64, >= 0 = mixed with that many refinements
+    int form = -1;
     // mixed-precision form (precision == DNSP_PRECISION_MIXED)
     int mxvar = 0, mzvar = 0, mtvar = 0;  // its FFT and tridiagonal variants
     int mrows = 0;                         // its partitioned rows per chunk

@@ -73,6 +73,7 @@ _SIGNATURES = [
     ("dnsp_poisson_get_mixed_variants", [_vp, _ip, _ip, _ip], ctypes.c_int),
     ("dnsp_poisson_autotune", [_vp, _vp, ctypes.c_int, _vp, _dp], ctypes.c_int),
     ("dnsp_poisson_solve_mixed", [_vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+    ("dnsp_poisson_set_form", [_vp, ctypes.c_int], ctypes.c_int),
     ("dnsp_poisson_run_mixed", [_vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     ("dnsp_poisson_solve_refine", [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _ip, _dp, _vp], ctypes.c_int),
     ("dnsp_poisson_residual", [_vp, _vp, _vp, _dp, _vp], ctypes.c_int),
@@ -245,6 +246,10 @@ class PoissonSolver:
                                               _vp(_device_ptr(P, self.numel)), int(refinements),
                                               _vp(_stream_ptr(stream))))
         return P
+
+    def set_form(self, refinements=-1):
+        """The form run/run_host use: -1 FP64, >= 0 the mixed form with that many refinements."""
+        _check(lib().dnsp_poisson_set_form(self._h, int(refinements)))
 
     def run_mixed(self, P, refinements=1, stream=None):
         return self.solve_mixed(P, P, refinements, stream)

@@ -287,3 +287,10 @@ def poisson_arrivals(n, rate, seed, n_cases):
     t = np.cumsum(rng.exponential(1.0 / rate, size=n))
     cases = rng.integers(0, n_cases, size=n)
     return t - t[0], cases.astype(np.int32), np.arange(n, dtype=np.uint64)
+
+
+def case_for_shim(nx, nz, ny, seed, gamma=1.6):
+    """A small channel case (tanh faces, U(-1,1) RHS) for driver tests."""
+    yu = tanh_faces(ny, gamma)
+    return dict(nx=nx, nz=nz, ny=ny, inv_dx=nx / LX, inv_dz=nz / LZ, yu=yu,
+                rhs=rhs_uniform((ny, nz, nx), seed, yu))

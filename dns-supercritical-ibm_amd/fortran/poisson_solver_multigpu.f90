@@ -41,6 +41,12 @@ module poisson_solver_multigpu
       type(c_ptr) :: h
       integer(c_int) :: rc
     end function
+    function c_dnsp_poisson_set_form(h, refinements) bind(C, name="dnsp_poisson_set_form") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: refinements
+      integer(c_int) :: rc
+    end function
     function c_dnsp_poisson_run(h, P, stream) bind(C, name="dnsp_poisson_run") result(rc)
       import :: c_int, c_ptr
       type(c_ptr), value :: h, P, stream
@@ -114,18 +120,26 @@ contains
     type(planinfo_hipfft) ::  plan_fwd_x, plan_bwd_x, plan_fwd_z, plan_bwd_z
     real(c_double), allocatable :: ba(:), bb(:), bc(:), ba0(:), bb0(:), bc0(:)
     real(c_double) :: min_abs_div_bacp
-    integer :: i, ii
+    integer :: i, ii, form_len, form_stat, prec
     real(dp) :: const_pi
+    character(len=16) :: form_env
 
     if (mpi_divs_y /= 1 .or. mpi_divs_z /= 1) error stop 'poisson_solver_multigpu (dnsproj): one rank only'
     if (ny_loc < 2) error stop 'ny_loc  <2'
     allocate(ba(ny_loc), bb(ny_loc), bc(ny_loc), ba0(ny_loc), bb0(ny_loc), bc0(ny_loc))
     ba = P_band_a(0:ny_loc-1);    bb = P_band_b(0:ny_loc-1);    bc = P_band_c(0:ny_loc-1)
     ba0 = P_band_a_00(0:ny_loc-1); bb0 = P_band_b_00(0:ny_loc-1); bc0 = P_band_c_00(0:ny_loc-1)
+    ! DNSP_POISSON_FORM=mixed selects the mixed-precision solve (FP32 transforms,
+    ! one FP64 residual refinement); default: FP64 as the reference
+    prec = 0
+    call get_environment_variable('DNSP_POISSON_FORM', form_env, form_len, form_stat)
+    if (form_stat == 0 .and. trim(form_env) == 'mixed') prec = 1
     call dnsp_check(c_dnsp_poisson_init(int(nx_glob, c_int), int(nz_glob, c_int), int(ny_loc, c_int), &
                                         real(inv_dx, c_double), real(inv_dz, c_double), &
-                                        ba, bb, bc, ba0, bb0, bc0, 0_c_int, plan_fwd_x%plan_0), &
+                                        ba, bb, bc, ba0, bb0, bc0, int(prec, c_int), plan_fwd_x%plan_0), &
                     'init_spec_poisson_multigpu')
+    if (prec == 1) call dnsp_check(c_dnsp_poisson_set_form(plan_fwd_x%plan_0, 1_c_int), &
+                                   'init_spec_poisson_multigpu: DNSP_POISSON_FORM')
     plan_bwd_x%plan_0 = plan_fwd_x%plan_0
     plan_fwd_z%plan_0 = plan_fwd_x%plan_0
     plan_bwd_z%plan_0 = plan_fwd_x%plan_0

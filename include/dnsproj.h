@@ -170,6 +170,11 @@ int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream,
  *     residual (each check reads P and rhs once more).  Synchronises.
  */
 #define DNSP_MAX_REFINEMENTS 8
+/* The form dnsp_poisson_run (and dnsp_poisson_run_host) uses: -1 = FP64
+ * (the default), 0..DNSP_MAX_REFINEMENTS = the mixed form with that many
+ * refinements (DNSP_PRECISION_MIXED handles only).  Lets an unmodified
+ * caller of the drop-in select the mixed form. */
+int dnsp_poisson_set_form(dnsp_poisson* h, int refinements);
 int dnsp_poisson_solve_mixed(dnsp_poisson* h, const double* rhs, double* P, int refinements, void* stream);
 int dnsp_poisson_run_mixed(dnsp_poisson* h, double* P, int refinements, void* stream);
 int dnsp_poisson_solve_refine(dnsp_poisson* h, const double* rhs, double* P, double tol, int max_refinements,

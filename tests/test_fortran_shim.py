@@ -94,3 +94,22 @@ def test_shim_driver_matches_oracle(cuda, tmp_path):
         lz, _ = O.eigen(c["nz"], c["inv_dz"])
         assert np.allclose(eig[:c["nx"]], lx, rtol=1e-15, atol=0)
         assert np.allclose(eig[nfx:nfx + c["nz"]], lz, rtol=1e-15, atol=0)
+
+
+@pytest.mark.gpu
+def test_shim_driver_mixed_form_reaches_target(cuda, tmp_path):
+    """DNSP_POISSON_FORM=mixed: the unchanged driver gets the mixed-precision
+    solve; relative residual <= 1e-10 against the oracle operator."""
+    assert os.path.exists(DRIVER), "build the library first (make -C dns-supercritical-ibm_amd)"
+    for n, c in enumerate([W.config1(), W.case_for_shim(96, 64, 40, 3)]):
+        print("[gpu-case] fortran shim mixed", n, c["nx"], c["nz"], c["ny"], flush=True)
+        bands = O.bands_from_yu(c["yu"])
+        fin, fout, feig = tmp_path / f"in{n}.dat", tmp_path / f"out{n}.dat", tmp_path / f"eig{n}.dat"
+        write_input(fin, c, bands)
+        env = dict(os.environ, DNSP_POISSON_FORM="mixed")
+        r = subprocess.run([DRIVER, str(fin), str(fout), str(feig)], capture_output=True, text=True, timeout=120,
+                           env=env)
+        assert r.returncode == 0, r.stdout + r.stderr
+        got = read_array(fout)
+        res = O.apply_operator(got, c["inv_dx"], c["inv_dz"], bands) - c["rhs"]
+        assert np.linalg.norm(res) / np.linalg.norm(c["rhs"]) <= 1e-10
