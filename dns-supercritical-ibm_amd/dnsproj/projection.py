@@ -355,3 +355,61 @@ class ProjectionStep:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# rough-wall IBM coefficient generation (include/dnsproj.h: dnsp_ibm_generate)
+# ---------------------------------------------------------------------------
+class _Surface(ctypes.Structure):
+    _fields_ = [("n_modes", _i), ("mnab", _dp), ("Lx", _d), ("Lz", _d), ("Ly", _d)]
+
+
+_GEN_SIGS = [
+    ("dnsp_ibm_generate", [ctypes.POINTER(_Surface), _i, _i, _i, _dp, _dp, _dp, _i, _i, _i, _d, _d, _i, _ip, _ip,
+                           _dp, _ip, _ip]),
+    ("dnsp_ibm_write_coeffs", [ctypes.c_char_p, _i, _ip, _dp]),
+    ("dnsp_ibm_write_slab", [ctypes.c_char_p, _i, _i, _ip]),
+]
+_SIGS.extend(_GEN_SIGS)
+
+
+def read_mnab(path):
+    """Fourier modes of a rough surface file (count, then rows m n a b), with a
+    and b halved as the examples' init_surf_obj does on reading."""
+    from . import io as IO
+    n = int(open(path).readline().split()[0])
+    cols = [IO.read_column(path, c, 0, n) for c in range(4)]
+    return np.stack([cols[0], cols[1], 0.5 * cols[2], 0.5 * cols[3]], axis=1)
+
+
+def ibm_generate(mnab, Lx, Lz, Ly, x, z, y, cc_x=1, cc_z=1, quady=False, bc_bot=0.0, bc_top=0.0):
+    """Records (ijk (n, 9), c (n, 3): cw, c1, c2) sorted by target (k, j, i),
+    and the column slabs (bottom, top) as (nz*cc_z, nx*cc_x) int arrays."""
+    m = np.ascontiguousarray(mnab, dtype=np.float64).reshape(-1, 4)
+    surf = _Surface(m.shape[0], m.ctypes.data_as(_dp), float(Lx), float(Lz), float(Ly))
+    x, z, y = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, z, y))
+    args = (ctypes.byref(surf), x.size, z.size, y.size, x.ctypes.data_as(_dp), z.ctypes.data_as(_dp),
+            y.ctypes.data_as(_dp), int(cc_x), int(cc_z), int(bool(quady)), float(bc_bot), float(bc_top))
+    n = _i()
+    _check(_lib().dnsp_ibm_generate(*args, 0, ctypes.byref(n), None, None, None, None))
+    ijk = np.zeros((n.value, 9), dtype=np.int32)
+    c = np.zeros((n.value, 3))
+    NX, NZ = x.size * int(cc_x), z.size * int(cc_z)
+    sb = np.zeros((NZ, NX), dtype=np.int32)
+    st = np.zeros((NZ, NX), dtype=np.int32)
+    _check(_lib().dnsp_ibm_generate(*args, n.value, ctypes.byref(n), ijk.ctypes.data_as(_ip), c.ctypes.data_as(_dp),
+                                    sb.ctypes.data_as(_ip), st.ctypes.data_as(_ip)))
+    return ijk, c, sb, st
+
+
+def ibm_write_coeffs(path, ijk, c):
+    ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 9)
+    c = np.ascontiguousarray(c, dtype=np.float64).reshape(-1, 3)
+    _check(_lib().dnsp_ibm_write_coeffs(str(path).encode(), ijk.shape[0], ijk.ctypes.data_as(_ip),
+                                        c.ctypes.data_as(_dp)))
+
+
+def ibm_write_slab(path, slab):
+    s = np.ascontiguousarray(slab, dtype=np.int32)
+    nz, nx = s.shape
+    _check(_lib().dnsp_ibm_write_slab(str(path).encode(), nx, nz, s.ctypes.data_as(_ip)))

@@ -341,6 +341,37 @@ int dnsp_ibm_read_coeffs(const char* path, int nx, int nz, int ny, int* n, int* 
 int dnsp_ibm_div_coeffs(int n, const int* ijk, const double* yp, int nx, int nz, int ny, double inv_dx,
                         double inv_dz, double* c);
 
+/* Rough-wall surface: H(x,z) = sum_i a_i cos(t_i) + b_i sin(t_i),
+ * t_i = 2 pi (m_i x/Lx + n_i z/Lz); the bottom wall is at H, the top wall at
+ * Ly - H (Mod_Height_Function of the examples' rough_surf_define.f90; a_i,
+ * b_i as that module holds them, i.e. after its halving of the file values). */
+typedef struct {
+    int n_modes;
+    const double* mnab; /* 4 per mode: m, n, a, b */
+    double Lx, Lz, Ly;
+} dnsp_surface;
+
+/* IBM coefficients of a rough wall (src/pycode/ibm_coeffs_engine.f90 and its
+ * driver main_runner_ibm_coeffs_engine.f90).  x (nx), z (nz), y (ny): the
+ * field component's coordinates over one surface tile (x and z uniform);
+ * cc_x, cc_z: tile repetitions; quady = 0: ghost points with mirrored image
+ * (make_rough_surface_ghost, apply_Fadlun), 1: quadratic through the wall
+ * (make_rough_surface_quady); bc_bot/bc_top: the component's wall values.
+ * Records are sorted by (k, j, i) of the target: ijk (9 per record: target,
+ * source 1, source 2) and c (3: cw, c1, c2), as dnsp_ibm_create takes them.
+ * Slabs (nx*cc_x by nz*cc_z, i fastest): bottom = highest bottom target per
+ * column (-1 if none), top = lowest top target (ny if none).  Call with
+ * ijk == NULL to get the record count in *n; then cap must be >= *n.
+ * Host only. */
+int dnsp_ibm_generate(const dnsp_surface* surf, int nx, int nz, int ny, const double* x, const double* z,
+                      const double* y, int cc_x, int cc_z, int quady, double bc_bot, double bc_top, int cap, int* n,
+                      int* ijk, double* c, int* slab_bot, int* slab_top);
+/* Writers of the files ibm_read_coeffs and ibm_fill_slabs_2d read: 60-byte
+ * records t(3 int32) c1 p1(3) c2 p2(3) cw after an int32 count; slabs: int32
+ * nx, nz then nx*nz int32, i fastest.  Host only. */
+int dnsp_ibm_write_coeffs(const char* path, int n, const int* ijk, const double* c);
+int dnsp_ibm_write_slab(const char* path, int nx, int nz, const int* slab);
+
 /* Rough-wall column slab (ibm_fill_slabs_2d, src/mod_utils.f90:215-239) minus
  * k_first, clipped as set_ibm_bands.f90:196-218 (bottom min(v, ny-1), top
  * max(v, 0)).  out: nx*nz ints, i fastest.  Host only. */

@@ -1,5 +1,6 @@
 """Copies the wall-normal face files (yu.txt) of the reference's four example
-cases into this folder and records their grid parameters from params.dat.
+cases into this folder, records their grid parameters from params.dat, and
+copies the grit-blasted rough-surface modes (fortran_surf_files/mnab_coeffs.dat).
 
 Run in a container that has the reference checkout; the fixtures it writes
 (reference_grids.json, yu_*.txt) are data and are committed."""
@@ -38,6 +39,10 @@ def main():
         manifest[name] = dict(params(os.path.join(src, "params.dat")), source=f"examples/{folder}/yu.txt")
     with open(os.path.join(HERE, "reference_grids.json"), "w") as fw:
         json.dump(manifest, fw, indent=1, sort_keys=True)
+    # the grit-blasted rough surface (Fourier modes m, n, a, b) of two examples
+    src = os.path.join(REF, "examples", CASES["peeters_sandham_ret180_k30"], "fortran_surf_files", "mnab_coeffs.dat")
+    with open(os.path.join(HERE, "mnab_gritblasted.dat"), "w") as fw:
+        fw.write(open(src).read())
 
 
 if __name__ == "__main__":
