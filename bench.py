@@ -39,7 +39,8 @@ def parse():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_fp64_768x384x384.json"),
                     help="rocprofv3 FETCH/WRITE summary to report as roofline.traffic")
     ap.add_argument("--mode", default="bench", choices=["bench", "counters"],
-                    help="counters: one 906 MB device copy (calibration) then 3 solves, for rocprofv3 --pmc")
+                    help="counters: one 906 MB device copy (calibration) then 3 FP64 and 3 mixed solves, "
+                         "for rocprofv3 --pmc")
     ap.add_argument("--extras", default="batch,step,serve",
                     help="comma list of extra workloads to measure: batch (48 uneven instances), step "
                          "(384x256x192 projection step with IBM), serve (15-case request trace); '' = none")
@@ -182,8 +183,11 @@ def main():
         for _ in range(3):
             solver.solve(f, P)
         torch.cuda.synchronize()
-        print(json.dumps({"mode": "counters", "copy_bytes_each_way": 8 * N, "variants": tune["picked"]}),
-              flush=True)
+        for _ in range(3):
+            solver.solve_mixed(f, P, 1)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "counters", "copy_bytes_each_way": 8 * N, "variants": tune["picked"],
+                          "mixed_variants": tune.get("mixed_picked")}), flush=True)
         return
     for _ in range(args.warmup):
         solver.solve(f, P)
