@@ -140,9 +140,8 @@ __global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri(const BInst
     constexpr int M = kBatchM;
     constexpr int P = kPartChunks;
     constexpr int NT = M * P;
-    constexpr int NR = 2 * P;
     using T = typename std::conditional<KIND == 0, double, float>::type;
-    __shared__ double ra[NR][M], rb[NR][M], rc[NR][M], rd[NR][M];
+    __shared__ TriRed<P, M> red;
     extern __shared__ double smb[];
     const long task = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int u = find_inst(start, cnt, task);
@@ -159,7 +158,7 @@ __global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri(const BInst
     tri_load_rows<MR, P, M, T>(d, S, grp * M + mi, b.plane, b.ny, s, msum);
     __syncthreads();
     tri_group<MR, P, M, T>(d, S, b.ax, b.az, b.i0x, b.i0z, smb, b.nx, b.ny, b.plane, b.pcr_min, grp * M + mi, msum,
-                           pbar, b.inv_nxnz, ra, rb, rc, rd);
+                           pbar, b.inv_nxnz, red);
 }
 
 // q(k) of the first refinement for every plane of every instance (one

@@ -303,12 +303,17 @@ __global__ __launch_bounds__(NT) void kt_zbwd(T* S, int nx, int ncol, int ntc,
 // by B_k).  An upward pass folds them into x_{s+1} = P - Q x_s - R x_e, which
 // turns row s into  a_s x_{s-1} + (beta_s - c_s Q) x_s - c_s R x_e = d_s - c_s P.
 // The 2P tip rows (s and e of every chunk) form a tridiagonal system per mode.
-// A workgroup whose modes all have |lambda| >= pcr_min solves it by parallel
-// cyclic reduction in LDS.  Cyclic reduction loses digits by cancellation on
-// the nearly singular low-wavenumber modes, so a workgroup holding any mode
-// with |lambda| < pcr_min (the (0,0) mode among them) solves it by a serial
-// Thomas sweep per mode instead.  The interior rows follow by the backward
-// recurrence above.  Reciprocals in the chunk sweeps are v_rcp_f64 refined by
+// Row s couples x_{s-1}, x_s, x_e and row e couples x_s, x_e, x_{e+1}; every
+// chunk eliminates its own x_s and the next chunk's x_{e+1} (= that chunk's
+// x_s, whose row arrives through LDS) from its row e, which leaves P rows in
+// the unknowns x_e, tridiagonal again (the stride-1 step of cyclic reduction
+// applied to the e rows only).  A workgroup whose modes all have
+// |lambda| >= pcr_min solves them by parallel cyclic reduction in LDS, one
+// row per thread, log2(P) steps.  Cyclic reduction loses digits by
+// cancellation on the nearly singular low-wavenumber modes, so a workgroup
+// holding any mode with |lambda| < pcr_min (the (0,0) mode among them) solves
+// them by a serial Thomas sweep per mode instead.  x_s then follows from row
+// s, the interior rows by the backward recurrence above.  Reciprocals in the chunk sweeps are v_rcp_f64 refined by
 // two Newton steps (full double precision for the nonzero pivots met here).
 // ---------------------------------------------------------------------------
 
@@ -344,15 +349,20 @@ __device__ __forceinline__ void tri_load_rows(double (&d)[MR], const T* S, long 
 // and writes its solution rows.  smb: the staged bands (plain set, then the
 // (0,0)-mode set where needed); ra..rd: the LDS reduced system.  Contains
 // workgroup barriers; all threads call it with the same group.
+// LDS of the reduced system: two buffers of P rows per mode, mode fastest.
+// b holds the diagonal for the serial sweep and its reciprocal for cyclic
+// reduction.  Buffer 1 first carries the s rows between neighbouring chunks.
+template <int P, int M>
+struct TriRed {
+    double a[2][P][M], b[2][P][M], c[2][P][M], d[2][P][M];
+};
+
 template <int MR, int P, int M, typename T>
 __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* __restrict__ ax,
                                           const double* __restrict__ az, const int* __restrict__ is0x,
                                           const int* __restrict__ is0z, const double* smb, int nx, int ny,
                                           long plane, double pcr_min, long mode, const double* __restrict__ msum,
-                                          double* __restrict__ pbar, double inv_nxnz, double (*ra)[M],
-                                          double (*rb)[M], double (*rc)[M], double (*rd)[M]) {
-    constexpr int NT = M * P;
-    constexpr int NR = 2 * P;  // reduced rows per mode
+                                          double* __restrict__ pbar, double inv_nxnz, TriRed<P, M>& R) {
     const int mi = (int)threadIdx.x % M;
     const int p = (int)threadIdx.x / M;
     const int s = p * MR;
@@ -399,73 +409,90 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
         Qq = Ah[l] - Ch[l] * Qq;
         Rr = -Ch[l] * Rr;
     }
-    {
-        const double cs = band_c(s);
-        ra[2 * p][mi] = band_a(s);
-        rb[2 * p][mi] = beta(s) - cs * Qq;
-        rc[2 * p][mi] = -cs * Rr;
-        rd[2 * p][mi] = d[0] - cs * Pp;
-        ra[2 * p + 1][mi] = Ae;
-        rb[2 * p + 1][mi] = Be;
-        rc[2 * p + 1][mi] = band_c(s + MR - 1);
-        rd[2 * p + 1][mi] = De;
-    }
+    // row s: sa x_{s-1} + sb x_s + sc x_e = sd (sib = 1/sb)
+    const double cs = band_c(s);
+    const double sa = band_a(s), sb = beta(s) - cs * Qq, sc = -cs * Rr, sd = d[0] - cs * Pp;
+    const double sib = rcp64(sb);
+    R.a[1][p][mi] = sa;
+    R.b[1][p][mi] = sib;
+    R.c[1][p][mi] = sc;
+    R.d[1][p][mi] = sd;
     const bool serial = __syncthreads_or(valid && fabs(lam) < pcr_min) != 0;
+    // row e with x_s and x_{e+1} eliminated: ya x_{e-} + yb x_e + yc x_{e+} = yd
+    double ya, yb, yc, yd;
+    {
+        const double al = -Ae * sib;
+        ya = p > 0 ? al * sa : 0.0;
+        yb = Be + al * sc;
+        yd = De + al * sd;
+        yc = 0.0;
+        if (p + 1 < P) {
+            const double g = -band_c(s + MR - 1) * R.b[1][p + 1][mi];
+            yc = g * R.c[1][p + 1][mi];
+            yb += g * R.a[1][p + 1][mi];
+            yd += g * R.d[1][p + 1][mi];
+        }
+    }
+    double yib = serial ? yb : rcp64(yb);
+    R.a[0][p][mi] = ya;
+    R.b[0][p][mi] = yib;
+    R.c[0][p][mi] = yc;
+    R.d[0][p][mi] = yd;
+    __syncthreads();
+    double xe, xem;  // x_e of this chunk and of the one below
     if (serial) {
-        // one thread per mode: Thomas on the NR tip rows; x overwrites rd, rb = 1
+        // one thread per mode: Thomas on the P rows; x overwrites d
         if (p == 0) {
-            for (int q = 1; q < NR; ++q) {
-                const double m = ra[q][mi] / rb[q - 1][mi];
-                rb[q][mi] = rb[q][mi] - m * rc[q - 1][mi];
-                rd[q][mi] = rd[q][mi] - m * rd[q - 1][mi];
+#pragma unroll 1
+            for (int q = 1; q < P; ++q) {
+                const double m = R.a[0][q][mi] / R.b[0][q - 1][mi];
+                R.b[0][q][mi] = R.b[0][q][mi] - m * R.c[0][q - 1][mi];
+                R.d[0][q][mi] = R.d[0][q][mi] - m * R.d[0][q - 1][mi];
             }
-            double xq = rd[NR - 1][mi] / rb[NR - 1][mi];
-            rd[NR - 1][mi] = xq;
-            rb[NR - 1][mi] = 1.0;
-            for (int q = NR - 2; q >= 0; --q) {
-                xq = (rd[q][mi] - rc[q][mi] * xq) / rb[q][mi];
-                rd[q][mi] = xq;
-                rb[q][mi] = 1.0;
+            double xq = R.d[0][P - 1][mi] / R.b[0][P - 1][mi];
+            R.d[0][P - 1][mi] = xq;
+#pragma unroll 1
+            for (int q = P - 2; q >= 0; --q) {
+                xq = (R.d[0][q][mi] - R.c[0][q][mi] * xq) / R.b[0][q][mi];
+                R.d[0][q][mi] = xq;
             }
         }
         __syncthreads();
+        xe = R.d[0][p][mi];
+        xem = p > 0 ? R.d[0][p - 1][mi] : 0.0;
+    } else {
+        // parallel cyclic reduction; step st reads buffer cur, writes cur ^ 1
+        int cur = 0;
+        for (int st = 1; st < P; st *= 2) {
+            double a2 = 0.0, b2 = yb, c2 = 0.0, d2 = yd;
+            if (p - st >= 0) {
+                const double f = -ya * R.b[cur][p - st][mi];
+                a2 = f * R.a[cur][p - st][mi];
+                b2 += f * R.c[cur][p - st][mi];
+                d2 += f * R.d[cur][p - st][mi];
+            }
+            if (p + st < P) {
+                const double g = -yc * R.b[cur][p + st][mi];
+                c2 = g * R.c[cur][p + st][mi];
+                b2 += g * R.a[cur][p + st][mi];
+                d2 += g * R.d[cur][p + st][mi];
+            }
+            ya = a2;
+            yb = b2;
+            yc = c2;
+            yd = d2;
+            yib = rcp64(yb);
+            cur ^= 1;
+            R.a[cur][p][mi] = ya;
+            R.b[cur][p][mi] = yib;
+            R.c[cur][p][mi] = yc;
+            R.d[cur][p][mi] = yd;
+            __syncthreads();
+        }
+        xe = yd * yib;
+        xem = p > 0 ? R.d[cur][p - 1][mi] * R.b[cur][p - 1][mi] : 0.0;
     }
-    // parallel cyclic reduction on the NR tip rows of each mode
-    for (int st = 1; !serial && st < NR; st *= 2) {
-        double na[2], nb[2], nc[2], nd[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int e = (int)threadIdx.x + h * NT;
-            const int mm = e % M;
-            const int q = e / M;
-            const double aq = ra[q][mm], bq = rb[q][mm], cq = rc[q][mm], dq = rd[q][mm];
-            double a2 = 0.0, b2 = bq, c2 = 0.0, d2 = dq;
-            if (q - st >= 0) {
-                const double al = -aq * rcp64(rb[q - st][mm]);
-                a2 = al * ra[q - st][mm];
-                b2 += al * rc[q - st][mm];
-                d2 += al * rd[q - st][mm];
-            }
-            if (q + st < NR) {
-                const double ga = -cq * rcp64(rb[q + st][mm]);
-                c2 = ga * rc[q + st][mm];
-                b2 += ga * ra[q + st][mm];
-                d2 += ga * rd[q + st][mm];
-            }
-            na[h] = a2; nb[h] = b2; nc[h] = c2; nd[h] = d2;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int e = (int)threadIdx.x + h * NT;
-            const int mm = e % M;
-            const int q = e / M;
-            ra[q][mm] = na[h]; rb[q][mm] = nb[h]; rc[q][mm] = nc[h]; rd[q][mm] = nd[h];
-        }
-        __syncthreads();
-    }
-    const double xs = serial ? rd[2 * p][mi] : rd[2 * p][mi] * rcp64(rb[2 * p][mi]);
-    const double xe = serial ? rd[2 * p + 1][mi] : rd[2 * p + 1][mi] * rcp64(rb[2 * p + 1][mi]);
+    const double xs = (sd - sa * xem - sc * xe) * sib;
     d[0] = xs;
     double xn = xe;
 #pragma unroll
@@ -498,9 +525,8 @@ __global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __
                                                           double* __restrict__ pbar = nullptr,
                                                           double inv_nxnz = 0.0) {
     constexpr int NT = M * P;
-    constexpr int NR = 2 * P;
-    __shared__ double ra[NR][M], rb[NR][M], rc[NR][M], rd[NR][M];  // mode fastest
-    extern __shared__ double smb[];                                  // 6*ny band values
+    __shared__ TriRed<P, M> red;
+    extern __shared__ double smb[];  // 6*ny band values
     const int mi = (int)threadIdx.x % M;
     const int s = ((int)threadIdx.x / M) * MR;
 
@@ -525,15 +551,16 @@ __global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __
     for (int e = (int)threadIdx.x; e < nband; e += NT) smb[e] = bands[e];
     double d[MR];
     tri_load_rows<MR, P, M, T>(d, S, (long)grp * M + mi, plane, ny, s, msum);
-    __syncthreads();  // bands staged.  Later groups need no barrier before
-                      // their rows: each thread's last reads of the reduced
-                      // rows are its own slots, which only it rewrites.
+    __syncthreads();  // bands staged
     for (; grp < g_end; grp += g_step) {
         tri_group<MR, P, M, T>(d, S, ax, az, is0x, is0z, smb, nx, ny, plane, pcr_min, (long)grp * M + mi, msum,
-                               pbar, inv_nxnz, ra, rb, rc, rd);
+                               pbar, inv_nxnz, red);
         if constexpr (!LOOP) break;
         const int nxt = grp + g_step;
         if (nxt < g_end) tri_load_rows<MR, P, M, T>(d, S, (long)nxt * M + mi, plane, ny, s, msum);
+        // the next group's s rows reuse the buffer whose neighbour values
+        // this group's threads read last
+        __syncthreads();
     }
 }
 
