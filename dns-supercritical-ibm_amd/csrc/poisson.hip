@@ -363,7 +363,6 @@ static void poisson_release(dnsp_poisson* h) {
 
 static int poisson_setup(dnsp_poisson* h, const double* const bands[6]) {
     DNSP_HIP_TRY(hipGetDevice(&h->device));
-    DNSP_HIP_TRY(hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device));
     const int nx = h->nx, nz = h->nz, ny = h->ny;
     if (!fft_factorize(nx, h->px) || !fft_factorize(nz, h->pz)) {
         set_error("nx=%d / nz=%d: FFT lengths must factor into 2, 3, 5, 7, 11 and 13", nx, nz);
@@ -655,8 +654,8 @@ int dnsp_poisson_stage(dnsp_poisson* h, const double* src, double* P, int stage,
             break;
         case DNSP_STAGE_TRIDIAG: {
             if (h->part_rows > 0) {
-                part_launch<double>(h->tvar, h->part_rows, P, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands,
-                                    h->nx, h->nz, h->ny, h->pcr_min, h->ncu, nullptr, nullptr, 0.0, st);
+                part_launch<double>(h->tvar, P, h->d_ax, h->d_az, h->d_bands,
+                                    h->nx, h->nz, h->ny, h->pcr_min, nullptr, nullptr, 0.0, st);
             } else {
                 const long plane = (long)h->nx * h->nz;
                 const int nb = (int)((plane + 255) / 256);

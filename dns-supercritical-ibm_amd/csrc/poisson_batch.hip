@@ -33,8 +33,6 @@ struct BInst {
     double ix2, iz2, pcr_min, inv_nxnz;
     const double* ax;
     const double* az;
-    const int* i0x;
-    const int* i0z;
     const double* bands;
     const double2* twx;
     const double2* twz;
@@ -149,16 +147,16 @@ __global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri(const BInst
     const long grp = task - start[u];
     const int mi = (int)threadIdx.x % M;
     const int s = ((int)threadIdx.x / M) * MR;
-    const int nband = (grp == 0 ? 6 : 3) * b.ny;
-    for (int e = (int)threadIdx.x; e < nband; e += NT) smb[e] = b.bands[e];
+    const long mode = tri_group_mode(grp, (b.plane + M - 1) / M, M, mi);
+    const double* set = b.bands + (grp == 0 ? 3 * b.ny : 0);
+    for (int e = (int)threadIdx.x; e < 3 * b.ny; e += NT) smb[e] = set[e];
     T* S = KIND == 0 ? reinterpret_cast<T*>(b.P) : reinterpret_cast<T*>(KIND == 1 ? b.A : b.B);
     const double* msum = KIND == 0 ? nullptr : b.small;
     double* pbar = KIND == 0 ? nullptr : b.small + (KIND == 1 ? 1 : 2) * b.ny;
     double d[MR];
-    tri_load_rows<MR, P, M, T>(d, S, grp * M + mi, b.plane, b.ny, s, msum);
+    tri_load_rows<MR, P, M, T>(d, S, mode, b.plane, b.ny, s, msum);
     __syncthreads();
-    tri_group<MR, P, M, T>(d, S, b.ax, b.az, b.i0x, b.i0z, smb, b.nx, b.ny, b.plane, b.pcr_min, grp * M + mi, msum,
-                           pbar, b.inv_nxnz, red);
+    tri_group<MR, P, M, T>(d, S, b.ax, b.az, smb, b.nx, b.ny, b.plane, b.pcr_min, mode, msum, pbar, b.inv_nxnz, red);
 }
 
 // q(k) of the first refinement for every plane of every instance (one
@@ -350,15 +348,13 @@ int hc_wave(int i, int n) {
 }
 
 // poisson_solver_multigpu.f90:139-152
-void eig(int n, double inv_d, std::vector<double>& lam, std::vector<int>& is00) {
+void eig(int n, double inv_d, std::vector<double>& lam) {
     const double pi = 4.0 * std::atan(1.0);
     lam.resize((size_t)n);
-    is00.resize((size_t)n);
     for (int i = 0; i < n; ++i) {
         const int ii = hc_wave(i, n);
         const double sv = std::sin(ii * pi / n);
         lam[(size_t)i] = -4 * (inv_d * inv_d) * (sv * sv);
-        is00[(size_t)i] = ii == 0 ? 1 : 0;
     }
 }
 
@@ -475,15 +471,12 @@ int setup_eig(dnsp_batch* b, const double* inv_dx, const double* inv_dz) {
     for (int i = 0; i < b->n; ++i) {
         BInst& I = b->h_inst[(size_t)i];
         std::vector<double> ax, az;
-        std::vector<int> ix, iz;
-        eig(I.nx, inv_dx[i], ax, ix);
-        eig(I.nz, inv_dz[i], az, iz);
+        eig(I.nx, inv_dx[i], ax);
+        eig(I.nz, inv_dz[i], az);
         I.ix2 = inv_dx[i] * inv_dx[i];
         I.iz2 = inv_dz[i] * inv_dz[i];
         int rc = dev_copy(b, const_cast<double**>(&I.ax), ax.data(), ax.size());
         if (!rc) rc = dev_copy(b, const_cast<double**>(&I.az), az.data(), az.size());
-        if (!rc) rc = dev_copy(b, const_cast<int**>(&I.i0x), ix.data(), ix.size());
-        if (!rc) rc = dev_copy(b, const_cast<int**>(&I.i0z), iz.data(), iz.size());
         if (rc) return rc;
     }
     return DNSP_OK;
@@ -509,7 +502,7 @@ int setup_groups(dnsp_batch* b) {
     std::vector<int> zero((size_t)n, 0);
     for (int i = 0; i < n; ++i) {
         const BInst& I = b->h_inst[(size_t)i];
-        tt[(size_t)i] = (I.plane + kBatchM - 1) / kBatchM;
+        tt[(size_t)i] = (I.plane + kBatchM - 1) / kBatchM + 1;  // plus the group of modes 1..M-1
         tq[(size_t)i] = I.ny;
     }
     int rc = make_groups(b, b->mr, tt, b->gt);
@@ -530,7 +523,7 @@ int upload_inst(dnsp_batch* b) {
     return DNSP_OK;
 }
 
-size_t tri_lds(const dnsp_batch* b) { return 6 * sizeof(double) * (size_t)b->max_ny; }
+size_t tri_lds(const dnsp_batch* b) { return 3 * sizeof(double) * (size_t)b->max_ny; }
 
 void run_fp64(dnsp_batch* b, int xv, int zv, int tv, hipStream_t st) {
     for (const BGroup& g : b->gx[xv]) bx_launch(BX_F64_FWD, xv, b->d_inst, g, 0, st);

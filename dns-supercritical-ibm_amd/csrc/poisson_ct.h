@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "fft_ct.h"
 
 namespace dnsp {
@@ -324,14 +326,12 @@ __device__ __forceinline__ double rcp64(double b) {
     return r;
 }
 
-// Without LOOP a workgroup solves one mode group of M modes.  With LOOP a
-// workgroup loops over mode groups and stages the bands in LDS once; the loop
-// costs registers (fewer resident waves).  OCC is the minimum number of
-// resident waves per SIMD the register allocation must allow.  T is the storage type of S
-// (double, or float for the mixed-precision form); the arithmetic is FP64.
-// When msum is not NULL the (0,0) mode is solved from the FP64 plane sums
-// msum[k] instead of S, its solution times inv_nxnz goes to pbar[k] and its
-// S entries are set to zero.
+// A workgroup solves one mode group of M modes.  OCC is the minimum number
+// of resident waves per SIMD the register allocation must allow.  T is the
+// storage type of S (double, or float for the mixed-precision form); the
+// arithmetic is FP64.  When msum is not NULL the (0,0) mode is solved from the
+// FP64 plane sums msum[k] instead of S, its solution times inv_nxnz goes to
+// pbar[k] and its S entries are set to zero.
 template <int MR, int P, int M, typename T>
 __device__ __forceinline__ void tri_load_rows(double (&d)[MR], const T* S, long mode, long plane, int ny, int s,
                                               const double* __restrict__ msum) {
@@ -345,10 +345,6 @@ __device__ __forceinline__ void tri_load_rows(double (&d)[MR], const T* S, long 
     }
 }
 
-// One mode group: thread (mi, p) holds rows s..s+MR-1 of mode `mode` in d[]
-// and writes its solution rows.  smb: the staged bands (plain set, then the
-// (0,0)-mode set where needed); ra..rd: the LDS reduced system.  Contains
-// workgroup barriers; all threads call it with the same group.
 // LDS of the reduced system: two buffers of P rows per mode, mode fastest.
 // b holds the diagonal for the serial sweep and its reciprocal for cyclic
 // reduction.  Buffer 1 first carries the s rows between neighbouring chunks.
@@ -357,10 +353,14 @@ struct TriRed {
     double a[2][P][M], b[2][P][M], c[2][P][M], d[2][P][M];
 };
 
+// One mode group: thread (mi, p) holds rows s..s+MR-1 of mode `mode` in d[]
+// and writes its solution rows (mode >= plane: an idle lane).  smb: the band
+// set of every mode of the group, staged in LDS; R: the LDS reduced system.
+// Contains workgroup barriers; all threads call it with the same group.
+
 template <int MR, int P, int M, typename T>
 __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* __restrict__ ax,
-                                          const double* __restrict__ az, const int* __restrict__ is0x,
-                                          const int* __restrict__ is0z, const double* smb, int nx, int ny,
+                                          const double* __restrict__ az, const double* smb, int nx, int ny,
                                           long plane, double pcr_min, long mode, const double* __restrict__ msum,
                                           double* __restrict__ pbar, double inv_nxnz, TriRed<P, M>& R) {
     const int mi = (int)threadIdx.x % M;
@@ -370,8 +370,9 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
     const long md = valid ? mode : 0;
     const int i = (int)(md % nx);
     const int j = (int)(md / nx);
-    const double lam = ax[i] + az[j];
-    const double* A = smb + ((is0x[i] && is0z[j]) ? 3 * ny : 0);
+    // idle lanes solve a regular system (the plain bands are singular at 0)
+    const double lam = valid ? ax[i] + az[j] : -1.0;
+    const double* A = smb;
     const double* B = A + ny;
     const double* C = B + ny;
     auto band_a = [&](int row) { return row < ny ? A[row] : 0.0; };
@@ -514,54 +515,41 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
     }
 }
 
-template <int MR, int P, int M, typename T = double, bool LOOP = false, int OCC = 1>
+// Mode groups: only mode 0 (kx = kz = 0) uses the (0,0) band set, so group 0
+// holds mode 0 alone and stages that set, group `ngroups` (one past the
+// plain ones) holds modes 1..M-1, and group g in between modes g*M..g*M+M-1;
+// every group stages one band set.
+__device__ __forceinline__ long tri_group_mode(long grp, long ngroups, int M, int mi) {
+    if (grp == 0) return mi == 0 ? 0 : LONG_MAX;
+    if (grp == ngroups) return mi == 0 ? LONG_MAX : mi;
+    return grp * M + mi;
+}
+
+// Grid: tri_groups(plane, M) workgroups of M * P threads, 3 * ny doubles of
+// dynamic LDS.
+template <int MR, int P, int M, typename T = double, int OCC = 1>
 __global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __restrict__ ax,
                                                           const double* __restrict__ az,
-                                                          const int* __restrict__ is0x,
-                                                          const int* __restrict__ is0z,
                                                           const double* __restrict__ bands, int nx, int ny,
-                                                          long plane, double pcr_min, int ngroups,
+                                                          long plane, double pcr_min,
                                                           const double* __restrict__ msum = nullptr,
                                                           double* __restrict__ pbar = nullptr,
                                                           double inv_nxnz = 0.0) {
     constexpr int NT = M * P;
     __shared__ TriRed<P, M> red;
-    extern __shared__ double smb[];  // 6*ny band values
+    extern __shared__ double smb[];  // the group's band set, 3*ny values
     const int mi = (int)threadIdx.x % M;
     const int s = ((int)threadIdx.x / M) * MR;
-
-    // Adjacent mode groups share cache lines, so they go to one XCD (each has
-    // its own L2; workgroups reach XCDs round-robin): without LOOP through
-    // xcd_tile, with LOOP as a contiguous range of groups per XCD, strided
-    // over that XCD's workgroups (gridDim.x >= 8).
-    int grp, g_end, g_step;
-    if constexpr (LOOP) {
-        const int xcd = (int)blockIdx.x % 8;
-        g_step = (int)gridDim.x / 8 + (xcd < (int)gridDim.x % 8 ? 1 : 0);
-        grp = (int)((long)ngroups * xcd / 8) + (int)blockIdx.x / 8;
-        g_end = (int)((long)ngroups * (xcd + 1) / 8);
-    } else {
-        grp = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-        g_end = grp + 1;
-        g_step = 1;
-    }
-    if (grp >= g_end) return;  // workgroup-uniform
-    // the (0,0)-mode band set is only needed by the workgroup holding mode 0
-    const int nband = (grp == 0 ? 6 : 3) * ny;
-    for (int e = (int)threadIdx.x; e < nband; e += NT) smb[e] = bands[e];
+    // adjacent mode groups share cache lines: keep them on one XCD (each has
+    // its own L2; workgroups reach XCDs round-robin)
+    const long grp = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const long mode = tri_group_mode(grp, (plane + M - 1) / M, M, mi);
+    const double* set = bands + (grp == 0 ? 3 * ny : 0);
+    for (int e = (int)threadIdx.x; e < 3 * ny; e += NT) smb[e] = set[e];
     double d[MR];
-    tri_load_rows<MR, P, M, T>(d, S, (long)grp * M + mi, plane, ny, s, msum);
+    tri_load_rows<MR, P, M, T>(d, S, mode, plane, ny, s, msum);
     __syncthreads();  // bands staged
-    for (; grp < g_end; grp += g_step) {
-        tri_group<MR, P, M, T>(d, S, ax, az, is0x, is0z, smb, nx, ny, plane, pcr_min, (long)grp * M + mi, msum,
-                               pbar, inv_nxnz, red);
-        if constexpr (!LOOP) break;
-        const int nxt = grp + g_step;
-        if (nxt < g_end) tri_load_rows<MR, P, M, T>(d, S, (long)nxt * M + mi, plane, ny, s, msum);
-        // the next group's s rows reuse the buffer whose neighbour values
-        // this group's threads read last
-        __syncthreads();
-    }
+    tri_group<MR, P, M, T>(d, S, ax, az, smb, nx, ny, plane, pcr_min, mode, msum, pbar, inv_nxnz, red);
 }
 
 }  // namespace dnsp

@@ -17,18 +17,16 @@ void ct_z_launch(int n, int variant, bool inv, double* S, int nx, int ny, const 
 void ct_z_launch(int n, int variant, bool inv, float* S, int nx, int ny, const float2* tw, const double* lsum,
                  double* msum, hipStream_t st);
 
-// Partitioned tridiagonal solve: kPartChunks chunks of part_rows_for(ny) rows.
-// Variants: 0 = 16 modes per workgroup with registers capped for 4 resident
-// waves per SIMD, 1 = 8 modes, one workgroup per mode group; 2 = as 1 with
-// registers capped for 4 resident waves per SIMD; 3 = 8 modes, persistent (as
-// many workgroups as are resident at once, each looping over mode groups).
+// Partitioned tridiagonal solve, ny <= 32 * kPartChunks.  Variants (P chunks
+// per mode of up to 32 rows, M modes per workgroup): 0 = P 32, M 16, registers
+// capped for 4 resident waves per SIMD; 1 = P 32, M 8; 2 = P 64, M 4;
+// 3 = P 64, M 8.
 constexpr int kPartChunks = 32;
 constexpr int kPartVariants = 4;
-int part_rows_for(int ny);
+int part_rows_for(int ny);  // rows per chunk at kPartChunks chunks (0: ny too large)
 template <typename T>
-void part_launch(int variant, int mr, T* S, const double* ax, const double* az, const int* i0x, const int* i0z,
-                 const double* bands, int nx, int nz, int ny, double pcr_min, int ncu, const double* msum,
-                 double* pbar, double inv_nxnz, hipStream_t st);
+void part_launch(int variant, T* S, const double* ax, const double* az, const double* bands, int nx, int nz,
+                 int ny, double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st);
 
 // Mixed-precision x kernels (poisson_mixed.h).
 enum MxKind { MX_FWD = 0, MX_RES_F32 = 1, MX_RES_F64 = 2, MX_BWD = 3, MX_ACC_F32 = 4, MX_ACC_F64 = 5 };

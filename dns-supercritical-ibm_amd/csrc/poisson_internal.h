@@ -12,7 +12,6 @@ struct dnsp_poisson {
     double inv_dx = 0, inv_dz = 0;
     int precision = 0;
     int device = 0;
-    int ncu = 0;  // compute units of the device (persistent launches)
     dnsp::FftPlan px{}, pz{};
     int Gx = 1, TAz = 1, ncol = 0;
     double* d_ax = nullptr;
@@ -27,7 +26,7 @@ struct dnsp_poisson {
     int tri_strategy = DNSP_TRIDIAG_AUTO;
     bool ct_x = false, ct_z = false;  // compile-time FFT kernels in use
     int xvar = 0, zvar = 0;           // their (tile, threads) variants
-    int tvar = 0;                     // partitioned tridiagonal variant
+    int tvar = 1;                     // partitioned tridiagonal variant
     int part_rows = 0;                // partitioned tridiagonal rows per chunk (0 = serial)
     double* d_host = nullptr;  // staging for dnsp_poisson_run_host
     double min_pivot = 0;

@@ -144,6 +144,36 @@ __global__ __launch_bounds__(NT) void kt_xfwd_mf(const double* __restrict__ f, f
 // line = j + nz*k; x neighbours wrap, z neighbours wrap, y neighbours stop at
 // the walls (the operator of the FP64 solve).
 // ---------------------------------------------------------------------------
+// Plane-strip tile order for the residual's wall-normal stencil: JB
+// consecutive tiles of plane k, then the same tiles of plane k+1, and so on
+// through all planes before the next strip, so a plane's lines are read again
+// (as the k+1 and k-1 neighbours) while they are still in L2; in plane order
+// they come back ~2 * nz lines later, after more traffic than L2 holds.  A
+// permutation of [0, ny * nz / (2G)) when 2G divides nz (identity otherwise).
+template <int G>
+__device__ __forceinline__ long strip_tile(long t, int nz, int ny) {
+    constexpr int LG = 2 * G;
+    constexpr int JB = 8;
+    if (nz % LG != 0) return t;
+    const int tpp = nz / LG;
+    if (t >= (long)tpp * ny) return t;
+    const int nfull = tpp / JB;
+    const long full = (long)nfull * JB * ny;
+    long k, jg;
+    if (t < full) {
+        const long sidx = t / ((long)JB * ny);
+        const long r = t - sidx * JB * ny;
+        k = r / JB;
+        jg = sidx * JB + (r - k * JB);
+    } else {
+        const int w = tpp - nfull * JB;
+        const long r = t - full;
+        k = r / w;
+        jg = (long)nfull * JB + (r - k * w);
+    }
+    return k * tpp + jg;
+}
+
 template <int N, int G, int NT, typename TP>
 __device__ __forceinline__ void xfwd_mr_body(const TP* __restrict__ p, const double* __restrict__ f,
                                              const double* __restrict__ qk, const double* __restrict__ bands,
@@ -154,7 +184,7 @@ __device__ __forceinline__ void xfwd_mr_body(const TP* __restrict__ p, const dou
     constexpr int LP = ct_pitch(N);
     constexpr int NL = (G * N + NT - 1) / NT;
     float* smf = reinterpret_cast<float*>(sm);
-    const long l0 = tile * 2 * G;
+    const long l0 = strip_tile<G>(tile, nz, ny) * 2 * G;
     const int lane = (int)threadIdx.x & 63;
     double ps[NL];
 #pragma unroll

@@ -152,7 +152,7 @@ int mixed_setup(dnsp_poisson* h) {
     if (rc) return rc;
     DNSP_HIP_TRY(hipMalloc(&h->d_A, sizeof(float) * N));
     DNSP_HIP_TRY(hipMalloc(&h->d_B, sizeof(float) * N));
-    h->mtvar = 3;
+    h->mtvar = 0;
     return DNSP_OK;
 }
 
@@ -203,8 +203,8 @@ void sweep0(dnsp_poisson* h, const double* f, double* out, hipStream_t st) {
     a.lsum = h->d_lsA;
     mx_x_launch(MX_FWD, h->nx, h->mxvar, a, st);
     ct_z_launch(h->nz, h->mzvar, false, h->d_A, h->nx, h->ny, h->d_twz32, h->d_lsA, sm.msum, st);
-    part_launch<float>(h->mtvar, h->mrows, h->d_A, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands, h->nx,
-                       h->nz, h->ny, h->pcr_min, h->ncu, sm.msum, sm.pbar0, inv_nxnz_of(h), st);
+    part_launch<float>(h->mtvar, h->d_A, h->d_ax, h->d_az, h->d_bands, h->nx,
+                       h->nz, h->ny, h->pcr_min, sm.msum, sm.pbar0, inv_nxnz_of(h), st);
     ct_z_launch(h->nz, h->mzvar, true, h->d_A, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
     MxArgs b = base_args(h);
     b.src = h->d_A;
@@ -236,8 +236,8 @@ void refine(dnsp_poisson* h, const double* f, double* P, bool first, bool sums, 
     a.lsum = h->d_lsB;
     mx_x_launch(first ? MX_RES_F32 : MX_RES_F64, h->nx, h->mxvar, a, st);
     ct_z_launch(h->nz, h->mzvar, false, h->d_B, h->nx, h->ny, h->d_twz32, h->d_lsB, sm.msum, st);
-    part_launch<float>(h->mtvar, h->mrows, h->d_B, h->d_ax, h->d_az, h->d_is0x, h->d_is0z, h->d_bands, h->nx,
-                       h->nz, h->ny, h->pcr_min, h->ncu, sm.msum, sm.pbar1, inv_nxnz_of(h), st);
+    part_launch<float>(h->mtvar, h->d_B, h->d_ax, h->d_az, h->d_bands, h->nx,
+                       h->nz, h->ny, h->pcr_min, sm.msum, sm.pbar1, inv_nxnz_of(h), st);
     ct_z_launch(h->nz, h->mzvar, true, h->d_B, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
     MxArgs b = base_args(h);
     b.src = h->d_B;
@@ -323,9 +323,8 @@ int mixed_autotune(dnsp_poisson* h, const double* f, int reps, hipStream_t st, d
                     ct_z_launch(h->nz, v, true, h->d_A, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
                 } else {
                     for (int r = 0; r < 2; ++r)
-                        part_launch<float>(v, h->mrows, h->d_A, h->d_ax, h->d_az, h->d_is0x, h->d_is0z,
-                                           h->d_bands, h->nx, h->nz, h->ny, h->pcr_min, h->ncu, nullptr, nullptr,
-                                           0.0, st);
+                        part_launch<float>(v, h->d_A, h->d_ax, h->d_az, h->d_bands, h->nx, h->nz, h->ny,
+                                           h->pcr_min, nullptr, nullptr, 0.0, st);
                 }
             };
             pair();  // warm-up

@@ -129,11 +129,9 @@ int dnsp_poisson_get_strategy(const dnsp_poisson* h, int* fft_ct_x, int* fft_ct_
 /* Kernel variants.  xvar, zvar (0..3): the compile-time-sized FFT kernels'
  * (complex elements per workgroup tile, threads) = (3072,256), (3072,512),
  * (1536,256), (6144,512); ignored where a length has no such kernels.
- * tvar (0..3): partitioned tridiagonal workgroups (x 32 chunks) of 16 modes
- * with registers capped for 4 resident waves per SIMD (0), of 8 modes (1),
- * of 8 modes capped the same way (2), one per mode group; 8 modes,
- * persistent (3: as many workgroups as are resident at once, each looping
- * over mode groups).  The
+ * tvar (0..3): partitioned tridiagonal workgroups of (chunks per mode, modes)
+ * = (32, 16) with registers capped for 4 resident waves per SIMD (0),
+ * (32, 8) (1), (64, 4) (2), (64, 8) (3); rows per chunk follow from ny.  The
  * mixed-precision form has its own set.  No counterpart in the reference. */
 int dnsp_poisson_set_variants(dnsp_poisson* h, int xvar, int zvar, int tvar);
 int dnsp_poisson_get_variants(const dnsp_poisson* h, int* xvar, int* zvar, int* tvar);

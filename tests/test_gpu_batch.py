@@ -77,9 +77,10 @@ def test_batch_fp64_equals_single_and_oracle(cuda, tvar):
         b.run()
         for c, fi, Pi in zip(cases, f, P):
             s = D.PoissonSolver(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], c["bands"])
-            # same kernels, tile decomposition and 8-mode groups as the batch
+            # same kernels, tile decomposition and 8-mode groups of 32 chunks as
+            # the batch (its register caps change no arithmetic)
             s.set_strategy(D.FFT_FIXED, D.TRIDIAG_PARTITION)
-            s.set_variants(xv, zv, 1 if tvar == 0 else 2)
+            s.set_variants(xv, zv, 1)
             Q = to_dev(np.zeros_like(c["rhs"]))
             s.solve(fi, Q)
             got = to_host(Pi)
@@ -102,7 +103,7 @@ def test_batch_mixed_equals_single_and_reaches_target(cuda):
         for c, fi, Pi in zip(cases, f, P):
             s = D.PoissonSolver(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], c["bands"],
                                 precision=D.PRECISION_MIXED)
-            s.set_mixed_variants(2, 1, 2)
+            s.set_mixed_variants(2, 1, 1)
             Q = to_dev(np.zeros_like(c["rhs"]))
             s.solve_mixed(fi, Q, refinements)
             got = to_host(Pi)
@@ -183,7 +184,7 @@ def test_config3_batch(cuda):
         s = D.PoissonSolver(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], c["bands"],
                             precision=D.PRECISION_MIXED)
         s.set_strategy(D.FFT_FIXED, D.TRIDIAG_PARTITION)
-        s.set_variants(v[0], v[1], 1 if v[2] == 0 else 2)
+        s.set_variants(v[0], v[1], 1)
         Q = torch.zeros_like(fi)
         s.solve(fi, Q)
         assert torch.equal(Q, Pi)
