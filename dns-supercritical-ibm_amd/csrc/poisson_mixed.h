@@ -174,6 +174,17 @@ __device__ __forceinline__ long strip_tile(long t, int nz, int ny) {
     return k * tpp + jg;
 }
 
+// Plane k and index j of line l0 + r, for a tile whose first line l0 is
+// k0 * nz + j0.  A 32-bit division (a short branch-free sequence): control
+// flow here would split the unrolled element loop into basic blocks, which
+// keeps the scheduler from issuing the next elements' loads early.
+__device__ __forceinline__ void tile_line_kj(int k0, int j0, int r, int nz, int& k, int& j) {
+    const unsigned jr = (unsigned)(j0 + r);
+    const unsigned dk = jr / (unsigned)nz;
+    k = k0 + (int)dk;
+    j = (int)(jr - dk * (unsigned)nz);
+}
+
 template <int N, int G, int NT, typename TP>
 __device__ __forceinline__ void xfwd_mr_body(const TP* __restrict__ p, const double* __restrict__ f,
                                              const double* __restrict__ qk, const double* __restrict__ bands,
@@ -185,55 +196,51 @@ __device__ __forceinline__ void xfwd_mr_body(const TP* __restrict__ p, const dou
     constexpr int NL = (G * N + NT - 1) / NT;
     float* smf = reinterpret_cast<float*>(sm);
     const long l0 = strip_tile<G>(tile, nz, ny) * 2 * G;
-    const int lane = (int)threadIdx.x & 63;
+    const int k0 = (int)(l0 / nz);
+    const int j0 = (int)(l0 - (long)k0 * nz);
+    // Branch-free: every address is clamped to a valid line and the wall
+    // terms carry a zero coefficient, so all loads of an element issue
+    // together (branches around loads serialise their round trips).
     double ps[NL];
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
         const int e = (int)threadIdx.x + q * NT;
         const int r = (2 * e) / N;
-        const long line = l0 + r;
         const int t0 = 2 * e - r * N;
         const bool in_tile = (G * N % NT == 0 || e < G * N);
-        const bool ok = in_tile && line < nlines;
-        P2 c;
-        c.x = TP(0);
-        c.y = TP(0);
-        if (ok) c = *reinterpret_cast<const P2*>(p + line * N + t0);
+        const bool ok = in_tile && l0 + r < nlines;
+        const int rr = ok ? r : 0;
+        const long line = l0 + rr;
+        int k, j;
+        tile_line_kj(k0, j0, rr, nz, k, j);
+        const TP* pl = p + line * N;
+        const P2 c = *reinterpret_cast<const P2*>(pl + t0);
+        const TP xl = pl[t0 == 0 ? N - 1 : t0 - 1];
+        const TP xr = pl[t0 + 2 == N ? 0 : t0 + 2];
+        const long lzm = line + (j == 0 ? nz - 1 : -1);
+        const long lzp = line + (j == nz - 1 ? 1 - nz : 1);
+        const P2 zm = *reinterpret_cast<const P2*>(p + lzm * N + t0);
+        const P2 zp = *reinterpret_cast<const P2*>(p + lzp * N + t0);
+        const P2 yd = *reinterpret_cast<const P2*>(p + (k > 0 ? line - nz : line) * N + t0);
+        const P2 yu = *reinterpret_cast<const P2*>(p + (k < ny - 1 ? line + nz : line) * N + t0);
+        const double2 fv = *reinterpret_cast<const double2*>(f + line * N + t0);
+        // loads unconditional (0 <= k < ny), the wall selection after them
+        const double av = bands[k], bk = bands[ny + k], cv = bands[2 * ny + k], qv = qk[k];
+        const double ak = k > 0 ? av : 0.0;
+        const double ck = k < ny - 1 ? cv : 0.0;
         const double cx = (double)c.x, cy = (double)c.y;
-        // x neighbours from the adjacent lanes (pairs e-1, e+1 of the same line)
-        double lft = __shfl_up(cy, 1, 64);
-        double rgt = __shfl_down(cx, 1, 64);
-        double r0 = 0.0, r1 = 0.0;
-        if (ok) {
-            const TP* pl = p + line * N;
-            if (lane == 0 || t0 == 0) lft = (double)pl[t0 == 0 ? N - 1 : t0 - 1];
-            if (lane == 63 || t0 + 2 == N) rgt = (double)pl[t0 + 2 == N ? 0 : t0 + 2];
-            const int k = (int)(line / nz);
-            const int j = (int)(line - (long)k * nz);
-            const long lzm = line + (j == 0 ? nz - 1 : -1);
-            const long lzp = line + (j == nz - 1 ? 1 - nz : 1);
-            const P2 zm = *reinterpret_cast<const P2*>(p + lzm * N + t0);
-            const P2 zp = *reinterpret_cast<const P2*>(p + lzp * N + t0);
-            const double2 fv = *reinterpret_cast<const double2*>(f + line * N + t0);
-            const double bk = bands[ny + k];
-            double v0 = ix2 * (cy - 2 * cx + lft) + iz2 * ((double)zp.x - 2 * cx + (double)zm.x) + bk * cx;
-            double v1 = ix2 * (rgt - 2 * cy + cx) + iz2 * ((double)zp.y - 2 * cy + (double)zm.y) + bk * cy;
-            if (k > 0) {
-                const P2 yd = *reinterpret_cast<const P2*>(p + (line - nz) * N + t0);
-                const double ak = bands[k];
-                v0 += ak * (double)yd.x;
-                v1 += ak * (double)yd.y;
-            }
-            if (k < ny - 1) {
-                const P2 yu = *reinterpret_cast<const P2*>(p + (line + nz) * N + t0);
-                const double ck = bands[2 * ny + k];
-                v0 += ck * (double)yu.x;
-                v1 += ck * (double)yu.y;
-            }
-            const double qv = qk[k];
-            r0 = fv.x - (v0 + qv);
-            r1 = fv.y - (v1 + qv);
-        }
+        double v0 = ix2 * (cy - 2 * cx + (double)xl) + iz2 * ((double)zp.x - 2 * cx + (double)zm.x) + bk * cx;
+        double v1 = ix2 * ((double)xr - 2 * cy + cx) + iz2 * ((double)zp.y - 2 * cy + (double)zm.y) + bk * cy;
+        v0 += ak * (double)yd.x;
+        v1 += ak * (double)yd.y;
+        v0 += ck * (double)yu.x;
+        v1 += ck * (double)yu.y;
+        // a 0/1 factor rather than a select, which the compiler would turn
+        // back into a branch holding half of the loads (clamped loads are
+        // finite, so the product is exact)
+        const double okf = ok ? 1.0 : 0.0;
+        const double r0 = (fv.x - (v0 + qv)) * okf;
+        const double r1 = (fv.y - (v1 + qv)) * okf;
         ps[q] = r0 + r1;
         if (in_tile) {
             const int g = r >> 1, cc = r & 1;
@@ -355,6 +362,8 @@ __device__ __forceinline__ void xbwd_ma_body(const float* __restrict__ S, const 
     constexpr int LP = ct_pitch(N);
     constexpr int NL = (G * N + NT - 1) / NT;
     const long l0 = tile * 2 * G;
+    const int k0 = (int)(l0 / nz);
+    const int j0 = (int)(l0 - (long)k0 * nz);
     xbwd_load_f32<N, G, NT>(sm, S, l0, nlines);
     fft_ct<N, G, NT, true>(sm, tw);
     const float* smf = reinterpret_cast<const float*>(sm);
@@ -368,7 +377,8 @@ __device__ __forceinline__ void xbwd_ma_body(const float* __restrict__ S, const 
             const int t = 2 * e - r * N;
             const int g = r >> 1, c = r & 1;
             const long line = l0 + r;
-            const int k = (int)(line / nz);
+            int k, j;
+            tile_line_kj(k0, j0, r, nz, k, j);
             const double add = (pbar0 ? pbar0[k] : 0.0) + (pbar1 ? pbar1[k] : 0.0);
             P2 b;
             b.x = TP(0);

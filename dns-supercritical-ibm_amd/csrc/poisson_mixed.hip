@@ -79,25 +79,30 @@ __global__ __launch_bounds__(256) void k_resid_norm(const double* __restrict__ p
                                                     double* __restrict__ part) {
     __shared__ double sr[4], sf[4];
     const long plane = (long)nx * nz;
-    const long n = plane * ny;
     double ar = 0.0, af = 0.0;
-    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (long)gridDim.x * 256) {
-        const int i = (int)(idx % nx);
-        const long line = idx / nx;
-        const int j = (int)(line % nz);
+    // workgroups stride over x lines (plane k, index j: one division per line)
+    for (long line = blockIdx.x; line < (long)nz * ny; line += gridDim.x) {
         const int k = (int)(line / nz);
-        const double c = p[idx];
-        const double xm = p[idx + (i == 0 ? nx - 1 : -1)];
-        const double xp = p[idx + (i == nx - 1 ? 1 - nx : 1)];
-        const double zm = p[idx + (j == 0 ? (long)(nz - 1) * nx : -(long)nx)];
-        const double zp = p[idx + (j == nz - 1 ? -(long)(nz - 1) * nx : (long)nx)];
-        double v = ix2 * (xp - 2 * c + xm) + iz2 * (zp - 2 * c + zm) + bands[ny + k] * c;
-        if (k > 0) v += bands[k] * p[idx - plane];
-        if (k < ny - 1) v += bands[2 * ny + k] * p[idx + plane];
-        const double fv = f[idx];
-        const double r = fv - (v + q[k]);
-        ar += r * r;
-        af += fv * fv;
+        const int j = (int)(line - (long)k * nz);
+        const long base = line * nx;
+        const long zmo = j == 0 ? (long)(nz - 1) * nx : -(long)nx;
+        const long zpo = j == nz - 1 ? -(long)(nz - 1) * nx : (long)nx;
+        const double bk = bands[ny + k], ak = bands[k], ck = bands[2 * ny + k], qk = q[k];
+        for (int i = (int)threadIdx.x; i < nx; i += 256) {
+            const long idx = base + i;
+            const double c = p[idx];
+            const double xm = p[idx + (i == 0 ? nx - 1 : -1)];
+            const double xp = p[idx + (i == nx - 1 ? 1 - nx : 1)];
+            const double zm = p[idx + zmo];
+            const double zp = p[idx + zpo];
+            double v = ix2 * (xp - 2 * c + xm) + iz2 * (zp - 2 * c + zm) + bk * c;
+            if (k > 0) v += ak * p[idx - plane];
+            if (k < ny - 1) v += ck * p[idx + plane];
+            const double fv = f[idx];
+            const double r = fv - (v + qk);
+            ar += r * r;
+            af += fv * fv;
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
