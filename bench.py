@@ -250,11 +250,15 @@ def main():
     dom = max(stage_ms, key=stage_ms.get)
     alg_bytes = 16.0 * N
     achieved = alg_bytes / (stage_ms[dom] * 1e-3) / 1e9
+    # measured HBM bytes of that kernel, per launch, for the variant the tuner picked
     traffic = None
     if args.traffic and os.path.exists(args.traffic):
         try:
             tj = json.load(open(args.traffic))
-            traffic = tj.get("per_launch_bytes", {}).get(dom)
+            picked = {"x_fwd": tune["picked"][0], "x_bwd": tune["picked"][0], "z_fwd": tune["picked"][1],
+                      "z_bwd": tune["picked"][1], "tridiag": tune["picked"][2]}
+            per = tj.get("per_launch_bytes", {}).get(dom)
+            traffic = per.get(str(picked[dom])) if isinstance(per, dict) else per
         except Exception:
             traffic = None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
