@@ -1,13 +1,15 @@
 """Numerical design check of the partitioned tridiagonal kernel
-(kt_tri_part in csrc/poisson_ct.h), emulated in numpy on single columns:
-32 chunks, chunk elimination with fill-in, the 64-row reduced system solved
-by parallel cyclic reduction or by a serial Thomas sweep, back-substitution.
+(tri_group in csrc/poisson_ct.h), emulated in numpy on single columns: P
+chunks of MR rows, chunk elimination with fill-in towards both chunk ends,
+elimination of every chunk's start row (its own and the next chunk's) from
+its end row, the P-row reduced system in the end unknowns solved by parallel
+cyclic reduction or by a serial Thomas sweep, back-substitution.
 
-Cyclic reduction loses digits on nearly singular (low |lambda|) modes, so the
-kernel switches to the serial reduced solve below |lambda| = min_k(a_k + c_k).
-This test shows both facts on the channel grids the benchmarks use: the
-hybrid keeps the residual within a small factor of a serial Thomas solve of
-the whole column, and pure PCR would not at lambda = 0.
+Cyclic reduction can lose digits on nearly singular (low |lambda|) modes, so
+the kernel switches to the serial reduced solve below
+|lambda| = min_k(a_k + c_k).  This test shows, on the channel grids the
+benchmarks use, that the hybrid keeps the residual within a small factor of a
+serial Thomas solve of the whole column, for 32 and 64 chunks.
 """
 import numpy as np
 import pytest
@@ -36,8 +38,7 @@ def partitioned(a, b, c, d, MR, P, reduced):
     n = MR * P
     A, B, C, D = np.zeros(n), np.ones(n), np.zeros(n), np.zeros(n)
     A[:ny], B[:ny], C[:ny], D[:ny] = a, b, c, d
-    ra, rb, rc, rd = (np.zeros(2 * P) for _ in range(4))
-    keep = []
+    srow, erow, keep = [], [], []
     for p in range(P):
         s = p * MR
         dl = D[s:s + MR].copy()
@@ -52,33 +53,52 @@ def partitioned(a, b, c, d, MR, P, reduced):
         for l in range(MR - 2, 0, -1):
             Pp, Qq, Rr = dl[l] - Ch[l] * Pp, Ah[l] - Ch[l] * Qq, -Ch[l] * Rr
         cs = C[s]
-        ra[2 * p], rb[2 * p], rc[2 * p], rd[2 * p] = A[s], B[s] - cs * Qq, -cs * Rr, dl[0] - cs * Pp
-        ra[2 * p + 1], rb[2 * p + 1], rc[2 * p + 1], rd[2 * p + 1] = Ak, Bk, C[s + MR - 1], Dk
+        sb = B[s] - cs * Qq
+        # row s: sa x_{s-1} + sb x_s + sc x_e = sd;  row e: Ae x_s + Be x_e + ce x_{e+1} = De
+        srow.append((A[s], sb, 1.0 / sb, -cs * Rr, dl[0] - cs * Pp))
+        erow.append((Ak, Bk, C[s + MR - 1], Dk))
         keep.append((dl, Ah, Ch))
-    NR = 2 * P
+    ra, rb, rc, rd = (np.zeros(P) for _ in range(4))
+    for p in range(P):
+        sa, sb, sib, sc, sd = srow[p]
+        Ae, Be, ce, De = erow[p]
+        al = -Ae * sib
+        ra[p] = al * sa if p > 0 else 0.0
+        rb[p] = Be + al * sc
+        rd[p] = De + al * sd
+        if p + 1 < P:
+            na, nb, nib, nc, nd = srow[p + 1]
+            g = -ce * nib
+            rc[p] = g * nc
+            rb[p] += g * na
+            rd[p] += g * nd
     if reduced == "serial":
-        xr = thomas(ra, rb, rc, rd)
+        y = thomas(ra, rb, rc, rd)
     else:
+        ib = 1.0 / rb
         st = 1
-        while st < NR:
-            na, nb, nc, nd = (np.zeros(NR) for _ in range(4))
-            for q in range(NR):
+        while st < P:
+            na, nb, nc, nd = (np.zeros(P) for _ in range(4))
+            for q in range(P):
                 a2, b2, c2, d2 = 0.0, rb[q], 0.0, rd[q]
                 if q - st >= 0:
-                    al = -ra[q] / rb[q - st]
-                    a2, b2, d2 = al * ra[q - st], b2 + al * rc[q - st], d2 + al * rd[q - st]
-                if q + st < NR:
-                    ga = -rc[q] / rb[q + st]
-                    c2, b2, d2 = ga * rc[q + st], b2 + ga * ra[q + st], d2 + ga * rd[q + st]
+                    f = -ra[q] * ib[q - st]
+                    a2, b2, d2 = f * ra[q - st], b2 + f * rc[q - st], d2 + f * rd[q - st]
+                if q + st < P:
+                    g = -rc[q] * ib[q + st]
+                    c2, b2, d2 = g * rc[q + st], b2 + g * ra[q + st], d2 + g * rd[q + st]
                 na[q], nb[q], nc[q], nd[q] = a2, b2, c2, d2
             ra, rb, rc, rd = na, nb, nc, nd
+            ib = 1.0 / rb
             st *= 2
-        xr = rd / rb
+        y = rd * ib
     x = np.zeros(n)
     for p in range(P):
         s = p * MR
         dl, Ah, Ch = keep[p]
-        xs, xe = xr[2 * p], xr[2 * p + 1]
+        sa, sb, sib, sc, sd = srow[p]
+        ym = y[p - 1] if p > 0 else 0.0
+        xs, xe = (sd - sa * ym - sc * y[p]) * sib, y[p]
         x[s], x[s + MR - 1] = xs, xe
         xn = xe
         for l in range(MR - 2, 0, -1):
@@ -87,11 +107,12 @@ def partitioned(a, b, c, d, MR, P, reduced):
     return x[:ny]
 
 
+@pytest.mark.parametrize("chunks", [32, 64])
 @pytest.mark.parametrize("grid", ["config1", "config2"])
-def test_hybrid_reduced_solve_matches_serial_thomas(grid):
+def test_hybrid_reduced_solve_matches_serial_thomas(grid, chunks):
     yu = W.config1()["yu"] if grid == "config1" else W.config2_grid()["yu"]
     ny = len(yu) - 1
-    MR = max(2, -(-ny // 32))
+    MR = max(2, -(-ny // chunks))
     bands = O.bands_from_yu(yu)
     a, b, c = bands[0], bands[1], bands[2]
     pcr_min = np.min(np.abs(a[1:-1]) + np.abs(c[1:-1]))
@@ -103,11 +124,24 @@ def test_hybrid_reduced_solve_matches_serial_thomas(grid):
         res = lambda x: np.abs(M @ x - d).max() / np.abs(d).max()
         r_serial = res(thomas(A, B + lam, C, d))
         reduced = "serial" if abs(lam) < pcr_min else "pcr"
-        r_hybrid = res(partitioned(A, B + lam, C, d, MR, 32, reduced))
+        r_hybrid = res(partitioned(A, B + lam, C, d, MR, chunks, reduced))
         assert r_hybrid <= 10 * r_serial + 2e-14, (grid, lam, r_hybrid, r_serial)
-    # pure cyclic reduction at lambda = 0 would not be
-    d = rng.uniform(-1, 1, ny)
-    M = np.diag(bands[4]) + np.diag(bands[3][1:], -1) + np.diag(bands[5][:-1], 1)
-    r_pcr = np.abs(M @ partitioned(bands[3], bands[4], bands[5], d, MR, 32, "pcr") - d).max()
-    r_ser = np.abs(M @ thomas(bands[3], bands[4], bands[5], d) - d).max()
-    assert r_pcr > 20 * r_ser
+
+
+def test_partitioned_solution_matches_dense_solve():
+    # every piece of the elimination (both chunk ends, the exchange of start
+    # rows, the reduced solve) against a dense LU solve on a small system,
+    # including padded chunks (ny not a multiple of the chunk count)
+    rng = np.random.default_rng(3)
+    for ny, MR, P in [(37, 2, 32), (64, 2, 32), (100, 4, 32), (97, 2, 64), (384, 12, 32)]:
+        a = rng.uniform(0.5, 1.5, ny)
+        c = rng.uniform(0.5, 1.5, ny)
+        a[0] = 0.0
+        c[-1] = 0.0
+        b = -(a + c) - rng.uniform(0.1, 1.0, ny)
+        d = rng.uniform(-1, 1, ny)
+        M = np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
+        ref = np.linalg.solve(M, d)
+        for reduced in ("pcr", "serial"):
+            x = partitioned(a, b, c, d, MR, P, reduced)
+            assert np.abs(x - ref).max() <= 1e-13 * np.abs(ref).max(), (ny, MR, P, reduced)
