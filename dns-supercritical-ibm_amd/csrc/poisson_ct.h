@@ -237,7 +237,6 @@ __global__ __launch_bounds__(NT) void kt_zfwd(T* S, int nx, int ncol, int ntc,
 template <int N, int TA, int NT, typename T>
 __device__ __forceinline__ void zbwd_body(T* S, int nx, int ntc, const typename Cx<T>::V* __restrict__ tw,
                                           int tile, typename Cx<T>::V* sm) {
-    using V = typename Cx<T>::V;
     constexpr int LP = ct_pitch(N);
     constexpr int W = 2 * TA;
     constexpr int H = N / 2 + 1;

@@ -175,6 +175,20 @@ int dnsp_step_create(int nx, int nz, int ny, const double* yu, double inv_dx, do
     return DNSP_OK;
 }
 
+int dnsp_step_info(const dnsp_step* s, int* nx, int* nz, int* ny, double* inv_dx, double* inv_dz, double* dtmax,
+                   int* use_incompressible) {
+    clear_error();
+    DNSP_ARG_CHECK(s, "NULL step");
+    if (nx) *nx = s->nx;
+    if (nz) *nz = s->nz;
+    if (ny) *ny = s->ny;
+    if (inv_dx) *inv_dx = s->inv_dx;
+    if (inv_dz) *inv_dz = s->inv_dz;
+    if (dtmax) *dtmax = s->dtmax;
+    if (use_incompressible) *use_incompressible = s->incompressible;
+    return DNSP_OK;
+}
+
 int dnsp_step_field(dnsp_step* s, int which, double** ptr) {
     clear_error();
     DNSP_ARG_CHECK(s && ptr, "NULL argument");

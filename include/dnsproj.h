@@ -457,6 +457,91 @@ int dnsp_step_enqueue(dnsp_step* s, int refinements, void* stream);
 int dnsp_step_capture(dnsp_step* s, int refinements, void* stream);
 int dnsp_step_launch(dnsp_step* s, void* stream);
 int dnsp_step_free(dnsp_step* s);
+/* Sizes and constants of a step (any output may be NULL). */
+int dnsp_step_info(const dnsp_step* s, int* nx, int* nz, int* ny, double* inv_dx, double* inv_dz, double* dtmax,
+                   int* use_incompressible);
+
+/* ------------------------------------------------------------------------ */
+/* Predictor: the time-step stages before the projection, in the order of    */
+/* build_vars (src/mod_cfd_subroutines.f90:10-468): periodicity and walls of */
+/* U, V, W, T (cfd_periodicity :860-896), div_now ghost planes (:120-144),   */
+/* U_hat, V_hat, W_hat (:147-354), T (:367-407), periodicity of the hats and */
+/* T, then for variable properties update_props (:776-857) and               */
+/* build_rhs_energy (:618-745).  The bulk-pressure forcing                   */
+/* (force_Sf_x_pressure, :184-237) is not included.  It works on a step's    */
+/* U, V, W, hats, rho and div_now; dnsp_predictor_enqueue followed by         */
+/* dnsp_step_enqueue is one time step (main.f90 time loop).                  */
+/* ------------------------------------------------------------------------ */
+typedef struct dnsp_predictor dnsp_predictor;
+
+typedef struct {
+    double nu, cond, Sf_x, Sf_z, Sq, Gb_x, Gb_y, Gb_z;
+    double walls[8]; /* wall_BC_U/V/W/T_bot, then wall_BC_U/V/W/T_top */
+    int kmin_Vhat;   /* first V_hat plane (1 on the bottom rank, main.f90:86-87) */
+} dnsp_pred_params;
+
+/* Predictor fields: halo arrays (nx+2)(nz+2)(ny+2) except the Adams-Bashforth
+ * histories and RHS_energy (nx*nz*ny).  The property fields (DNSP_PRED_MU
+ * and after) exist in the variable-property form only. */
+#define DNSP_PRED_T          0
+#define DNSP_PRED_RU_OLD     1
+#define DNSP_PRED_RV_OLD     2
+#define DNSP_PRED_RW_OLD     3
+#define DNSP_PRED_RT_OLD     4
+#define DNSP_PRED_MU         5
+#define DNSP_PRED_COND       6
+#define DNSP_PRED_BUOY       7
+#define DNSP_PRED_CP         8
+#define DNSP_PRED_ENTH       9
+#define DNSP_PRED_DIVJAC     10
+#define DNSP_PRED_T_EXT      11
+#define DNSP_PRED_RHS_ENERGY 12
+#define DNSP_PRED_NFIELDS    13
+
+/* Property laws of T (the examples' expressions_rho_mu_cond_buoyancy_cp_div_jacobian.f90):
+ * DNSP_LAW_CONST data = {c}; DNSP_LAW_POWER data = {c, p}: c T^p; DNSP_LAW_CUBIC
+ * data = n segments of {hi, x0, c0, c1, c2, c3}, hi increasing (the last one
+ * unused): T takes the first segment with T <= hi and
+ * c0 + c1 (T-x0) + c2 (T-x0)^2 + c3 (T-x0)^3. */
+#define DNSP_LAW_CONST 0
+#define DNSP_LAW_POWER 1
+#define DNSP_LAW_CUBIC 2
+#define DNSP_Q_ENTH    0
+#define DNSP_Q_RHO     1
+#define DNSP_Q_MU      2
+#define DNSP_Q_COND    3
+#define DNSP_Q_CP      4
+#define DNSP_Q_DRHO_DT 5
+
+/* y-derivative coefficients of the predictor from the ny+1 faces
+ * (calc_bands.py:212-215, 272-277, 285-320, 345-357), host only: out holds
+ * Ruw_ddy (3 ny, a b c per row), Ruw_d2dy2 (3 ny), Rv_ddy (3 (ny+1)),
+ * Rv_d2dy2 (3 (ny+1)), Rv_interp_uw bottom/top (2 (ny+1)), inv_dy_u (ny+1),
+ * extrap_T_wall_0, _1, _m1, _m2: dnsp_predictor_geom_size(ny) doubles. */
+size_t dnsp_predictor_geom_size(int ny);
+int dnsp_predictor_geom(int ny, const double* yu, double* out);
+/* Laws default to constants rho = cp = 1, mu = nu, cond = cond, enth = 0;
+ * div_jacobian 0, buoyancy 0. */
+int dnsp_predictor_create(dnsp_step* s, const double* yu, const dnsp_pred_params* p, dnsp_predictor** out);
+int dnsp_predictor_field(dnsp_predictor* pr, int which, double** ptr, size_t* count);
+/* Device-to-device copies of a whole field (count as dnsp_predictor_field). */
+int dnsp_predictor_write(dnsp_predictor* pr, int which, const double* src, void* stream);
+int dnsp_predictor_read(dnsp_predictor* pr, int which, double* dst, void* stream);
+int dnsp_predictor_set_law(dnsp_predictor* pr, int quantity, int kind, const double* data, int n);
+/* div_jacobian: formula != 0: -1/(rho^2 cp) drho_dT, else the constant;
+ * buoyancy: linear != 0: b1 (rho - b0), else 0. */
+int dnsp_predictor_set_derived(dnsp_predictor* pr, int divjac_formula, double divjac_const, int buoy_linear,
+                               double b1, double b0);
+/* Rough-wall temperature columns (rough_T_bot/top_inds, nx*nz ints, i
+ * fastest; both NULL: smooth walls): T_ext takes the wall values at
+ * k <= bot and k >= top, div_now is zero there. */
+int dnsp_predictor_set_columns(dnsp_predictor* pr, const int* bot, const int* top);
+int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream);
+/* Records the predictor (and, with_step != 0, the step's sequence after
+ * it: one full time step) into a graph; dnsp_predictor_launch replays it. */
+int dnsp_predictor_capture(dnsp_predictor* pr, int with_step, int refinements, void* stream);
+int dnsp_predictor_launch(dnsp_predictor* pr, void* stream);
+int dnsp_predictor_free(dnsp_predictor* pr);
 
 /* ------------------------------------------------------------------------ */
 /* Serving engine (no counterpart in the reference): projection-step         */

@@ -29,8 +29,9 @@ def test_library_exports_every_declared_symbol():
     import dnsproj.projection as DP
     import dnsproj.serving as SV
     import dnsproj.io as IO
+    import dnsproj.predictor as PR
     bound = (set(D.exported_names()) | set(DP.exported_names()) | set(SV.exported_names()) |
-             set(IO.exported_names()))
+             set(IO.exported_names()) | set(PR.exported_names()))
     assert set(names) == bound, (set(names) - bound, bound - set(names))
 
 
