@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--mode", default="bench", choices=["bench", "counters"],
                     help="counters: one 906 MB device copy (calibration) then 3 FP64 and 3 mixed solves, "
                          "for rocprofv3 --pmc")
-    ap.add_argument("--extras", default="batch,step,serve",
+    ap.add_argument("--extras", default="batch,step,timestep,serve",
                     help="comma list of extra workloads to measure: batch (48 uneven instances), step "
                          "(384x256x192 projection step with IBM), serve (15-case request trace); '' = none")
     ap.add_argument("--serve-requests", type=int, default=400)
@@ -124,6 +124,62 @@ def bench_step(torch, D, W):
     res.update({"workload": "variable-property projection step nx=384 ny=256 nz=192, rho ratio ~10, rough "
                             "bottom wall k/delta=1/6 (64 Fourier modes, seed 3), one graph",
                 "mixed_U_err_vs_fp64": float(d)})
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_timestep(torch, D, W):
+    """Full time step on the 384x256x192 rough-wall grid: predictor (momentum,
+    T, supercritical-CO2 property tables of the reference's Wan 2025 example,
+    energy right-hand side) + projection step, one graph: FP64 vs mixed."""
+    import json as _json
+    from dnsproj import projection as DP
+    from dnsproj import predictor as PR
+    c = W.config4()
+    nx, nz, ny = c["nx"], c["nz"], c["ny"]
+    tab = _json.load(open(os.path.join(ROOT, "tests", "golden", "props_wan2025.json")))
+    rng = np.random.default_rng(11)
+    yu = np.asarray(c["yu"])
+    yp = 0.5 * (yu[:-1] + yu[1:])
+    ypad = np.concatenate([[2 * yu[0] - yp[0]], yp, [2 * yu[-1] - yp[-1]]])
+    sh = (ny + 2, nz + 2, nx + 2)
+    T = 317.0 - 19.0 * (ypad / ypad[-1])[:, None, None] + 0.5 * rng.standard_normal(sh)
+    f = dict(c["fields"])
+    f.update(U=f["U_hat"], V=f["V_hat"], W=f["W_hat"])
+    res = {}
+    for name, prec, ref in (("fp64", 0, -1), ("mixed", D.PRECISION_MIXED, 1)):
+        s = DP.ProjectionStep.from_case(c, precision=prec)
+        s.load(f)
+        pr = PR.Predictor(s, yu, 6e-5, 0.08, [0, 0, 0, 317.0, 0, 0, 0, 298.0], Sf_x=1.0, Gb=(0.0, 1.0, 0.0))
+        for q, segs in tab["laws"].items():
+            pr.set_law(q, ("cubic", [tuple(v) for v in segs]))
+        pr.set_derived(("formula",), ("linear", *tab["buoyancy"]))
+        pr.set_columns(c["bot"], c["top"])
+        pr.load({"T": np.clip(T, 298.5, 316.5)})
+        st = torch.cuda.Stream()
+        pr.enqueue(stream=st)  # properties consistent with T
+        K = 20
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        if name == "fp64":
+            e0.record(st)
+            for _ in range(K):
+                pr.enqueue(stream=st)
+            e1.record(st)
+            st.synchronize()
+            res["predictor_ms"] = round(e0.elapsed_time(e1) / K, 4)
+        pr.capture(with_step=True, refinements=ref, stream=st)
+        for _ in range(3):
+            pr.launch(stream=st)
+        e0.record(st)
+        for _ in range(K):
+            pr.launch(stream=st)
+        e1.record(st)
+        st.synchronize()
+        res[name + "_step_ms"] = round(e0.elapsed_time(e1) / K, 4)
+        pr.close()
+        s.close()
+    res["workload"] = ("full time step nx=384 ny=256 nz=192: predictor (variable properties, supercritical-CO2 "
+                       "spline tables, rough bottom wall k/delta=1/6 columns) + projection with IBM, one graph")
     torch.cuda.empty_cache()
     return res
 
@@ -309,6 +365,8 @@ def main():
                 extras["batched"] = bench_batch(torch, D, W)
             elif name == "step":
                 extras["projection_step"] = bench_step(torch, D, W)
+            elif name == "timestep":
+                extras["time_step"] = bench_timestep(torch, D, W)
             elif name == "serve":
                 extras["serving"] = bench_serve(torch, D, W, args.serve_requests)
         except Exception as e:  # reported, never hides the headline
