@@ -13,6 +13,7 @@
 // expressions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -38,6 +39,8 @@ struct Grid {
 };
 
 constexpr int kThreads = 256;
+// point-wise kernels: 64 x 4 (x, z) blocks, one plane per grid row in z
+constexpr int kBX = 64, kBZ = 4;
 
 inline int blocks_for(long n) { return (int)((n + kThreads - 1) / kThreads); }
 
@@ -156,7 +159,8 @@ __global__ void k_div_finish(double* D, Grid g) {
 // One thread per interior point: the x and z momentum right-hand sides at
 // the U and W points of (i, j, k), the y one at the V point when
 // k >= kmin_Vhat; X_hat = X + dt (1.5 R - 0.5 R_old), R_old = R
-// (mod_cfd_subroutines.f90:147-354).  Grid (ceil(nx/256), nz, ny).
+// (mod_cfd_subroutines.f90:147-354).  Grid (ceil(nx/64), ceil(nz/4), ny) of
+// 64 x 4 blocks: rows j-1..j+1 of a block share cache lines.
 template <bool VARPROP>
 __global__ __launch_bounds__(kThreads) void k_momentum(
     const double* __restrict__ U, const double* __restrict__ V, const double* __restrict__ W,
@@ -164,10 +168,10 @@ __global__ __launch_bounds__(kThreads) void k_momentum(
     const double* __restrict__ buoy, double* __restrict__ Ru_old, double* __restrict__ Rv_old,
     double* __restrict__ Rw_old, double* __restrict__ U_hat, double* __restrict__ V_hat, double* __restrict__ W_hat,
     Grid g, Geom G, Phys p, int kmin_Vhat) {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int j = (int)blockIdx.y;
+    const int i = (int)(blockIdx.x * kBX + threadIdx.x);
+    const int j = (int)(blockIdx.y * kBZ + threadIdx.y);
     const int k = (int)blockIdx.z;
-    if (i >= g.nx) return;
+    if (i >= g.nx || j >= g.nz) return;
     const long c = g.at(i, j, k);
     const long sz = g.sz(), sy = g.sy();
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * k);
@@ -289,10 +293,10 @@ __global__ __launch_bounds__(kThreads) void k_temp_rhs(const double* __restrict_
                                                        const double* __restrict__ rho, const double* __restrict__ cp,
                                                        const double* __restrict__ RHS, double* __restrict__ Rt_now,
                                                        Grid g, Geom G, Phys p) {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int j = (int)blockIdx.y;
+    const int i = (int)(blockIdx.x * kBX + threadIdx.x);
+    const int j = (int)(blockIdx.y * kBZ + threadIdx.y);
     const int k = (int)blockIdx.z;
-    if (i >= g.nx) return;
+    if (i >= g.nx || j >= g.nz) return;
     const long c = g.at(i, j, k);
     const long sz = g.sz(), sy = g.sy();
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * k);
@@ -319,10 +323,10 @@ __global__ __launch_bounds__(kThreads) void k_temp_rhs(const double* __restrict_
 
 __global__ void k_temp_update(double* __restrict__ T, const double* __restrict__ Rt_now, double* __restrict__ Rt_old,
                               Grid g, double dt) {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int j = (int)blockIdx.y;
+    const int i = (int)(blockIdx.x * kBX + threadIdx.x);
+    const int j = (int)(blockIdx.y * kBZ + threadIdx.y);
     const int k = (int)blockIdx.z;
-    if (i >= g.nx) return;
+    if (i >= g.nx || j >= g.nz) return;
     const long c = g.at(i, j, k);
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * k);
     const double rt = Rt_now[o];
@@ -352,7 +356,7 @@ struct PropSet {
     double b1, b0;
 };
 
-__device__ __forceinline__ double law_eval(const Law& L, const double* __restrict__ seg, double T) {
+__device__ __forceinline__ double law_eval(const Law& L, const double* seg, double T) {
     if (L.kind == 0) return L.c;
     if (L.kind == 1) {
         if (L.p == -1.0) return L.c / T;
@@ -375,49 +379,56 @@ __device__ __forceinline__ double law_eval(const Law& L, const double* __restric
 // T_ext and the seven properties at every padded point (update_props): in
 // rough-wall columns (bot/top given) T_ext takes the wall values for
 // k <= bot(i, j) and k >= top(i, j), halo points those of their periodic
-// image.
+// image.  Each workgroup stages the cubic laws' segments (nseg of them) in
+// LDS once and strides over the points, so the interval searches read LDS.
+constexpr int kMaxSegments = 1024;  // 48 KB of LDS
+
 __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T, double* __restrict__ T_ext,
                                                     double* __restrict__ enth, double* __restrict__ rho,
                                                     double* __restrict__ mu, double* __restrict__ cond,
                                                     double* __restrict__ cp, double* __restrict__ divjac,
                                                     double* __restrict__ buoy, const int* __restrict__ bot,
                                                     const int* __restrict__ top, double t_bot, double t_top,
-                                                    const double* __restrict__ seg, PropSet ps, Grid g) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= g.total()) return;
-    double te = T[idx];
-    if (bot != nullptr) {
-        const long pe = g.sy();
-        const int k = (int)(idx / pe) - 1;
-        const long r = idx - (long)(k + 1) * pe;
-        const int j = (int)(r / (g.nx + 2)) - 1;
-        const int i = (int)(r - (long)(j + 1) * (g.nx + 2)) - 1;
-        const int wi = i < 0 ? g.nx - 1 : (i >= g.nx ? 0 : i);
-        const int wj = j < 0 ? g.nz - 1 : (j >= g.nz ? 0 : j);
-        if (k >= 0 && k < g.ny) {
-            const long col = (long)wi + (long)g.nx * wj;
-            if (k <= bot[col]) te = t_bot;
-            if (k >= top[col]) te = t_top;
+                                                    const double* __restrict__ seg_g, int nseg, PropSet ps, Grid g) {
+    extern __shared__ double seg[];
+    for (int e = (int)threadIdx.x; e < 6 * nseg; e += kThreads) seg[e] = seg_g[e];
+    __syncthreads();
+    const long n = g.total();
+    for (long idx = (long)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (long)gridDim.x * kThreads) {
+        double te = T[idx];
+        if (bot != nullptr) {
+            const long pe = g.sy();
+            const int k = (int)(idx / pe) - 1;
+            const long r = idx - (long)(k + 1) * pe;
+            const int j = (int)(r / (g.nx + 2)) - 1;
+            const int i = (int)(r - (long)(j + 1) * (g.nx + 2)) - 1;
+            const int wi = i < 0 ? g.nx - 1 : (i >= g.nx ? 0 : i);
+            const int wj = j < 0 ? g.nz - 1 : (j >= g.nz ? 0 : j);
+            if (k >= 0 && k < g.ny) {
+                const long col = (long)wi + (long)g.nx * wj;
+                if (k <= bot[col]) te = t_bot;
+                if (k >= top[col]) te = t_top;
+            }
         }
+        const double h = law_eval(ps.law[Q_ENTH], seg, te);
+        const double r = law_eval(ps.law[Q_RHO], seg, te);
+        const double m = law_eval(ps.law[Q_MU], seg, te);
+        const double k = law_eval(ps.law[Q_COND], seg, te);
+        const double c = law_eval(ps.law[Q_CP], seg, te);
+        T_ext[idx] = te;
+        enth[idx] = h;
+        rho[idx] = r;
+        mu[idx] = m;
+        cond[idx] = k;
+        cp[idx] = c;
+        if (ps.divjac_formula) {
+            const double dr = law_eval(ps.law[Q_DRHO], seg, te);
+            divjac[idx] = (-1.0 / ((r * r) * c)) * dr;
+        } else {
+            divjac[idx] = ps.divjac_c;
+        }
+        buoy[idx] = ps.buoy_linear ? ps.b1 * (r - ps.b0) : 0.0;
     }
-    const double h = law_eval(ps.law[Q_ENTH], seg, te);
-    const double r = law_eval(ps.law[Q_RHO], seg, te);
-    const double m = law_eval(ps.law[Q_MU], seg, te);
-    const double k = law_eval(ps.law[Q_COND], seg, te);
-    const double c = law_eval(ps.law[Q_CP], seg, te);
-    T_ext[idx] = te;
-    enth[idx] = h;
-    rho[idx] = r;
-    mu[idx] = m;
-    cond[idx] = k;
-    cp[idx] = c;
-    if (ps.divjac_formula) {
-        const double dr = law_eval(ps.law[Q_DRHO], seg, te);
-        divjac[idx] = (-1.0 / ((r * r) * c)) * dr;
-    } else {
-        divjac[idx] = ps.divjac_c;
-    }
-    buoy[idx] = ps.buoy_linear ? ps.b1 * (r - ps.b0) : 0.0;
 }
 
 // RHS_energy and div_now = divjac RHS_energy at interior points
@@ -428,10 +439,10 @@ __global__ __launch_bounds__(kThreads) void k_rhs_energy(const double* __restric
                                                          double* __restrict__ RHS, double* __restrict__ dvn,
                                                          const int* __restrict__ bot, const int* __restrict__ top,
                                                          Grid g, Geom G, Phys p) {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int j = (int)blockIdx.y;
+    const int i = (int)(blockIdx.x * kBX + threadIdx.x);
+    const int j = (int)(blockIdx.y * kBZ + threadIdx.y);
     const int kk = (int)blockIdx.z;
-    if (i >= g.nx) return;
+    if (i >= g.nx || j >= g.nz) return;
     const long c = g.at(i, j, kk);
     const long sz = g.sz(), sy = g.sy();
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * kk);
@@ -701,6 +712,8 @@ int dnsp_predictor_set_law(dnsp_predictor* pr, int quantity, int kind, const dou
             o.off = off;
             off += o.nseg;
         }
+        DNSP_ARG_CHECK(off + n <= kMaxSegments, "at most %d cubic segments over all laws (got %d)", kMaxSegments,
+                       off + n);
         L.nseg = n;
         L.off = off;
         keep.insert(keep.end(), data, data + 6 * (size_t)n);
@@ -772,7 +785,8 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     const double* wl = pr->walls;
     const long per_plane = 2L * (g.nz + 2) + 2L * g.nx;
     const long nper = 9 * g.sy() + 4 * per_plane * g.ny;
-    const dim3 grid3((unsigned)((g.nx + kThreads - 1) / kThreads), (unsigned)g.nz, (unsigned)g.ny);
+    const dim3 grid3((unsigned)((g.nx + kBX - 1) / kBX), (unsigned)((g.nz + kBZ - 1) / kBZ), (unsigned)g.ny);
+    const dim3 blk3(kBX, kBZ, 1);
     const bool vp = !pr->incompressible;
     // 1. cfd_periodicity(U, V, W, T)
     k_periodicity4<<<blocks_for(nper), kThreads, 0, st>>>(U, V, W, T, g, wl[0], wl[1], wl[2], wl[3], wl[4], wl[5],
@@ -785,31 +799,33 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     double* mu = pr->f[DNSP_PRED_MU];
     double* buoy = pr->f[DNSP_PRED_BUOY];
     if (vp)
-        k_momentum<true><<<grid3, kThreads, 0, st>>>(U, V, W, mu, rho, dvn, buoy, pr->f[DNSP_PRED_RU_OLD],
+        k_momentum<true><<<grid3, blk3, 0, st>>>(U, V, W, mu, rho, dvn, buoy, pr->f[DNSP_PRED_RU_OLD],
                                                      pr->f[DNSP_PRED_RV_OLD], pr->f[DNSP_PRED_RW_OLD], Uh, Vh, Wh, g,
                                                      pr->G, pr->p, pr->kmin_Vhat);
     else
-        k_momentum<false><<<grid3, kThreads, 0, st>>>(U, V, W, nullptr, nullptr, nullptr, nullptr,
+        k_momentum<false><<<grid3, blk3, 0, st>>>(U, V, W, nullptr, nullptr, nullptr, nullptr,
                                                       pr->f[DNSP_PRED_RU_OLD], pr->f[DNSP_PRED_RV_OLD],
                                                       pr->f[DNSP_PRED_RW_OLD], Uh, Vh, Wh, g, pr->G, pr->p,
                                                       pr->kmin_Vhat);
     // 4. temperature
     if (vp)
-        k_temp_rhs<true><<<grid3, kThreads, 0, st>>>(T, U, V, W, rho, pr->f[DNSP_PRED_CP],
+        k_temp_rhs<true><<<grid3, blk3, 0, st>>>(T, U, V, W, rho, pr->f[DNSP_PRED_CP],
                                                      pr->f[DNSP_PRED_RHS_ENERGY], pr->d_rt_now, g, pr->G, pr->p);
     else
-        k_temp_rhs<false><<<grid3, kThreads, 0, st>>>(T, U, V, W, nullptr, nullptr, nullptr, pr->d_rt_now, g, pr->G,
+        k_temp_rhs<false><<<grid3, blk3, 0, st>>>(T, U, V, W, nullptr, nullptr, nullptr, pr->d_rt_now, g, pr->G,
                                                       pr->p);
-    k_temp_update<<<grid3, kThreads, 0, st>>>(T, pr->d_rt_now, pr->f[DNSP_PRED_RT_OLD], g, pr->p.dt);
+    k_temp_update<<<grid3, blk3, 0, st>>>(T, pr->d_rt_now, pr->f[DNSP_PRED_RT_OLD], g, pr->p.dt);
     // 5. cfd_periodicity(U_hat, V_hat, W_hat, T)
     k_periodicity4<<<blocks_for(nper), kThreads, 0, st>>>(Uh, Vh, Wh, T, g, wl[0], wl[1], wl[2], wl[3], wl[4],
                                                           wl[5], wl[6], wl[7]);
     // 6. properties at the new temperature, energy right-hand side, div_now
     if (vp) {
-        k_props<<<blocks_for(g.total()), kThreads, 0, st>>>(
+        const int nseg = (int)(pr->h_seg.size() / 6);
+        const int nbp = std::min(blocks_for(g.total()), 2048);
+        k_props<<<nbp, kThreads, sizeof(double) * 6 * nseg, st>>>(
             T, pr->f[DNSP_PRED_T_EXT], pr->f[DNSP_PRED_ENTH], rho, mu, pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_CP],
-            pr->f[DNSP_PRED_DIVJAC], buoy, pr->d_bot, pr->d_top, wl[3], wl[7], pr->d_seg, pr->ps, g);
-        k_rhs_energy<<<grid3, kThreads, 0, st>>>(T, pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_DIVJAC],
+            pr->f[DNSP_PRED_DIVJAC], buoy, pr->d_bot, pr->d_top, wl[3], wl[7], pr->d_seg, nseg, pr->ps, g);
+        k_rhs_energy<<<grid3, blk3, 0, st>>>(T, pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_DIVJAC],
                                                  pr->f[DNSP_PRED_RHS_ENERGY], dvn, pr->d_bot, pr->d_top, g, pr->G,
                                                  pr->p);
         k_div_finish<<<blocks_for(2 * g.sy() + per_plane * g.ny), kThreads, 0, st>>>(dvn, g);
