@@ -112,6 +112,28 @@ __global__ void k_periodicity4(double* A, double* B, double* C, double* T, Grid 
     X[g.at(i, j, k)] = X[g.at(wi, wj, k)];
 }
 
+// Wall planes of ibm_part_1 (src/ibm_setup/mod_cfd_ibm.f90:26-38): k = -1
+// and ny of A, B, C, T over the whole padded plane (B bottom: k = -1 only).
+__global__ void k_walls4(double* A, double* B, double* C, double* T, Grid g, double ab, double bb, double cb,
+                         double tb, double at, double bt, double ct, double tt) {
+    const long pe = g.sy();
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 8 * pe) return;
+    const int which = (int)(idx / pe);
+    const long o = idx % pe;
+    const long top = (long)(g.ny + 1) * pe + o;
+    switch (which) {
+        case 0: A[o] = ab; break;
+        case 1: B[o] = bb; break;
+        case 2: C[o] = cb; break;
+        case 3: T[o] = tb; break;
+        case 4: A[top] = at; break;
+        case 5: B[top] = bt; break;
+        case 6: C[top] = ct; break;
+        default: T[top] = tt; break;
+    }
+}
+
 // div_now ghost planes by linear extrapolation of the first two interior
 // planes (mod_cfd_subroutines.f90:120-144), whole padded planes
 __global__ void k_div_walls(double* D, Grid g, double e0, double e1, double em1, double em2) {
@@ -489,6 +511,7 @@ struct dnsp_predictor {
     size_t d_seg_n = 0;
     int* d_bot = nullptr;
     int* d_top = nullptr;
+    dnsp_ibm* ibm[5] = {};  // U, V, W, T, div_now forcing (borrowed)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -766,6 +789,14 @@ int dnsp_predictor_set_columns(dnsp_predictor* pr, const int* bot, const int* to
     return DNSP_OK;
 }
 
+int dnsp_predictor_set_ibm(dnsp_predictor* pr, int component, dnsp_ibm* h) {
+    clear_error();
+    DNSP_ARG_CHECK(pr, "NULL predictor");
+    DNSP_ARG_CHECK(component >= 0 && component < 5, "component %d out of range (U, V, W, T, div)", component);
+    pr->ibm[component] = h;
+    return DNSP_OK;
+}
+
 int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     clear_error();
     DNSP_ARG_CHECK(pr, "NULL predictor");
@@ -788,9 +819,23 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     const dim3 grid3((unsigned)((g.nx + kBX - 1) / kBX), (unsigned)((g.nz + kBZ - 1) / kBZ), (unsigned)g.ny);
     const dim3 blk3(kBX, kBZ, 1);
     const bool vp = !pr->incompressible;
-    // 1. cfd_periodicity(U, V, W, T)
-    k_periodicity4<<<blocks_for(nper), kThreads, 0, st>>>(U, V, W, T, g, wl[0], wl[1], wl[2], wl[3], wl[4], wl[5],
-                                                          wl[6], wl[7]);
+    const bool rough = pr->d_bot || pr->ibm[0] || pr->ibm[1] || pr->ibm[2] || pr->ibm[3] || pr->ibm[4];
+    void* vs = stream;
+    if (!rough) {
+        // 1. cfd_periodicity(U, V, W, T)
+        k_periodicity4<<<blocks_for(nper), kThreads, 0, st>>>(U, V, W, T, g, wl[0], wl[1], wl[2], wl[3], wl[4],
+                                                              wl[5], wl[6], wl[7]);
+    } else {
+        // 1. ibm_part_1 (mod_cfd_ibm.f90:10-45): forcing of U, V, W, T, wall planes, forcing of div_now
+        double* F4[4] = {U, V, W, T};
+        for (int c = 0; c < 4 && !rc; ++c)
+            if (pr->ibm[c]) rc = dnsp_ibm_exec(pr->ibm[c], F4[c], vs);
+        if (rc) return rc;
+        k_walls4<<<blocks_for(8 * g.sy()), kThreads, 0, st>>>(U, V, W, T, g, wl[0], wl[1], wl[2], wl[3], wl[4],
+                                                              wl[5], wl[6], wl[7]);
+        if (vp && pr->ibm[4]) rc = dnsp_ibm_exec(pr->ibm[4], dvn, vs);
+        if (rc) return rc;
+    }
     // 2. div_now ghost planes
     if (vp)
         k_div_walls<<<blocks_for(2 * g.sy()), kThreads, 0, st>>>(dvn, g, pr->extrap[0], pr->extrap[1],
@@ -815,7 +860,18 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
         k_temp_rhs<false><<<grid3, blk3, 0, st>>>(T, U, V, W, nullptr, nullptr, nullptr, pr->d_rt_now, g, pr->G,
                                                       pr->p);
     k_temp_update<<<grid3, blk3, 0, st>>>(T, pr->d_rt_now, pr->f[DNSP_PRED_RT_OLD], g, pr->p.dt);
-    // 5. cfd_periodicity(U_hat, V_hat, W_hat, T)
+    // 5. rough walls: columns of the hats and T (ibm_part_2, mod_cfd_ibm.f90:47-141)
+    if (rough && pr->d_bot) {
+        double* H4[4] = {Uh, Vh, Wh, T};
+        const double wb[4] = {wl[0], wl[1], wl[2], wl[3]}, wt[4] = {wl[4], wl[5], wl[6], wl[7]};
+        for (int c = 0; c < 4 && !rc; ++c)
+            rc = dnsp_ibm_part2(H4[c], pr->d_bot, pr->d_top, g.nx, g.nz, g.ny, c == 1 ? g.ny : g.ny - 1, wb[c], wt[c],
+                                vs);
+        if (rc) return rc;
+    }
+    // periodicity of the hats and T with their wall planes (with rough walls
+    // the reference sets only the halos the right-hand side reads, V_hat's
+    // walls and T's; the extra halo and wall values are read by nothing)
     k_periodicity4<<<blocks_for(nper), kThreads, 0, st>>>(Uh, Vh, Wh, T, g, wl[0], wl[1], wl[2], wl[3], wl[4],
                                                           wl[5], wl[6], wl[7]);
     // 6. properties at the new temperature, energy right-hand side, div_now
@@ -825,6 +881,13 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
         k_props<<<nbp, kThreads, sizeof(double) * 6 * nseg, st>>>(
             T, pr->f[DNSP_PRED_T_EXT], pr->f[DNSP_PRED_ENTH], rho, mu, pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_CP],
             pr->f[DNSP_PRED_DIVJAC], buoy, pr->d_bot, pr->d_top, wl[3], wl[7], pr->d_seg, nseg, pr->ps, g);
+        if (pr->ibm[3]) {
+            // build_rhs_energy (:640-645): T forcing, then T's wall planes
+            rc = dnsp_ibm_exec(pr->ibm[3], T, vs);
+            if (!rc) rc = dnsp_set_val_slice(T, g.nx, g.nz, g.ny, 0, 0, wl[3], vs);
+            if (!rc) rc = dnsp_set_val_slice(T, g.nx, g.nz, g.ny, -1, -1, wl[7], vs);
+            if (rc) return rc;
+        }
         k_rhs_energy<<<grid3, blk3, 0, st>>>(T, pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_DIVJAC],
                                                  pr->f[DNSP_PRED_RHS_ENERGY], dvn, pr->d_bot, pr->d_top, g, pr->G,
                                                  pr->p);

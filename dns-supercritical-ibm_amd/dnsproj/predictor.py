@@ -38,6 +38,7 @@ _SIGS = [
     ("dnsp_predictor_set_law", [_vp, _i, _i, _dp, _i]),
     ("dnsp_predictor_set_derived", [_vp, _i, _d, _i, _d, _d]),
     ("dnsp_predictor_set_columns", [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    ("dnsp_predictor_set_ibm", [_vp, _i, _vp]),
     ("dnsp_predictor_enqueue", [_vp, _vp]),
     ("dnsp_predictor_capture", [_vp, _i, _i, _vp]),
     ("dnsp_predictor_launch", [_vp, _vp]),
@@ -155,6 +156,13 @@ class Predictor:
         t = np.ascontiguousarray(top, dtype=np.int32)
         ip = ctypes.POINTER(_i)
         _check(_lib().dnsp_predictor_set_columns(self._h, b.ctypes.data_as(ip), t.ctypes.data_as(ip)))
+
+    def set_ibm(self, forcings):
+        """forcings: {"U"|"V"|"W"|"T"|"div": IbmForcing or None}; the handles must outlive the predictor."""
+        self._ibm = dict(forcings)
+        for k, name in enumerate(("U", "V", "W", "T", "div")):
+            h = self._ibm.get(name)
+            _check(_lib().dnsp_predictor_set_ibm(self._h, k, h._h if h is not None else _vp(None)))
 
     def enqueue(self, stream=None):
         _check(_lib().dnsp_predictor_enqueue(self._h, _vp(_stream_ptr(stream))))

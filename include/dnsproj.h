@@ -536,6 +536,15 @@ int dnsp_predictor_set_derived(dnsp_predictor* pr, int divjac_formula, double di
  * fastest; both NULL: smooth walls): T_ext takes the wall values at
  * k <= bot and k >= top, div_now is zero there. */
 int dnsp_predictor_set_columns(dnsp_predictor* pr, const int* bot, const int* top);
+/* Rough-wall forcing (borrowed handles, NULL: none) for component 0..4 =
+ * U, V, W, T, div_now (the reference's obj_ibm_U/V/W/T/div).  With any
+ * forcing or columns set the predictor follows build_vars' rough-wall
+ * branch (use_rough_surf without quady): ibm_part_1 (forcing of U, V, W, T,
+ * wall planes, div_now forcing) instead of the opening periodicity,
+ * ibm_part_2 on the hats and T after the predictor (the column slabs of
+ * dnsp_predictor_set_columns for all four), and the T forcing at the start
+ * of build_rhs_energy. */
+int dnsp_predictor_set_ibm(dnsp_predictor* pr, int component, dnsp_ibm* h);
 int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream);
 /* Records the predictor (and, with_step != 0, the step's sequence after
  * it: one full time step) into a graph; dnsp_predictor_launch replays it. */

@@ -29,7 +29,7 @@ import math
 
 import numpy as np
 
-from projection import halo_periodic_xz, set_val_slice
+from projection import halo_periodic_xz, ibm_exec, ibm_part2, set_val_slice
 
 THIRD = 0.3333333333333333
 TWO_THIRDS = 0.6666666666666666
@@ -397,20 +397,53 @@ def rhs_energy(T, cond, divjac, div_now, g, p, bot=None, top=None):
 # ---------------------------------------------------------------------------
 # the predictor stage of one time step, in build_vars' order (smooth walls)
 # ---------------------------------------------------------------------------
-def predictor_step(st, g, p, laws=None, derived=None, incompressible=True, kmin_Vhat=1, bot=None, top=None):
+def predictor_step(st, g, p, laws=None, derived=None, incompressible=True, kmin_Vhat=1, bot=None, top=None,
+                   ibm=None):
     """st: dict of arrays (U, V, W, T, mu, rho, div_now, buoy, cp, cond, divjac,
     enth, T_ext halo arrays; Ru_old, Rv_old, Rw_old, Rt_old, RHS_energy interior
-    arrays); updated in place; returns (U_hat, V_hat, W_hat)."""
+    arrays); updated in place; returns (U_hat, V_hat, W_hat).  bot/top: column
+    slabs (nx, nz); ibm: {"U"|"V"|"W"|"T"|"div": (ijk, c)} forcing records.
+    With columns or forcing, build_vars' rough-wall branch (without quady):
+    ibm_part_1 opens (mod_cfd_ibm.f90:10-45), ibm_part_2 on the hats and T
+    follows the predictor (:47-141), T is forced again in build_rhs_energy
+    (mod_cfd_subroutines.f90:640-645)."""
     walls = p["walls"]
-    periodicity(st["U"], st["V"], st["W"], st["T"], walls)
+    ibm = ibm or {}
+    rough = bot is not None or bool(ibm)
+    ny2, nz2, nx2 = st["U"].shape
+    nx, nz, ny = nx2 - 2, nz2 - 2, ny2 - 2
+
+    def force(name, A):
+        if name in ibm:
+            A[...] = ibm_exec(A, ibm[name][0], ibm[name][1], nx, nz, ny, 1, 0)
+
+    if not rough:
+        periodicity(st["U"], st["V"], st["W"], st["T"], walls)
+    else:
+        for name in ("U", "V", "W", "T"):
+            force(name, st[name])
+        for name in ("U", "V", "W", "T"):
+            set_val_slice(st[name], 0, 0, walls[name + "_bot"])
+        for name in ("U", "V", "W", "T"):
+            set_val_slice(st[name], -1, -1, walls[name + "_top"])
+        if not incompressible:
+            force("div", st["div_now"])
     if not incompressible:
         div_now_walls(st["div_now"], g["extrap"])
     U_hat, V_hat, W_hat = momentum(st["U"], st["V"], st["W"], st["mu"], st["rho"], st["div_now"], st["buoy"],
                                    st["Ru_old"], st["Rv_old"], st["Rw_old"], g, p, incompressible, kmin_Vhat)
     advance_T(st["T"], st["U"], st["V"], st["W"], st["rho"], st["cp"], st["RHS_energy"], st["Rt_old"], g, p,
               incompressible)
+    if rough and bot is not None:
+        bj, tj = np.asarray(bot).T, np.asarray(top).T  # (nz, nx)
+        for A, name, kmax in ((U_hat, "U", ny - 1), (V_hat, "V", ny), (W_hat, "W", ny - 1), (st["T"], "T", ny - 1)):
+            A[...] = ibm_part2(A, bj, tj, kmax, walls[name + "_bot"], walls[name + "_top"])
     periodicity(U_hat, V_hat, W_hat, st["T"], walls)
     if not incompressible:
         update_props(st["T"], st, laws, derived, walls, bot, top)
+        if "T" in ibm:
+            force("T", st["T"])
+            set_val_slice(st["T"], 0, 0, walls["T_bot"])
+            set_val_slice(st["T"], -1, -1, walls["T_top"])
         st["RHS_energy"][...] = rhs_energy(st["T"], st["cond"], st["divjac"], st["div_now"], g, p, bot, top)
     return U_hat, V_hat, W_hat

@@ -159,6 +159,36 @@ def test_predictor_rough_columns_match_oracle(cuda):
     s.close()
 
 
+@pytest.mark.parametrize("kind", ["incomp", "wan"])
+def test_predictor_rough_ibm_matches_oracle(cuda, kind):
+    """The rough-wall branch with forcing of U, V, W, T and div_now plus columns."""
+    say("predictor rough ibm", kind)
+    nx, nz, ny = 24, 12, 20
+    c = make_case(nx, nz, ny, seed=21, kind=kind)
+    h = W.rough_height_map(nx, nz, 0.08, 5)
+    recs, bot, top = W.rough_wall_ibm(nx, nz, ny, c["yu"], h)
+    yp = 0.5 * (c["yu"][:-1] + c["yu"][1:])
+    recs["T"] = recs["W"]
+    recs["div"] = (recs["W"][0], DP.ibm_div_coeffs(recs["W"][0], yp, nx, nz, c["p"]["inv_dx"], c["p"]["inv_dz"]))
+    s, pr = build(c, bot, top)
+    handles = {k: DP.IbmForcing(nx, nz, ny, 1, 0, *recs[k]) for k in ("U", "V", "W", "T", "div")}
+    pr.set_ibm(handles)
+    load(s, pr, c)
+    pr.enqueue()
+    ibm = {k: (recs[k][0].reshape(-1, 3, 3), recs[k][1]) for k in recs}
+    st = {k: v.copy() for k, v in list(c["f"].items()) + list(c["pf"].items())}
+    shp = c["f"]["U"].shape
+    for k in ("mu", "buoy", "cp", "cond", "enth", "divjac", "T_ext"):
+        st.setdefault(k, np.zeros(shp))
+    st.setdefault("RHS_energy", np.zeros((ny, nz, nx)))
+    Uh, Vh, Wh = PRO.predictor_step(st, PRO.predictor_geom(c["yu"]), c["p"], c["laws"], c["derived"],
+                                    c["incompressible"], 1, bot.T, top.T, ibm)
+    st.update(U_hat=Uh, V_hat=Vh, W_hat=Wh)
+    compare(s, pr, st, c)
+    pr.close()
+    s.close()
+
+
 def test_full_step_graph_replays_match_eager(cuda):
     import torch
     say("full time step graph")
