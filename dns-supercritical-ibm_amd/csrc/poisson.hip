@@ -513,10 +513,10 @@ int dnsp_poisson_get_variants(const dnsp_poisson* h, int* xvar, int* zvar, int* 
 // and pairs of tridiagonal solves per partitioned variant, on `work`
 // (zero-filled first; NULL = a temporary device array) and keeps the fastest;
 // for a mixed-precision handle the same for its kernels.  times_ms (optional,
-// DNSP_TUNE_SLOTS entries): mean ms of the x, z and tridiagonal variants (4
-// slots each), then the same for the mixed-precision kernels; -1 where not
-// applicable.
-static_assert(kCtVariants == 4 && kPartVariants == 4, "tuner table layout");
+// DNSP_TUNE_SLOTS entries): mean ms of the x, z (4 slots each) and
+// tridiagonal (8 slots) variants, then the same for the mixed-precision
+// kernels; -1 where not applicable.
+static_assert(kCtVariants == 4 && kPartVariants == 8 && DNSP_TUNE_SLOTS == 2 * kTuneFormSlots, "tuner table layout");
 int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream, double* times_ms) {
     clear_error();
     DNSP_ARG_CHECK(h, "NULL handle");
@@ -565,7 +565,7 @@ int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream,
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, e0, e1);
                 const double per = (double)ms / reps;
-                if (times_ms) times_ms[dir * 4 + v] = per;
+                if (times_ms) times_ms[kTuneSlot[dir] + v] = per;
                 if (per < best[dir]) {
                     best[dir] = per;
                     pick[dir] = v;
@@ -585,7 +585,7 @@ int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream,
     h->xvar = pick[0];
     h->zvar = pick[1];
     h->tvar = pick[2];
-    if (h->precision == DNSP_PRECISION_MIXED) rc = mixed_autotune(h, buf, reps, st, times_ms ? times_ms + 12 : nullptr);
+    if (h->precision == DNSP_PRECISION_MIXED) rc = mixed_autotune(h, buf, reps, st, times_ms ? times_ms + kTuneFormSlots : nullptr);
     if (!work && buf) (void)hipFree(buf);
     return rc;
 }

@@ -130,8 +130,9 @@ __global__ __launch_bounds__(ct_var_threads(V)) void kb_z(const BInst* __restric
 // Tridiagonal: one mode group of 8 modes per task.  KIND 0: FP64 on P;
 // 1: FP32 on A with the (0,0) mode -> pbar0; 2: FP32 on B -> pbar1.
 constexpr int kBatchM = 8;
+constexpr int kBatchTriVariants = 4;  // {registers uncapped, capped for 4 waves} x {one-pass, two-pass}
 
-template <int MR, int KIND, int OCC>
+template <int MR, int KIND, int OCC, bool TP>
 __global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri(const BInst* __restrict__ inst,
                                                                       const int* __restrict__ ids,
                                                                       const long* __restrict__ start, int cnt) {
@@ -148,15 +149,15 @@ __global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri(const BInst
     const int mi = (int)threadIdx.x % M;
     const int s = ((int)threadIdx.x / M) * MR;
     const long mode = tri_group_mode(grp, (b.plane + M - 1) / M, M, mi);
-    const double* set = b.bands + (grp == 0 ? 3 * b.ny : 0);
-    for (int e = (int)threadIdx.x; e < 3 * b.ny; e += NT) smb[e] = set[e];
+    tri_stage_bands<MR, P>(smb, b.bands + (grp == 0 ? 3 * b.ny : 0), b.ny, NT);
     T* S = KIND == 0 ? reinterpret_cast<T*>(b.P) : reinterpret_cast<T*>(KIND == 1 ? b.A : b.B);
     const double* msum = KIND == 0 ? nullptr : b.small;
     double* pbar = KIND == 0 ? nullptr : b.small + (KIND == 1 ? 1 : 2) * b.ny;
     double d[MR];
     tri_load_rows<MR, P, M, T>(d, S, mode, b.plane, b.ny, s, msum);
     __syncthreads();
-    tri_group<MR, P, M, T>(d, S, b.ax, b.az, smb, b.nx, b.ny, b.plane, b.pcr_min, mode, msum, pbar, b.inv_nxnz, red);
+    tri_group<MR, P, M, T, TP>(d, S, b.ax, b.az, smb, b.nx, b.ny, b.plane, b.pcr_min, mode, msum, pbar, b.inv_nxnz,
+                               red);
 }
 
 // q(k) of the first refinement for every plane of every instance (one
@@ -271,22 +272,25 @@ static void bz_launch(int kind, int v, const BInst* inst, const BGroup& g, hipSt
     }
 }
 
-template <int MR, int OCC>
-static void bt_launch_m(int kind, const BInst* inst, const BGroup& g, size_t lds, hipStream_t st) {
+template <int MR, int OCC, bool TP>
+static void bt_launch_m(int kind, const BInst* inst, const BGroup& g, hipStream_t st) {
     const int nb = (int)g.total;
     constexpr int NT = kBatchM * kPartChunks;
-    if (kind == 0) kb_tri<MR, 0, OCC><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
-    else if (kind == 1) kb_tri<MR, 1, OCC><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
-    else kb_tri<MR, 2, OCC><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
+    constexpr size_t lds = 3 * sizeof(double) * MR * kPartChunks;
+    if (kind == 0) kb_tri<MR, 0, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
+    else if (kind == 1) kb_tri<MR, 1, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
+    else kb_tri<MR, 2, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
 }
 
-static void bt_launch(int kind, int v, const BInst* inst, const BGroup& g, size_t lds, hipStream_t st) {
+static void bt_launch(int kind, int v, const BInst* inst, const BGroup& g, hipStream_t st) {
     if (g.total <= 0) return;
     switch (g.key) {
 #define DNSP_CASE(MR)                                            \
     case MR:                                                     \
-        if (v == 0) bt_launch_m<MR, 1>(kind, inst, g, lds, st);  \
-        else bt_launch_m<MR, 4>(kind, inst, g, lds, st);         \
+        if (v == 0) bt_launch_m<MR, 1, false>(kind, inst, g, st);      \
+        else if (v == 1) bt_launch_m<MR, 4, false>(kind, inst, g, st); \
+        else if (v == 2) bt_launch_m<MR, 1, true>(kind, inst, g, st);  \
+        else bt_launch_m<MR, 4, true>(kind, inst, g, st);              \
         break;
         DNSP_PART_ROWS(DNSP_CASE)
 #undef DNSP_CASE
@@ -523,12 +527,11 @@ int upload_inst(dnsp_batch* b) {
     return DNSP_OK;
 }
 
-size_t tri_lds(const dnsp_batch* b) { return 3 * sizeof(double) * (size_t)b->max_ny; }
 
 void run_fp64(dnsp_batch* b, int xv, int zv, int tv, hipStream_t st) {
     for (const BGroup& g : b->gx[xv]) bx_launch(BX_F64_FWD, xv, b->d_inst, g, 0, st);
     for (const BGroup& g : b->gz[zv]) bz_launch(BZ_F64_FWD, zv, b->d_inst, g, st);
-    for (const BGroup& g : b->gt) bt_launch(0, tv, b->d_inst, g, tri_lds(b), st);
+    for (const BGroup& g : b->gt) bt_launch(0, tv, b->d_inst, g, st);
     for (const BGroup& g : b->gz[zv]) bz_launch(BZ_F64_BWD, zv, b->d_inst, g, st);
     for (const BGroup& g : b->gx[xv]) bx_launch(BX_F64_BWD, xv, b->d_inst, g, 0, st);
 }
@@ -536,7 +539,7 @@ void run_fp64(dnsp_batch* b, int xv, int zv, int tv, hipStream_t st) {
 void run_mixed(dnsp_batch* b, int refinements, int xv, int zv, int tv, hipStream_t st) {
     for (const BGroup& g : b->gx[xv]) bx_launch(BX_MX_FWD, xv, b->d_inst, g, 0, st);
     for (const BGroup& g : b->gz[zv]) bz_launch(BZ_A_FWD, zv, b->d_inst, g, st);
-    for (const BGroup& g : b->gt) bt_launch(1, tv, b->d_inst, g, tri_lds(b), st);
+    for (const BGroup& g : b->gt) bt_launch(1, tv, b->d_inst, g, st);
     for (const BGroup& g : b->gz[zv]) bz_launch(BZ_A_BWD, zv, b->d_inst, g, st);
     if (refinements == 0) {
         for (const BGroup& g : b->gx[xv]) bx_launch(BX_MX_OUT0, xv, b->d_inst, g, 0, st);
@@ -546,7 +549,7 @@ void run_mixed(dnsp_batch* b, int refinements, int xv, int zv, int tv, hipStream
     if (b->gq.total > 0) kb_q<<<(int)b->gq.total, 64, 0, st>>>(b->d_inst, b->gq.ids, b->gq.start, b->gq.cnt);
     for (const BGroup& g : b->gx[xv]) bx_launch(BX_MX_RES, xv, b->d_inst, g, 0, st);
     for (const BGroup& g : b->gz[zv]) bz_launch(BZ_B_FWD, zv, b->d_inst, g, st);
-    for (const BGroup& g : b->gt) bt_launch(2, tv, b->d_inst, g, tri_lds(b), st);
+    for (const BGroup& g : b->gt) bt_launch(2, tv, b->d_inst, g, st);
     for (const BGroup& g : b->gz[zv]) bz_launch(BZ_B_BWD, zv, b->d_inst, g, st);
     for (const BGroup& g : b->gx[xv]) bx_launch(BX_MX_ACC, xv, b->d_inst, g, 0, st);
 }
@@ -646,7 +649,7 @@ int dnsp_batch_set_variants(dnsp_batch* b, int xvar, int zvar, int tvar, int mxv
     clear_error();
     DNSP_ARG_CHECK(b, "NULL batch");
     for (int v : {xvar, zvar, mxvar, mzvar}) DNSP_ARG_CHECK(v >= 0 && v < kCtVariants, "FFT variant %d", v);
-    for (int v : {tvar, mtvar}) DNSP_ARG_CHECK(v == 0 || v == 1, "tridiagonal variant %d", v);
+    for (int v : {tvar, mtvar}) DNSP_ARG_CHECK(v >= 0 && v < kBatchTriVariants, "tridiagonal variant %d", v);
     b->xvar = xvar;
     b->zvar = zvar;
     b->tvar = tvar;
@@ -667,8 +670,9 @@ int dnsp_batch_get_variants(const dnsp_batch* b, int* v6) {
 // Tunes the variants of the whole batched solve: each FP64 (and, for a mixed
 // batch, mixed) solve is timed with one knob changed at a time, x then z then
 // tridiagonal; overwrites the bound solution arrays.  times_ms (optional,
-// DNSP_TUNE_SLOTS): ms per solve, FP64 x, z, tridiagonal variants (4 slots
-// each), then the mixed ones; -1 where not applicable.
+// DNSP_TUNE_SLOTS): ms per solve, FP64 x, z, tridiagonal variants (at the
+// slots of dnsp_poisson_autotune), then the mixed ones; -1 where not
+// applicable.
 int dnsp_batch_autotune(dnsp_batch* b, int reps, void* stream, double* times_ms) {
     clear_error();
     int rc = check_bound(b);
@@ -688,7 +692,7 @@ int dnsp_batch_autotune(dnsp_batch* b, int reps, void* stream, double* times_ms)
         if (form == 1 && b->precision != DNSP_PRECISION_MIXED) break;
         int* var[3] = {form ? &b->mxvar : &b->xvar, form ? &b->mzvar : &b->zvar, form ? &b->mtvar : &b->tvar};
         for (int knob = 0; knob < 3 && rc == DNSP_OK; ++knob) {
-            const int nv = knob == 2 ? 2 : kCtVariants;
+            const int nv = knob == 2 ? kBatchTriVariants : kCtVariants;
             double best = 1e30;
             int pick = *var[knob];
             for (int v = 0; v < nv; ++v) {
@@ -709,7 +713,7 @@ int dnsp_batch_autotune(dnsp_batch* b, int reps, void* stream, double* times_ms)
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, e0, e1);
                 const double per = (double)ms / reps;
-                if (times_ms) times_ms[form * 12 + knob * 4 + v] = per;
+                if (times_ms) times_ms[form * kTuneFormSlots + kTuneSlot[knob] + v] = per;
                 if (per < best) {
                     best = per;
                     pick = v;

@@ -352,12 +352,34 @@ struct TriRed {
     double a[2][P][M], b[2][P][M], c[2][P][M], d[2][P][M];
 };
 
+// Stages one band set (a, b, c of ny rows each) into LDS as three arrays of
+// MR * P rows.  Rows past ny get a = c = 0, b = -1: decoupled from the rows
+// above and below, with the diagonal -1 + lambda nonzero for every mode
+// (lambda <= 0), so that they solve to x = 0 from d = 0 and the band reads
+// need no bounds test.
+template <int MR, int P>
+__device__ __forceinline__ void tri_stage_bands(double* smb, const double* __restrict__ set, int ny, int nt) {
+    constexpr int NP = MR * P;
+    for (int e = (int)threadIdx.x; e < 3 * NP; e += nt) {
+        const int which = e / NP, row = e - which * NP;
+        smb[e] = row < ny ? set[which * ny + row] : (which == 1 ? -1.0 : 0.0);
+    }
+}
+
 // One mode group: thread (mi, p) holds rows s..s+MR-1 of mode `mode` in d[]
 // and writes its solution rows (mode >= plane: an idle lane).  smb: the band
-// set of every mode of the group, staged in LDS; R: the LDS reduced system.
-// Contains workgroup barriers; all threads call it with the same group.
-
-template <int MR, int P, int M, typename T>
+// set of every mode of the group, staged in LDS by tri_stage_bands; R: the
+// LDS reduced system.  Contains workgroup barriers; all threads call it with
+// the same group.
+//
+// TP = false: the elimination keeps the normalised rows (A^, C^, D^) of the
+// chunk interior in registers and back-substitutes them once x_s and x_e are
+// known.  TP = true: the elimination keeps only the right-hand side and
+// accumulates x_{s+1} = Pp - Qq x_s - Rr x_e forwards; once x_s and x_e are
+// known the interior rows are solved again as a tridiagonal system with those
+// two values as boundary data.  About a third fewer values stay live across
+// the reduced solve, for one more elimination of the chunk.
+template <int MR, int P, int M, typename T, bool TP = false>
 __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* __restrict__ ax,
                                           const double* __restrict__ az, const double* smb, int nx, int ny,
                                           long plane, double pcr_min, long mode, const double* __restrict__ msum,
@@ -367,20 +389,45 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
     const int s = p * MR;
     const bool valid = mode < plane;
     const long md = valid ? mode : 0;
-    const int i = (int)(md % nx);
-    const int j = (int)(md / nx);
+    // plane is far below 2^32 (the FFT lengths are bounded by LDS)
+    const unsigned um = (unsigned)md;
+    const int i = (int)(um % (unsigned)nx);
+    const int j = (int)(um / (unsigned)nx);
     // idle lanes solve a regular system (the plain bands are singular at 0)
     const double lam = valid ? ax[i] + az[j] : -1.0;
     const double* A = smb;
-    const double* B = A + ny;
-    const double* C = B + ny;
-    auto band_a = [&](int row) { return row < ny ? A[row] : 0.0; };
-    auto band_c = [&](int row) { return row < ny ? C[row] : 0.0; };
-    auto beta = [&](int row) { return row < ny ? B[row] + lam : 1.0; };
+    const double* B = A + MR * P;
+    const double* C = B + MR * P;
+    auto band_a = [&](int row) { return A[row]; };
+    auto band_c = [&](int row) { return C[row]; };
+    auto beta = [&](int row) { return B[row] + lam; };
 
-    double Ah[MR], Ch[MR];  // d[] is overwritten by D^ for interior rows
+    double Ah[TP ? 1 : MR], Ch[TP ? 1 : MR];  // TP = false: d[] is overwritten by D^ for interior rows
     double Ae, Be, De;
-    {
+    double Pp = 0.0, Qq = 0.0, Rr = -1.0;  // x_{s+1} = Pp - Qq x_s - Rr x_e
+    if constexpr (TP) {
+        double Ak = band_a(s + 1);
+        double Bk = beta(s + 1);
+        double Dk = d[1];
+        double cprev = band_c(s + 1);
+        double pi = 1.0;  // product of -C^ over the rows eliminated so far
+#pragma unroll
+        for (int l = 2; l < MR; ++l) {
+            const double rinv = rcp64(Bk);
+            Pp += pi * (Dk * rinv);
+            Qq += pi * (Ak * rinv);
+            pi = -pi * (cprev * rinv);
+            const double f = band_a(s + l) * rinv;
+            Ak = -f * Ak;
+            Bk = beta(s + l) - f * cprev;
+            Dk = d[l] - f * Dk;
+            cprev = band_c(s + l);
+        }
+        Rr = -pi;
+        Ae = Ak;
+        Be = Bk;
+        De = Dk;
+    } else {
         double Ak = band_a(s + 1);
         double Bk = beta(s + 1);
         double Dk = d[1];
@@ -400,14 +447,13 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
         Ae = Ak;
         Be = Bk;
         De = Dk;
-    }
-    // upward pass: x_{s+1} = Pp - Qq x_s - Rr x_e
-    double Pp = 0.0, Qq = 0.0, Rr = -1.0;
+        // upward pass
 #pragma unroll
-    for (int l = MR - 2; l >= 1; --l) {
-        Pp = d[l] - Ch[l] * Pp;
-        Qq = Ah[l] - Ch[l] * Qq;
-        Rr = -Ch[l] * Rr;
+        for (int l = MR - 2; l >= 1; --l) {
+            Pp = d[l] - Ch[l] * Pp;
+            Qq = Ah[l] - Ch[l] * Qq;
+            Rr = -Ch[l] * Rr;
+        }
     }
     // row s: sa x_{s-1} + sb x_s + sc x_e = sd (sib = 1/sb)
     const double cs = band_c(s);
@@ -494,21 +540,58 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
     }
     const double xs = (sd - sa * xem - sc * xe) * sib;
     d[0] = xs;
-    double xn = xe;
+    if constexpr (TP) {
+        // rows s+1..e-1 with x_s and x_e as boundary data; d[] becomes the
+        // forward-eliminated right-hand side, then the solution.  The bands
+        // are read from LDS again rather than kept in registers across the
+        // reduced solve.
+        asm volatile("" ::: "memory");
+        if constexpr (MR > 2) {
+            double cp[MR];
 #pragma unroll
-    for (int l = MR - 2; l >= 1; --l) {
-        xn = d[l] - Ah[l] * xs - Ch[l] * xn;
-        d[l] = xn;
+            for (int l = 1; l < MR - 1; ++l) {
+                const double al = band_a(s + l);
+                const double cl = band_c(s + l);
+                double piv = beta(s + l), r = d[l];
+                if (l == 1) r -= al * xs;
+                else {
+                    piv -= al * cp[l - 1];
+                    r -= al * d[l - 1];
+                }
+                if (l == MR - 2) r -= cl * xe;
+                const double rinv = rcp64(piv);
+                cp[l] = cl * rinv;
+                d[l] = r * rinv;
+            }
+            double xn = d[MR - 2];
+#pragma unroll
+            for (int l = MR - 3; l >= 1; --l) {
+                xn = d[l] - cp[l] * xn;
+                d[l] = xn;
+            }
+        }
+    } else {
+        double xn = xe;
+#pragma unroll
+        for (int l = MR - 2; l >= 1; --l) {
+            xn = d[l] - Ah[l] * xs - Ch[l] * xn;
+            d[l] = xn;
+        }
     }
     d[MR - 1] = xe;
     if (valid) {
         const bool mean = msum != nullptr && mode == 0;
+        // addresses from one offset the compiler cannot relate to the loads'
+        // (otherwise it keeps every row index live across the solve)
+        size_t off = (size_t)s * plane + md;
+        int nrow = ny - s;
+        asm volatile("" : "+v"(off), "+v"(nrow));
+        T* q = S + off;
 #pragma unroll
         for (int r = 0; r < MR; ++r) {
-            const int row = s + r;
-            if (row < ny) {
-                if (mean) pbar[row] = d[r] * inv_nxnz;
-                S[(size_t)row * plane + md] = mean ? T(0) : (T)d[r];
+            if (r < nrow) {
+                if (mean) pbar[s + r] = d[r] * inv_nxnz;
+                q[(size_t)r * plane] = mean ? T(0) : (T)d[r];
             }
         }
     }
@@ -524,9 +607,9 @@ __device__ __forceinline__ long tri_group_mode(long grp, long ngroups, int M, in
     return grp * M + mi;
 }
 
-// Grid: tri_groups(plane, M) workgroups of M * P threads, 3 * ny doubles of
-// dynamic LDS.
-template <int MR, int P, int M, typename T = double, int OCC = 1>
+// Grid: tri_groups(plane, M) workgroups of M * P threads, 3 * MR * P doubles
+// of dynamic LDS.
+template <int MR, int P, int M, typename T = double, int OCC = 1, bool TP = false>
 __global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __restrict__ ax,
                                                           const double* __restrict__ az,
                                                           const double* __restrict__ bands, int nx, int ny,
@@ -536,19 +619,18 @@ __global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __
                                                           double inv_nxnz = 0.0) {
     constexpr int NT = M * P;
     __shared__ TriRed<P, M> red;
-    extern __shared__ double smb[];  // the group's band set, 3*ny values
+    extern __shared__ double smb[];  // the group's band set, 3 * MR * P values
     const int mi = (int)threadIdx.x % M;
     const int s = ((int)threadIdx.x / M) * MR;
     // adjacent mode groups share cache lines: keep them on one XCD (each has
     // its own L2; workgroups reach XCDs round-robin)
     const long grp = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const long mode = tri_group_mode(grp, (plane + M - 1) / M, M, mi);
-    const double* set = bands + (grp == 0 ? 3 * ny : 0);
-    for (int e = (int)threadIdx.x; e < 3 * ny; e += NT) smb[e] = set[e];
+    tri_stage_bands<MR, P>(smb, bands + (grp == 0 ? 3 * ny : 0), ny, NT);
     double d[MR];
     tri_load_rows<MR, P, M, T>(d, S, mode, plane, ny, s, msum);
     __syncthreads();  // bands staged
-    tri_group<MR, P, M, T>(d, S, ax, az, smb, nx, ny, plane, pcr_min, mode, msum, pbar, inv_nxnz, red);
+    tri_group<MR, P, M, T, TP>(d, S, ax, az, smb, nx, ny, plane, pcr_min, mode, msum, pbar, inv_nxnz, red);
 }
 
 }  // namespace dnsp

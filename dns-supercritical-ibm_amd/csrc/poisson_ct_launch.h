@@ -22,7 +22,10 @@ void ct_z_launch(int n, int variant, bool inv, float* S, int nx, int ny, const f
 // capped for 4 resident waves per SIMD; 1 = P 32, M 8; 2 = P 64, M 4;
 // 3 = P 64, M 8.
 constexpr int kPartChunks = 32;
-constexpr int kPartVariants = 4;
+constexpr int kPartVariants = 8;
+// tuner table: first slot of the x, z and tridiagonal variants of one form
+constexpr int kTuneSlot[3] = {0, 4, 8};
+constexpr int kTuneFormSlots = 16;
 int part_rows_for(int ny);  // rows per chunk at kPartChunks chunks (0: ny too large)
 template <typename T>
 void part_launch(int variant, T* S, const double* ax, const double* az, const double* bands, int nx, int nz,

@@ -310,7 +310,8 @@ int mixed_autotune(dnsp_poisson* h, const double* f, int reps, hipStream_t st, d
     int pick[3] = {h->mxvar, h->mzvar, h->mtvar};
     double best[3] = {1e30, 1e30, 1e30};
     for (int dir = 0; dir < 3 && rc == DNSP_OK; ++dir) {
-        for (int v = 0; v < 4 && rc == DNSP_OK; ++v) {
+        const int nvar = dir == 2 ? kPartVariants : kCtVariants;
+        for (int v = 0; v < nvar && rc == DNSP_OK; ++v) {
             auto pair = [&]() {
                 if (dir == 0) {
                     MxArgs a = base_args(h);
@@ -344,7 +345,7 @@ int mixed_autotune(dnsp_poisson* h, const double* f, int reps, hipStream_t st, d
             float ms = 0.f;
             (void)hipEventElapsedTime(&ms, e0, e1);
             const double per = (double)ms / reps;
-            if (times_ms) times_ms[dir * 4 + v] = per;
+            if (times_ms) times_ms[kTuneSlot[dir] + v] = per;
             if (per < best[dir]) {
                 best[dir] = per;
                 pick[dir] = v;

@@ -17,24 +17,24 @@ static int rows_for(int ny, int chunks) {
 
 int part_rows_for(int ny) { return rows_for(ny, kPartChunks); }
 
-template <int MR, int P, int M, typename T, int OCC>
+template <int MR, int P, int M, typename T, int OCC, bool TP>
 static void part_launch_mr(T* S, const double* ax, const double* az, const double* bands, int nx, int nz, int ny,
                            double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st) {
     const long plane = (long)nx * nz;
     const int ngroups = (int)((plane + M - 1) / M) + 1;  // plus the group of modes 1..M-1
-    const size_t lds = 3 * sizeof(double) * ny;
-    kt_tri_part<MR, P, M, T, OCC><<<ngroups, M * P, lds, st>>>(S, ax, az, bands, nx, ny, plane, pcr_min, msum, pbar,
+    const size_t lds = 3 * sizeof(double) * MR * P;
+    kt_tri_part<MR, P, M, T, OCC, TP><<<ngroups, M * P, lds, st>>>(S, ax, az, bands, nx, ny, plane, pcr_min, msum, pbar,
                                                                 inv_nxnz);
 }
 
 // P chunks per mode; rows per chunk from ny (at most 32 rows, ny <= 32 * P)
-template <int P, int M, typename T, int OCC>
+template <int P, int M, typename T, int OCC, bool TP>
 static void part_launch_pm(T* S, const double* ax, const double* az, const double* bands, int nx, int nz, int ny,
                            double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st) {
     switch (rows_for(ny, P)) {
 #define DNSP_CASE(MR)                                                                                         \
     case MR:                                                                                                  \
-        part_launch_mr<MR, P, M, T, OCC>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st);   \
+        part_launch_mr<MR, P, M, T, OCC, TP>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st);   \
         break;
         DNSP_PART_ROWS(DNSP_CASE)
 #undef DNSP_CASE
@@ -45,20 +45,18 @@ static void part_launch_pm(T* S, const double* ax, const double* az, const doubl
 template <typename T>
 void part_launch(int variant, T* S, const double* ax, const double* az, const double* bands, int nx, int nz,
                  int ny, double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st) {
+#define DNSP_PV(P, M, OCC, TP) part_launch_pm<P, M, T, OCC, TP>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st)
     switch (variant) {
-        case 0:
-            part_launch_pm<32, 16, T, 4>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st);
-            break;
-        case 1:
-            part_launch_pm<32, 8, T, 1>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st);
-            break;
-        case 2:
-            part_launch_pm<64, 4, T, 1>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st);
-            break;
-        default:
-            part_launch_pm<64, 8, T, 1>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st);
-            break;
+        case 0: DNSP_PV(32, 16, 4, false); break;
+        case 1: DNSP_PV(32, 8, 1, false); break;
+        case 2: DNSP_PV(64, 4, 1, false); break;
+        case 3: DNSP_PV(64, 8, 1, false); break;
+        case 4: DNSP_PV(32, 16, 4, true); break;
+        case 5: DNSP_PV(32, 8, 1, true); break;
+        case 6: DNSP_PV(64, 4, 1, true); break;
+        default: DNSP_PV(64, 8, 1, true); break;
     }
+#undef DNSP_PV
 }
 
 template void part_launch<double>(int, double*, const double*, const double*, const double*, int, int, int, double,

@@ -31,7 +31,7 @@ TRIDIAG_AUTO, TRIDIAG_SERIAL, TRIDIAG_PARTITION = 0, 1, 2
 
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 MAX_REFINEMENTS = 8
-TUNE_SLOTS = 24
+TUNE_SLOTS = 32
 
 STAGE_X_FWD, STAGE_Z_FWD, STAGE_TRIDIAG, STAGE_Z_BWD, STAGE_X_BWD = range(5)
 STAGE_NAMES = ("x_fwd", "z_fwd", "tridiag", "z_bwd", "x_bwd")
@@ -231,11 +231,11 @@ class PoissonSolver:
         wp = _vp(_device_ptr(work, self.numel)) if work is not None else _vp(None)
         _check(lib().dnsp_poisson_autotune(self._h, wp, int(reps), _vp(_stream_ptr(stream)),
                                            times.ctypes.data_as(_dp)))
-        out = {"x_ms": times[:4].tolist(), "z_ms": times[4:8].tolist(), "tri_ms": times[8:12].tolist(),
+        out = {"x_ms": times[:4].tolist(), "z_ms": times[4:8].tolist(), "tri_ms": times[8:16].tolist(),
                "picked": self.variants}
         if self.precision == PRECISION_MIXED:
-            out.update({"mixed_x_ms": times[12:16].tolist(), "mixed_z_ms": times[16:20].tolist(),
-                        "mixed_tri_ms": times[20:24].tolist(), "mixed_picked": self.mixed_variants})
+            out.update({"mixed_x_ms": times[16:20].tolist(), "mixed_z_ms": times[20:24].tolist(),
+                        "mixed_tri_ms": times[24:32].tolist(), "mixed_picked": self.mixed_variants})
         return out
 
     def solve_mixed(self, rhs, P, refinements=1, stream=None):
@@ -387,10 +387,10 @@ class PoissonBatch:
     def autotune(self, reps=3, stream=None):
         times = np.zeros(TUNE_SLOTS)
         _check(lib().dnsp_batch_autotune(self._h, int(reps), _vp(_stream_ptr(stream)), times.ctypes.data_as(_dp)))
-        out = {"x_ms": times[:4].tolist(), "z_ms": times[4:8].tolist(), "tri_ms": times[8:10].tolist()}
+        out = {"x_ms": times[:4].tolist(), "z_ms": times[4:8].tolist(), "tri_ms": times[8:12].tolist()}
         if self.precision == PRECISION_MIXED:
-            out.update({"mixed_x_ms": times[12:16].tolist(), "mixed_z_ms": times[16:20].tolist(),
-                        "mixed_tri_ms": times[20:22].tolist()})
+            out.update({"mixed_x_ms": times[16:20].tolist(), "mixed_z_ms": times[20:24].tolist(),
+                        "mixed_tri_ms": times[24:28].tolist()})
         out["picked"] = self.variants
         return out
 

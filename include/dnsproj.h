@@ -129,10 +129,13 @@ int dnsp_poisson_get_strategy(const dnsp_poisson* h, int* fft_ct_x, int* fft_ct_
 /* Kernel variants.  xvar, zvar (0..3): the compile-time-sized FFT kernels'
  * (complex elements per workgroup tile, threads) = (3072,256), (3072,512),
  * (1536,256), (6144,512); ignored where a length has no such kernels.
- * tvar (0..3): partitioned tridiagonal workgroups of (chunks per mode, modes)
+ * tvar (0..7): partitioned tridiagonal workgroups of (chunks per mode, modes)
  * = (32, 16) with registers capped for 4 resident waves per SIMD (0),
- * (32, 8) (1), (64, 4) (2), (64, 8) (3); rows per chunk follow from ny.  The
- * mixed-precision form has its own set.  No counterpart in the reference. */
+ * (32, 8) (1), (64, 4) (2), (64, 8) (3); rows per chunk follow from ny.
+ * 4..7: the same shapes with two-pass chunks (the chunk interior is solved
+ * again once its end values are known, instead of being kept in registers
+ * across the reduced solve).  The mixed-precision form has its own set.  No
+ * counterpart in the reference. */
 int dnsp_poisson_set_variants(dnsp_poisson* h, int xvar, int zvar, int tvar);
 int dnsp_poisson_get_variants(const dnsp_poisson* h, int* xvar, int* zvar, int* tvar);
 int dnsp_poisson_set_mixed_variants(dnsp_poisson* h, int xvar, int zvar, int tvar);
@@ -144,9 +147,10 @@ int dnsp_poisson_get_mixed_variants(const dnsp_poisson* h, int* xvar, int* zvar,
  * temporary allocation) and keeps the fastest in the handle; for a
  * DNSP_PRECISION_MIXED handle the same for the mixed-precision kernels.
  * times_ms (optional, DNSP_TUNE_SLOTS doubles) receives the mean ms per pair:
- * 4 x, 4 z and 4 tridiagonal variants of the FP64 form, then of the mixed
- * form; -1 where not applicable.  Synchronises `stream`. */
-#define DNSP_TUNE_SLOTS 24
+ * 4 x, 4 z and 8 tridiagonal variants of the FP64 form (slots 0..15), then
+ * the same of the mixed form (16..31); -1 where not applicable.  Synchronises
+ * `stream`. */
+#define DNSP_TUNE_SLOTS 32
 int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream, double* times_ms);
 
 /*
@@ -196,8 +200,8 @@ int dnsp_poisson_solve_refine(dnsp_poisson* h, const double* rhs, double* P, dou
  *   caller may capture them in a graph.
  *   dnsp_batch_autotune: times the whole batched solve per variant (x, z:
  *     0..3 as dnsp_poisson_set_variants; tridiagonal: 0 = 8-mode groups,
- *     1 = the same with registers capped for 4 waves per SIMD), keeps the
- *     fastest; overwrites the bound P arrays; times_ms as for
+ *     1 = the same with registers capped for 4 waves per SIMD, 2 and 3 =
+ *     0 and 1 with two-pass chunks), keeps the fastest; overwrites the bound P arrays; times_ms as for
  *     dnsp_poisson_autotune.  Synchronises.
  *   dnsp_batch_set_variants / get_variants: FP64 x, z, tridiagonal, then the
  *     mixed form's (6 ints).

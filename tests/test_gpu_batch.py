@@ -66,7 +66,7 @@ def bind(batch, cases):
     return f, P
 
 
-@pytest.mark.parametrize("tvar", [0, 1])
+@pytest.mark.parametrize("tvar", [0, 1, 2, 3])
 def test_batch_fp64_equals_single_and_oracle(cuda, tvar):
     say("batch fp64", tvar)
     cases = small_cases(1)
@@ -77,10 +77,11 @@ def test_batch_fp64_equals_single_and_oracle(cuda, tvar):
         b.run()
         for c, fi, Pi in zip(cases, f, P):
             s = D.PoissonSolver(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], c["bands"])
-            # same kernels, tile decomposition and 8-mode groups of 32 chunks as
-            # the batch (its register caps change no arithmetic)
+            # same kernels, tile decomposition and 8-mode groups of 32 chunks
+            # (one- or two-pass) as the batch (its register caps change no
+            # arithmetic)
             s.set_strategy(D.FFT_FIXED, D.TRIDIAG_PARTITION)
-            s.set_variants(xv, zv, 1)
+            s.set_variants(xv, zv, 1 if tvar < 2 else 5)
             Q = to_dev(np.zeros_like(c["rhs"]))
             s.solve(fi, Q)
             got = to_host(Pi)

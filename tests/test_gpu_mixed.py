@@ -141,7 +141,7 @@ def test_every_mixed_variant_and_the_tuner(cuda):
     s, bands = make_solver(c)
     f = to_dev(c["rhs"])
     for v in range(4):
-        for tv in range(4):
+        for tv in range(8):
             s.set_mixed_variants(v, (v + 2) % 4, tv)
             P = to_dev(np.zeros_like(c["rhs"]))
             s.solve_mixed(f, P, 1)

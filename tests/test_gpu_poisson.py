@@ -171,7 +171,7 @@ def test_every_kernel_variant_matches_oracle(cuda, nx, nz, ny):
     s, bands = make_solver(c)
     ref = O.solve(c["rhs"], c["inv_dx"], c["inv_dz"], bands)
     for v in range(4):
-        for tv in range(4):
+        for tv in range(8):
             s.set_variants(v, (v + 1) % 4, tv)
             t = to_dev(c["rhs"])
             s.run(t)

@@ -3,7 +3,12 @@
 chunks of MR rows, chunk elimination with fill-in towards both chunk ends,
 elimination of every chunk's start row (its own and the next chunk's) from
 its end row, the P-row reduced system in the end unknowns solved by parallel
-cyclic reduction or by a serial Thomas sweep, back-substitution.
+cyclic reduction or by a serial Thomas sweep, back-substitution.  The
+two-pass form keeps only the chunk's right-hand side through the reduced
+solve: the elimination accumulates x_{s+1} = Pp - Qq x_s - Rr x_e forwards,
+and once x_s and x_e are known the chunk interior is solved again as a
+tridiagonal system with those two values as boundary data.  Padding rows
+(past ny) are a = c = 0, b = -1, as the kernel stages them.
 
 Cyclic reduction can lose digits on nearly singular (low |lambda|) modes, so
 the kernel switches to the serial reduced solve below
@@ -33,10 +38,10 @@ def thomas(a, b, c, d):
     return dd
 
 
-def partitioned(a, b, c, d, MR, P, reduced):
+def partitioned(a, b, c, d, MR, P, reduced, two_pass=False):
     ny = len(b)
     n = MR * P
-    A, B, C, D = np.zeros(n), np.ones(n), np.zeros(n), np.zeros(n)
+    A, B, C, D = np.zeros(n), -np.ones(n), np.zeros(n), np.zeros(n)
     A[:ny], B[:ny], C[:ny], D[:ny] = a, b, c, d
     srow, erow, keep = [], [], []
     for p in range(P):
@@ -44,14 +49,20 @@ def partitioned(a, b, c, d, MR, P, reduced):
         dl = D[s:s + MR].copy()
         Ah, Ch = np.zeros(MR), np.zeros(MR)
         Ak, Bk, Dk, cprev = A[s + 1], B[s + 1], dl[1], C[s + 1]
+        Pp, Qq, Rr, pi = 0.0, 0.0, -1.0, 1.0
         for l in range(2, MR):
             r = 1.0 / Bk
-            dl[l - 1], Ah[l - 1], Ch[l - 1] = Dk * r, Ak * r, cprev * r
+            if two_pass:
+                Pp, Qq, pi = Pp + pi * (Dk * r), Qq + pi * (Ak * r), -pi * (cprev * r)
+            else:
+                dl[l - 1], Ah[l - 1], Ch[l - 1] = Dk * r, Ak * r, cprev * r
             f = A[s + l] * r
             Ak, Bk, Dk, cprev = -f * Ak, B[s + l] - f * cprev, dl[l] - f * Dk, C[s + l]
-        Pp, Qq, Rr = 0.0, 0.0, -1.0
-        for l in range(MR - 2, 0, -1):
-            Pp, Qq, Rr = dl[l] - Ch[l] * Pp, Ah[l] - Ch[l] * Qq, -Ch[l] * Rr
+        if two_pass:
+            Rr = -pi
+        else:
+            for l in range(MR - 2, 0, -1):
+                Pp, Qq, Rr = dl[l] - Ch[l] * Pp, Ah[l] - Ch[l] * Qq, -Ch[l] * Rr
         cs = C[s]
         sb = B[s] - cs * Qq
         # row s: sa x_{s-1} + sb x_s + sc x_e = sd;  row e: Ae x_s + Be x_e + ce x_{e+1} = De
@@ -100,6 +111,13 @@ def partitioned(a, b, c, d, MR, P, reduced):
         ym = y[p - 1] if p > 0 else 0.0
         xs, xe = (sd - sa * ym - sc * y[p]) * sib, y[p]
         x[s], x[s + MR - 1] = xs, xe
+        if two_pass:
+            if MR > 2:
+                rhs = dl[1:MR - 1].copy()
+                rhs[0] -= A[s + 1] * xs
+                rhs[-1] -= C[s + MR - 2] * xe
+                x[s + 1:s + MR - 1] = thomas(A[s + 1:s + MR - 1], B[s + 1:s + MR - 1], C[s + 1:s + MR - 1], rhs)
+            continue
         xn = xe
         for l in range(MR - 2, 0, -1):
             xn = dl[l] - Ah[l] * xs - Ch[l] * xn
@@ -107,9 +125,10 @@ def partitioned(a, b, c, d, MR, P, reduced):
     return x[:ny]
 
 
+@pytest.mark.parametrize("two_pass", [False, True])
 @pytest.mark.parametrize("chunks", [32, 64])
 @pytest.mark.parametrize("grid", ["config1", "config2"])
-def test_hybrid_reduced_solve_matches_serial_thomas(grid, chunks):
+def test_hybrid_reduced_solve_matches_serial_thomas(grid, chunks, two_pass):
     yu = W.config1()["yu"] if grid == "config1" else W.config2_grid()["yu"]
     ny = len(yu) - 1
     MR = max(2, -(-ny // chunks))
@@ -124,7 +143,7 @@ def test_hybrid_reduced_solve_matches_serial_thomas(grid, chunks):
         res = lambda x: np.abs(M @ x - d).max() / np.abs(d).max()
         r_serial = res(thomas(A, B + lam, C, d))
         reduced = "serial" if abs(lam) < pcr_min else "pcr"
-        r_hybrid = res(partitioned(A, B + lam, C, d, MR, chunks, reduced))
+        r_hybrid = res(partitioned(A, B + lam, C, d, MR, chunks, reduced, two_pass))
         assert r_hybrid <= 10 * r_serial + 2e-14, (grid, lam, r_hybrid, r_serial)
 
 
@@ -143,5 +162,6 @@ def test_partitioned_solution_matches_dense_solve():
         M = np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
         ref = np.linalg.solve(M, d)
         for reduced in ("pcr", "serial"):
-            x = partitioned(a, b, c, d, MR, P, reduced)
-            assert np.abs(x - ref).max() <= 1e-13 * np.abs(ref).max(), (ny, MR, P, reduced)
+            for two_pass in (False, True):
+                x = partitioned(a, b, c, d, MR, P, reduced, two_pass)
+                assert np.abs(x - ref).max() <= 1e-13 * np.abs(ref).max(), (ny, MR, P, reduced, two_pass)
