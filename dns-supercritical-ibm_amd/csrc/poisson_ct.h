@@ -178,42 +178,84 @@ __device__ __forceinline__ void zfwd_body(T* S, int nx, int ntc, const typename 
     const int k = tile / ntc;
     const int r0 = (tile - k * ntc) * W;  // first real row of the tile
     T* Sk = S + (size_t)k * nx * N;
-    T v[NL];
+    // even nx (and an aligned array): the two rows of a complex column are
+    // adjacent in memory, so each lane moves one (re, im) pair (the tile's
+    // start r0 is even)
+    constexpr int TOT2 = TA * N;
+    constexpr int NL2 = (TOT2 + NT - 1) / NT;
+    const bool pairs = (nx & 1) == 0 && ((size_t)S & (sizeof(V) - 1)) == 0;
+    if (pairs) {
+        V v[NL2];
 #pragma unroll
-    for (int q = 0; q < NL; ++q) {
-        const int e = (int)threadIdx.x + q * NT;
-        const int r = e % W;
-        const int j = e / W;
-        const bool ok = (TOT % NT == 0 || e < TOT) && r0 + r < nx;
-        v[q] = ok ? Sk[(size_t)j * nx + r0 + r] : T(0);
-    }
+        for (int q = 0; q < NL2; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            const int tt = e % TA;
+            const int j = e / TA;
+            const bool ok = (TOT2 % NT == 0 || e < TOT2) && r0 + 2 * tt < nx;
+            v[q] = Cx<T>::mk(T(0), T(0));
+            if (ok) v[q] = *reinterpret_cast<const V*>(Sk + (size_t)j * nx + r0 + 2 * tt);
+        }
 #pragma unroll
-    for (int q = 0; q < NL; ++q) {
-        const int e = (int)threadIdx.x + q * NT;
-        if (TOT % NT == 0 || e < TOT) {
+        for (int q = 0; q < NL2; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            if (TOT2 % NT == 0 || e < TOT2) sm[(e % TA) * LP + ct_pad(e / TA)] = v[q];
+        }
+    } else {
+        T v[NL];
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
             const int r = e % W;
             const int j = e / W;
-            smd[((r >> 1) * LP + ct_pad(j)) * 2 + (r & 1)] = v[q];
+            const bool ok = (TOT % NT == 0 || e < TOT) && r0 + r < nx;
+            v[q] = ok ? Sk[(size_t)j * nx + r0 + r] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            if (TOT % NT == 0 || e < TOT) {
+                const int r = e % W;
+                const int j = e / W;
+                smd[((r >> 1) * LP + ct_pad(j)) * 2 + (r & 1)] = v[q];
+            }
         }
     }
     __syncthreads();
     fft_ct<N, TA, NT, false>(sm, tw);
+    const T h = T(0.5);
+    if (pairs) {
 #pragma unroll
-    for (int q = 0; q < NL; ++q) {
-        const int e = (int)threadIdx.x + q * NT;
-        const int r = e % W;
-        const int b = e / W;
-        if ((TOT % NT == 0 || e < TOT) && r0 + r < nx) {
-            const int tt = r >> 1;
-            const bool lo = 2 * b <= N;
-            const int qq = lo ? b : N - b;
-            const V Z = sm[tt * LP + ct_pad(qq)];
-            const V Zm = sm[tt * LP + ct_pad(qq == 0 ? 0 : N - qq)];
-            const T h = T(0.5);
-            T val;
-            if ((r & 1) == 0) val = lo ? h * (Z.x + Zm.x) : h * (Z.y - Zm.y);
-            else              val = lo ? h * (Z.y + Zm.y) : h * (Zm.x - Z.x);
-            Sk[(size_t)b * nx + r0 + r] = val;
+        for (int q = 0; q < NL2; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            const int tt = e % TA;
+            const int b = e / TA;
+            if ((TOT2 % NT == 0 || e < TOT2) && r0 + 2 * tt < nx) {
+                const bool lo = 2 * b <= N;
+                const int qq = lo ? b : N - b;
+                const V Z = sm[tt * LP + ct_pad(qq)];
+                const V Zm = sm[tt * LP + ct_pad(qq == 0 ? 0 : N - qq)];
+                const T ve = lo ? h * (Z.x + Zm.x) : h * (Z.y - Zm.y);
+                const T vo = lo ? h * (Z.y + Zm.y) : h * (Zm.x - Z.x);
+                *reinterpret_cast<V*>(Sk + (size_t)b * nx + r0 + 2 * tt) = Cx<T>::mk(ve, vo);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            const int r = e % W;
+            const int b = e / W;
+            if ((TOT % NT == 0 || e < TOT) && r0 + r < nx) {
+                const int tt = r >> 1;
+                const bool lo = 2 * b <= N;
+                const int qq = lo ? b : N - b;
+                const V Z = sm[tt * LP + ct_pad(qq)];
+                const V Zm = sm[tt * LP + ct_pad(qq == 0 ? 0 : N - qq)];
+                T val;
+                if ((r & 1) == 0) val = lo ? h * (Z.x + Zm.x) : h * (Z.y - Zm.y);
+                else              val = lo ? h * (Z.y + Zm.y) : h * (Zm.x - Z.x);
+                Sk[(size_t)b * nx + r0 + r] = val;
+            }
         }
     }
     if (lsum != nullptr && r0 == 0 && threadIdx.x < 64) {
@@ -246,6 +288,8 @@ __device__ __forceinline__ void zbwd_body(T* S, int nx, int ntc, const typename 
     const int k = tile / ntc;
     const int r0 = (tile - k * ntc) * W;
     T* Sk = S + (size_t)k * nx * N;
+    // (element-wise loads here: measured faster than pairwise ones for this
+    // gather of rows q and N - q)
     T ur[NI], ui[NI], vr[NI], vi[NI];
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
@@ -274,14 +318,30 @@ __device__ __forceinline__ void zbwd_body(T* S, int nx, int ntc, const typename 
     }
     __syncthreads();
     fft_ct<N, TA, NT, true>(sm, tw);
-    const T* smd = reinterpret_cast<const T*>(sm);
+    // even nx (and an aligned array): the two rows of a complex column are
+    // adjacent, stored as one (re, im) pair
+    if ((nx & 1) == 0 && ((size_t)S & (2 * sizeof(T) - 1)) == 0) {
+        using V = typename Cx<T>::V;
+        constexpr int TOT2 = TA * N;
+        constexpr int NL2 = (TOT2 + NT - 1) / NT;
 #pragma unroll
-    for (int q = 0; q < NL; ++q) {
-        const int e = (int)threadIdx.x + q * NT;
-        const int r = e % W;
-        const int j = e / W;
-        if ((TOT % NT == 0 || e < TOT) && r0 + r < nx)
-            Sk[(size_t)j * nx + r0 + r] = smd[((r >> 1) * LP + ct_pad(j)) * 2 + (r & 1)];
+        for (int q = 0; q < NL2; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            const int tt = e % TA;
+            const int j = e / TA;
+            if ((TOT2 % NT == 0 || e < TOT2) && r0 + 2 * tt < nx)
+                *reinterpret_cast<V*>(Sk + (size_t)j * nx + r0 + 2 * tt) = sm[tt * LP + ct_pad(j)];
+        }
+    } else {
+        const T* smd = reinterpret_cast<const T*>(sm);
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            const int r = e % W;
+            const int j = e / W;
+            if ((TOT % NT == 0 || e < TOT) && r0 + r < nx)
+                Sk[(size_t)j * nx + r0 + r] = smd[((r >> 1) * LP + ct_pad(j)) * 2 + (r & 1)];
+        }
     }
 }
 
