@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "common.h"
+#include "fft_ct.h"
 
 #pragma clang fp contract(off)
 
@@ -43,6 +44,46 @@ constexpr int kThreads = 256;
 constexpr int kBX = 64, kBZ = 4;
 
 inline int blocks_for(long n) { return (int)((n + kThreads - 1) / kThreads); }
+
+// Block order of the point-wise and stencil kernels.  The (x, z) tiles of a
+// plane are split into bands of whole tile rows, one band per XCD (each has
+// its own L2 and receives every 8th workgroup), and an XCD walks its band
+// plane by plane: the blocks it holds at once then cover a few consecutive
+// planes of one band, so the k +- 1 planes and the halo rows its stencils
+// read are shared in its L2.  (Dispatching plane by plane spreads each
+// plane's tiles over all XCDs; the momentum kernel then read 177 B/pt from
+// beyond L2 against 80 algorithmic.)
+struct BandGrid {
+    int nbx, hb, nbands, ny;
+    __host__ __device__ int per_band() const { return nbx * hb * ny; }
+    __host__ __device__ int blocks() const { return per_band() * nbands; }
+};
+
+inline BandGrid band_grid(const Grid& g) {
+    BandGrid b;
+    b.nbx = (g.nx + kBX - 1) / kBX;
+    const int nbz = (g.nz + kBZ - 1) / kBZ;
+    b.hb = (nbz + 7) / 8;
+    b.nbands = (nbz + b.hb - 1) / b.hb;
+    b.ny = g.ny;
+    return b;
+}
+
+// This block's point (i, j, k); false for a point outside the grid (rows of
+// the last band past nz, columns past nx).
+__device__ __forceinline__ bool band_point(const BandGrid& b, const Grid& g, int& i, int& j, int& k) {
+    const int L = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int band = L / b.per_band();
+    const int r = L - band * b.per_band();
+    const int tb = b.nbx * b.hb;
+    k = r / tb;
+    const int t = r - k * tb;
+    const int bz = band * b.hb + t / b.nbx;
+    const int bx = t % b.nbx;
+    i = bx * kBX + (int)threadIdx.x;
+    j = bz * kBZ + (int)threadIdx.y;
+    return i < g.nx && j < g.nz;
+}
 
 // Coefficients of the y derivatives (device layout of dnsp_predictor_geom)
 struct Geom {
@@ -181,19 +222,17 @@ __global__ void k_div_finish(double* D, Grid g) {
 // One thread per interior point: the x and z momentum right-hand sides at
 // the U and W points of (i, j, k), the y one at the V point when
 // k >= kmin_Vhat; X_hat = X + dt (1.5 R - 0.5 R_old), R_old = R
-// (mod_cfd_subroutines.f90:147-354).  Grid (ceil(nx/64), ceil(nz/4), ny) of
-// 64 x 4 blocks: rows j-1..j+1 of a block share cache lines.
+// (mod_cfd_subroutines.f90:147-354).  Blocks of 64 x 4 points of one
+// plane in band order (BandGrid): rows j-1..j+1 of a block share cache lines.
 template <bool VARPROP>
 __global__ __launch_bounds__(kThreads) void k_momentum(
     const double* __restrict__ U, const double* __restrict__ V, const double* __restrict__ W,
     const double* __restrict__ mu, const double* __restrict__ rho, const double* __restrict__ dvn,
     const double* __restrict__ buoy, double* __restrict__ Ru_old, double* __restrict__ Rv_old,
     double* __restrict__ Rw_old, double* __restrict__ U_hat, double* __restrict__ V_hat, double* __restrict__ W_hat,
-    Grid g, Geom G, Phys p, int kmin_Vhat) {
-    const int i = (int)(blockIdx.x * kBX + threadIdx.x);
-    const int j = (int)(blockIdx.y * kBZ + threadIdx.y);
-    const int k = (int)blockIdx.z;
-    if (i >= g.nx || j >= g.nz) return;
+    Grid g, BandGrid bg, Geom G, Phys p, int kmin_Vhat) {
+    int i, j, k;
+    if (!band_point(bg, g, i, j, k)) return;
     const long c = g.at(i, j, k);
     const long sz = g.sz(), sy = g.sy();
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * k);
@@ -314,11 +353,9 @@ __global__ __launch_bounds__(kThreads) void k_temp_rhs(const double* __restrict_
                                                        const double* __restrict__ V, const double* __restrict__ W,
                                                        const double* __restrict__ rho, const double* __restrict__ cp,
                                                        const double* __restrict__ RHS, double* __restrict__ Rt_now,
-                                                       Grid g, Geom G, Phys p) {
-    const int i = (int)(blockIdx.x * kBX + threadIdx.x);
-    const int j = (int)(blockIdx.y * kBZ + threadIdx.y);
-    const int k = (int)blockIdx.z;
-    if (i >= g.nx || j >= g.nz) return;
+                                                       Grid g, BandGrid bg, Geom G, Phys p) {
+    int i, j, k;
+    if (!band_point(bg, g, i, j, k)) return;
     const long c = g.at(i, j, k);
     const long sz = g.sz(), sy = g.sy();
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * k);
@@ -344,11 +381,9 @@ __global__ __launch_bounds__(kThreads) void k_temp_rhs(const double* __restrict_
 }
 
 __global__ void k_temp_update(double* __restrict__ T, const double* __restrict__ Rt_now, double* __restrict__ Rt_old,
-                              Grid g, double dt) {
-    const int i = (int)(blockIdx.x * kBX + threadIdx.x);
-    const int j = (int)(blockIdx.y * kBZ + threadIdx.y);
-    const int k = (int)blockIdx.z;
-    if (i >= g.nx || j >= g.nz) return;
+                              Grid g, BandGrid bg, double dt) {
+    int i, j, k;
+    if (!band_point(bg, g, i, j, k)) return;
     const long c = g.at(i, j, k);
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * k);
     const double rt = Rt_now[o];
@@ -460,11 +495,9 @@ __global__ __launch_bounds__(kThreads) void k_rhs_energy(const double* __restric
                                                          const double* __restrict__ divjac,
                                                          double* __restrict__ RHS, double* __restrict__ dvn,
                                                          const int* __restrict__ bot, const int* __restrict__ top,
-                                                         Grid g, Geom G, Phys p) {
-    const int i = (int)(blockIdx.x * kBX + threadIdx.x);
-    const int j = (int)(blockIdx.y * kBZ + threadIdx.y);
-    const int kk = (int)blockIdx.z;
-    if (i >= g.nx || j >= g.nz) return;
+                                                         Grid g, BandGrid bg, Geom G, Phys p) {
+    int i, j, kk;
+    if (!band_point(bg, g, i, j, kk)) return;
     const long c = g.at(i, j, kk);
     const long sz = g.sz(), sy = g.sy();
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * kk);
@@ -816,7 +849,8 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     const double* wl = pr->walls;
     const long per_plane = 2L * (g.nz + 2) + 2L * g.nx;
     const long nper = 9 * g.sy() + 4 * per_plane * g.ny;
-    const dim3 grid3((unsigned)((g.nx + kBX - 1) / kBX), (unsigned)((g.nz + kBZ - 1) / kBZ), (unsigned)g.ny);
+    const BandGrid bg = band_grid(g);
+    const dim3 grid3((unsigned)bg.blocks());
     const dim3 blk3(kBX, kBZ, 1);
     const bool vp = !pr->incompressible;
     const bool rough = pr->d_bot || pr->ibm[0] || pr->ibm[1] || pr->ibm[2] || pr->ibm[3] || pr->ibm[4];
@@ -846,20 +880,20 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     if (vp)
         k_momentum<true><<<grid3, blk3, 0, st>>>(U, V, W, mu, rho, dvn, buoy, pr->f[DNSP_PRED_RU_OLD],
                                                      pr->f[DNSP_PRED_RV_OLD], pr->f[DNSP_PRED_RW_OLD], Uh, Vh, Wh, g,
-                                                     pr->G, pr->p, pr->kmin_Vhat);
+                                                     bg, pr->G, pr->p, pr->kmin_Vhat);
     else
         k_momentum<false><<<grid3, blk3, 0, st>>>(U, V, W, nullptr, nullptr, nullptr, nullptr,
                                                       pr->f[DNSP_PRED_RU_OLD], pr->f[DNSP_PRED_RV_OLD],
-                                                      pr->f[DNSP_PRED_RW_OLD], Uh, Vh, Wh, g, pr->G, pr->p,
+                                                      pr->f[DNSP_PRED_RW_OLD], Uh, Vh, Wh, g, bg, pr->G, pr->p,
                                                       pr->kmin_Vhat);
     // 4. temperature
     if (vp)
         k_temp_rhs<true><<<grid3, blk3, 0, st>>>(T, U, V, W, rho, pr->f[DNSP_PRED_CP],
-                                                     pr->f[DNSP_PRED_RHS_ENERGY], pr->d_rt_now, g, pr->G, pr->p);
+                                                     pr->f[DNSP_PRED_RHS_ENERGY], pr->d_rt_now, g, bg, pr->G, pr->p);
     else
-        k_temp_rhs<false><<<grid3, blk3, 0, st>>>(T, U, V, W, nullptr, nullptr, nullptr, pr->d_rt_now, g, pr->G,
+        k_temp_rhs<false><<<grid3, blk3, 0, st>>>(T, U, V, W, nullptr, nullptr, nullptr, pr->d_rt_now, g, bg, pr->G,
                                                       pr->p);
-    k_temp_update<<<grid3, blk3, 0, st>>>(T, pr->d_rt_now, pr->f[DNSP_PRED_RT_OLD], g, pr->p.dt);
+    k_temp_update<<<grid3, blk3, 0, st>>>(T, pr->d_rt_now, pr->f[DNSP_PRED_RT_OLD], g, bg, pr->p.dt);
     // 5. rough walls: columns of the hats and T (ibm_part_2, mod_cfd_ibm.f90:47-141)
     if (rough && pr->d_bot) {
         double* H4[4] = {Uh, Vh, Wh, T};
@@ -889,8 +923,8 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
             if (rc) return rc;
         }
         k_rhs_energy<<<grid3, blk3, 0, st>>>(T, pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_DIVJAC],
-                                                 pr->f[DNSP_PRED_RHS_ENERGY], dvn, pr->d_bot, pr->d_top, g, pr->G,
-                                                 pr->p);
+                                                 pr->f[DNSP_PRED_RHS_ENERGY], dvn, pr->d_bot, pr->d_top, g, bg,
+                                                 pr->G, pr->p);
         k_div_finish<<<blocks_for(2 * g.sy() + per_plane * g.ny), kThreads, 0, st>>>(dvn, g);
     }
     DNSP_LAUNCH_CHECK();
