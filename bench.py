@@ -101,6 +101,7 @@ def bench_step(torch, D, W):
     outs = {}
     for name, prec, ref in (("fp64", 0, -1), ("mixed", D.PRECISION_MIXED, 1)):
         s = DP.ProjectionStep.from_case(c, precision=prec)
+        s.autotune(reps=3)
         s.load(c["fields"])
         st = torch.cuda.Stream()
         s.capture(ref, stream=st)
@@ -149,6 +150,7 @@ def bench_timestep(torch, D, W):
     res = {}
     for name, prec, ref in (("fp64", 0, -1), ("mixed", D.PRECISION_MIXED, 1)):
         s = DP.ProjectionStep.from_case(c, precision=prec)
+        s.autotune(reps=3)
         s.load(f)
         pr = PR.Predictor(s, yu, 6e-5, 0.08, [0, 0, 0, 317.0, 0, 0, 0, 298.0], Sf_x=1.0, Gb=(0.0, 1.0, 0.0))
         for q, segs in tab["laws"].items():

@@ -20,7 +20,7 @@
 #include <vector>
 
 #include "common.h"
-#include "fft_ct.h"
+#include "stencil_grid.h"
 
 #pragma clang fp contract(off)
 
@@ -40,50 +40,9 @@ struct Grid {
 };
 
 constexpr int kThreads = 256;
-// point-wise kernels: 64 x 4 (x, z) blocks, one plane per grid row in z
-constexpr int kBX = 64, kBZ = 4;
 
 inline int blocks_for(long n) { return (int)((n + kThreads - 1) / kThreads); }
 
-// Block order of the point-wise and stencil kernels.  The (x, z) tiles of a
-// plane are split into bands of whole tile rows, one band per XCD (each has
-// its own L2 and receives every 8th workgroup), and an XCD walks its band
-// plane by plane: the blocks it holds at once then cover a few consecutive
-// planes of one band, so the k +- 1 planes and the halo rows its stencils
-// read are shared in its L2.  (Dispatching plane by plane spreads each
-// plane's tiles over all XCDs; the momentum kernel then read 177 B/pt from
-// beyond L2 against 80 algorithmic.)
-struct BandGrid {
-    int nbx, hb, nbands, ny;
-    __host__ __device__ int per_band() const { return nbx * hb * ny; }
-    __host__ __device__ int blocks() const { return per_band() * nbands; }
-};
-
-inline BandGrid band_grid(const Grid& g) {
-    BandGrid b;
-    b.nbx = (g.nx + kBX - 1) / kBX;
-    const int nbz = (g.nz + kBZ - 1) / kBZ;
-    b.hb = (nbz + 7) / 8;
-    b.nbands = (nbz + b.hb - 1) / b.hb;
-    b.ny = g.ny;
-    return b;
-}
-
-// This block's point (i, j, k); false for a point outside the grid (rows of
-// the last band past nz, columns past nx).
-__device__ __forceinline__ bool band_point(const BandGrid& b, const Grid& g, int& i, int& j, int& k) {
-    const int L = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    const int band = L / b.per_band();
-    const int r = L - band * b.per_band();
-    const int tb = b.nbx * b.hb;
-    k = r / tb;
-    const int t = r - k * tb;
-    const int bz = band * b.hb + t / b.nbx;
-    const int bx = t % b.nbx;
-    i = bx * kBX + (int)threadIdx.x;
-    j = bz * kBZ + (int)threadIdx.y;
-    return i < g.nx && j < g.nz;
-}
 
 // Coefficients of the y derivatives (device layout of dnsp_predictor_geom)
 struct Geom {
@@ -223,7 +182,8 @@ __global__ void k_div_finish(double* D, Grid g) {
 // the U and W points of (i, j, k), the y one at the V point when
 // k >= kmin_Vhat; X_hat = X + dt (1.5 R - 0.5 R_old), R_old = R
 // (mod_cfd_subroutines.f90:147-354).  Blocks of 64 x 4 points of one
-// plane in band order (BandGrid): rows j-1..j+1 of a block share cache lines.
+// plane in band order (stencil_grid.h): rows j-1..j+1 of a block share
+// cache lines.
 template <bool VARPROP>
 __global__ __launch_bounds__(kThreads) void k_momentum(
     const double* __restrict__ U, const double* __restrict__ V, const double* __restrict__ W,
@@ -232,7 +192,7 @@ __global__ __launch_bounds__(kThreads) void k_momentum(
     double* __restrict__ Rw_old, double* __restrict__ U_hat, double* __restrict__ V_hat, double* __restrict__ W_hat,
     Grid g, BandGrid bg, Geom G, Phys p, int kmin_Vhat) {
     int i, j, k;
-    if (!band_point(bg, g, i, j, k)) return;
+    if (!band_point(bg, i, j, k)) return;
     const long c = g.at(i, j, k);
     const long sz = g.sz(), sy = g.sy();
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * k);
@@ -355,7 +315,7 @@ __global__ __launch_bounds__(kThreads) void k_temp_rhs(const double* __restrict_
                                                        const double* __restrict__ RHS, double* __restrict__ Rt_now,
                                                        Grid g, BandGrid bg, Geom G, Phys p) {
     int i, j, k;
-    if (!band_point(bg, g, i, j, k)) return;
+    if (!band_point(bg, i, j, k)) return;
     const long c = g.at(i, j, k);
     const long sz = g.sz(), sy = g.sy();
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * k);
@@ -383,7 +343,7 @@ __global__ __launch_bounds__(kThreads) void k_temp_rhs(const double* __restrict_
 __global__ void k_temp_update(double* __restrict__ T, const double* __restrict__ Rt_now, double* __restrict__ Rt_old,
                               Grid g, BandGrid bg, double dt) {
     int i, j, k;
-    if (!band_point(bg, g, i, j, k)) return;
+    if (!band_point(bg, i, j, k)) return;
     const long c = g.at(i, j, k);
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * k);
     const double rt = Rt_now[o];
@@ -497,7 +457,7 @@ __global__ __launch_bounds__(kThreads) void k_rhs_energy(const double* __restric
                                                          const int* __restrict__ bot, const int* __restrict__ top,
                                                          Grid g, BandGrid bg, Geom G, Phys p) {
     int i, j, kk;
-    if (!band_point(bg, g, i, j, kk)) return;
+    if (!band_point(bg, i, j, kk)) return;
     const long c = g.at(i, j, kk);
     const long sz = g.sz(), sy = g.sy();
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * kk);
@@ -849,9 +809,9 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     const double* wl = pr->walls;
     const long per_plane = 2L * (g.nz + 2) + 2L * g.nx;
     const long nper = 9 * g.sy() + 4 * per_plane * g.ny;
-    const BandGrid bg = band_grid(g);
+    const BandGrid bg = band_grid(g.nx, g.nz, g.ny);
     const dim3 grid3((unsigned)bg.blocks());
-    const dim3 blk3(kBX, kBZ, 1);
+    const dim3 blk3 = band_block();
     const bool vp = !pr->incompressible;
     const bool rough = pr->d_bot || pr->ibm[0] || pr->ibm[1] || pr->ibm[2] || pr->ibm[3] || pr->ibm[4];
     void* vs = stream;

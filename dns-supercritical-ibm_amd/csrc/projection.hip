@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "common.h"
+#include "stencil_grid.h"
 
 namespace dnsp {
 
@@ -80,14 +81,11 @@ __global__ void k_neumann_y(double* A, Halo h) {
 
 __global__ void k_rhs_incomp(double* P, const double* __restrict__ Uh, const double* __restrict__ Wh,
                              const double* __restrict__ Vh, const double* __restrict__ inv_dy_p, Halo h,
-                             double inv_dtmax, double inv_dx, double inv_dz) {
+                             BandGrid bg, double inv_dtmax, double inv_dx, double inv_dz) {
 #pragma clang fp contract(off)
-    const long n = (long)h.nx * h.nz * h.ny;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
-    const int i = (int)(idx % h.nx);
-    const int j = (int)((idx / h.nx) % h.nz);
-    const int k = (int)(idx / ((long)h.nx * h.nz));
+    int i, j, k;
+    if (!band_point(bg, i, j, k)) return;
+    const long idx = (long)i + (long)h.nx * ((long)j + (long)h.nz * k);
     const long c = h.at(i, j, k);
     P[idx] = inv_dtmax * (((Uh[c + 1] - Uh[c]) * inv_dx + (Wh[c + h.sz()] - Wh[c]) * inv_dz) +
                           (Vh[c + h.sy()] - Vh[c]) * inv_dy_p[k]);
@@ -103,15 +101,12 @@ __global__ void k_rhs_varprop(double* P, const double* __restrict__ Uh, const do
                               const double* __restrict__ Po, const double* __restrict__ rho,
                               const double* __restrict__ div_now, const double* __restrict__ inv_dy_p,
                               const double* __restrict__ inv_dy_p_pad, const double* __restrict__ rho_min_p,
-                              Halo h, double inv_dtmax, double inv_dx, double inv_dz, double inv_dx2,
+                              Halo h, BandGrid bg, double inv_dtmax, double inv_dx, double inv_dz, double inv_dx2,
                               double inv_dz2) {
 #pragma clang fp contract(off)
-    const long n = (long)h.nx * h.nz * h.ny;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
-    const int i = (int)(idx % h.nx);
-    const int j = (int)((idx / h.nx) % h.nz);
-    const int k = (int)(idx / ((long)h.nx * h.nz));
+    int i, j, k;
+    if (!band_point(bg, i, j, k)) return;
+    const long idx = (long)i + (long)h.nx * ((long)j + (long)h.nz * k);
     const long c = h.at(i, j, k);
     const long sx = 1, sz = h.sz(), sy = h.sy();
     const double rho_min = *rho_min_p;
@@ -153,16 +148,19 @@ __global__ void k_min_init(double* out) {
         *reinterpret_cast<unsigned long long*>(out) = 0xffffffffffffffffull;
 }
 
+// Workgroups stride over the interior x rows (row -> (j, k) by one uniform
+// division); the threads of a workgroup over the row.
 __global__ void k_min_interior(const double* __restrict__ A, Halo h, double* out) {
     __shared__ unsigned long long red[kStencilThreads];
-    const long n = (long)h.nx * h.nz * h.ny;
+    const int rows = h.nz * h.ny;
     unsigned long long best = 0xffffffffffffffffull;
-    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(idx % h.nx);
-        const int j = (int)((idx / h.nx) % h.nz);
-        const int k = (int)(idx / ((long)h.nx * h.nz));
-        const unsigned long long key = dkey(A[h.at(i, j, k)]);
-        best = key < best ? key : best;
+    for (int row = (int)blockIdx.x; row < rows; row += (int)gridDim.x) {
+        const int k = row / h.nz, j = row - k * h.nz;
+        const double* a = A + h.at(0, j, k);
+        for (int i = (int)threadIdx.x; i < h.nx; i += kStencilThreads) {
+            const unsigned long long key = dkey(a[i]);
+            best = key < best ? key : best;
+        }
     }
     red[threadIdx.x] = best;
     __syncthreads();
@@ -191,15 +189,12 @@ __device__ __forceinline__ long pidx(const Halo& h, int i, int j, int k) {
 __global__ void k_update_incomp(const double* __restrict__ P, double* U, double* V, double* W,
                                 const double* __restrict__ Uh, const double* __restrict__ Vh,
                                 const double* __restrict__ Wh, const double* __restrict__ hi,
-                                const double* __restrict__ lo, Halo h, int kmin_Vhat, double dtmax, double inv_dx,
-                                double inv_dz) {
+                                const double* __restrict__ lo, Halo h, BandGrid bg, int kmin_Vhat, double dtmax,
+                                double inv_dx, double inv_dz) {
 #pragma clang fp contract(off)
-    const long n = (long)h.nx * h.nz * h.ny;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
-    const int i = (int)(idx % h.nx);
-    const int j = (int)((idx / h.nx) % h.nz);
-    const int k = (int)(idx / ((long)h.nx * h.nz));
+    int i, j, k;
+    if (!band_point(bg, i, j, k)) return;
+    const long idx = (long)i + (long)h.nx * ((long)j + (long)h.nz * k);
     const long c = h.at(i, j, k);
     const double p = P[idx];
     U[c] = Uh[c] - dtmax * (p - P[pidx(h, i - 1, j, k)]) * inv_dx;
@@ -212,15 +207,12 @@ __global__ void k_update_varprop(const double* __restrict__ P, double* U, double
                                  const double* __restrict__ Wh, const double* __restrict__ Pn,
                                  const double* __restrict__ Po, const double* __restrict__ rho,
                                  const double* __restrict__ hi, const double* __restrict__ lo,
-                                 const double* __restrict__ rho_min_p, Halo h, int kmin_Vhat, double dtmax,
-                                 double inv_dx, double inv_dz) {
+                                 const double* __restrict__ rho_min_p, Halo h, BandGrid bg, int kmin_Vhat,
+                                 double dtmax, double inv_dx, double inv_dz) {
 #pragma clang fp contract(off)
-    const long n = (long)h.nx * h.nz * h.ny;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
-    const int i = (int)(idx % h.nx);
-    const int j = (int)((idx / h.nx) % h.nz);
-    const int k = (int)(idx / ((long)h.nx * h.nz));
+    int i, j, k;
+    if (!band_point(bg, i, j, k)) return;
+    const long idx = (long)i + (long)h.nx * ((long)j + (long)h.nz * k);
     const long c = h.at(i, j, k);
     const double inv_rho_min = 1.0 / *rho_min_p;
     const double p = P[idx];
@@ -244,12 +236,11 @@ __global__ void k_update_varprop(const double* __restrict__ P, double* U, double
 
 // P_per over the whole padded array, and (varprop) P_old <- P_now, P_now <- P_per
 __global__ void k_pper_rotate(const double* __restrict__ P, double* P_per, double* Pn, double* Po, Halo h,
-                              int rotate) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= h.total()) return;
-    const int i = (int)(idx % (h.nx + 2)) - 1;
-    const int j = (int)((idx / (h.nx + 2)) % (h.nz + 2)) - 1;
-    const int k = (int)(idx / h.sy()) - 1;
+                              BandGrid bg, int rotate) {
+    int ip, jp, kp;  // padded coordinates (bg spans the padded array)
+    if (!band_point(bg, ip, jp, kp)) return;
+    const long idx = (long)ip + (long)(h.nx + 2) * ((long)jp + (long)(h.nz + 2) * kp);
+    const int i = ip - 1, j = jp - 1, k = kp - 1;
     const double v = P[pidx(h, i, j, k)];
     if (P_per) P_per[idx] = v;
     if (rotate) {
@@ -355,9 +346,9 @@ int dnsp_build_p_rhs(double* P, const double* U_hat, const double* W_hat, const 
     clear_error();
     DNSP_ARG_CHECK(P && U_hat && W_hat && V_hat && inv_dy_p && halo_ok(nx, nz, ny), "invalid arguments");
     Halo h{nx, nz, ny};
-    const long n = (long)nx * nz * ny;
-    k_rhs_incomp<<<blocks_for(n), kStencilThreads, 0, as_stream(stream)>>>(P, U_hat, W_hat, V_hat, inv_dy_p, h,
-                                                                          inv_dtmax, inv_dx, inv_dz);
+    const BandGrid bg = band_grid(nx, nz, ny);
+    k_rhs_incomp<<<bg.blocks(), band_block(), 0, as_stream(stream)>>>(P, U_hat, W_hat, V_hat, inv_dy_p, h, bg,
+                                                                      inv_dtmax, inv_dx, inv_dz);
     DNSP_LAUNCH_CHECK();
     return DNSP_OK;
 }
@@ -372,9 +363,9 @@ int dnsp_build_p_rhs_varprop(double* P, const double* U_hat, const double* W_hat
                        rho_min && halo_ok(nx, nz, ny),
                    "invalid arguments");
     Halo h{nx, nz, ny};
-    const long n = (long)nx * nz * ny;
-    k_rhs_varprop<<<blocks_for(n), kStencilThreads, 0, as_stream(stream)>>>(
-        P, U_hat, W_hat, V_hat, P_now, P_old, rho, div_now, inv_dy_p, inv_dy_p_pad, rho_min, h, inv_dtmax, inv_dx,
+    const BandGrid bg = band_grid(nx, nz, ny);
+    k_rhs_varprop<<<bg.blocks(), band_block(), 0, as_stream(stream)>>>(
+        P, U_hat, W_hat, V_hat, P_now, P_old, rho, div_now, inv_dy_p, inv_dy_p_pad, rho_min, h, bg, inv_dtmax, inv_dx,
         inv_dz, inv_dx2, inv_dz2);
     DNSP_LAUNCH_CHECK();
     return DNSP_OK;
@@ -385,8 +376,7 @@ int dnsp_rho_min(const double* rho, int nx, int nz, int ny, double* out, void* s
     DNSP_ARG_CHECK(rho && out && halo_ok(nx, nz, ny), "invalid arguments");
     Halo h{nx, nz, ny};
     hipStream_t st = as_stream(stream);
-    const long n = (long)nx * nz * ny;
-    const int nb = std::min(2048, blocks_for(n));
+    const int nb = std::min(2048, nz * ny);
     k_min_init<<<1, 1, 0, st>>>(out);
     k_min_interior<<<nb, kStencilThreads, 0, st>>>(rho, h, out);
     k_min_decode<<<1, 1, 0, st>>>(out);
@@ -408,18 +398,19 @@ int dnsp_update_vars(const double* P, double* P_per, double* P_now, double* P_ol
                    "variable-density correction needs P_now, P_old, rho and rho_min");
     Halo h{nx, nz, ny};
     hipStream_t st = as_stream(stream);
-    const long n = (long)nx * nz * ny;
+    const BandGrid bg = band_grid(nx, nz, ny);
     if (use_incompressible)
-        k_update_incomp<<<blocks_for(n), kStencilThreads, 0, st>>>(P, U, V, W, U_hat, V_hat, W_hat, dPdy_ddy_hi,
-                                                                  dPdy_ddy_lo, h, kmin_Vhat, dtmax, inv_dx, inv_dz);
+        k_update_incomp<<<bg.blocks(), band_block(), 0, st>>>(P, U, V, W, U_hat, V_hat, W_hat, dPdy_ddy_hi,
+                                                              dPdy_ddy_lo, h, bg, kmin_Vhat, dtmax, inv_dx, inv_dz);
     else
-        k_update_varprop<<<blocks_for(n), kStencilThreads, 0, st>>>(P, U, V, W, U_hat, V_hat, W_hat, P_now, P_old,
-                                                                   rho, dPdy_ddy_hi, dPdy_ddy_lo, rho_min, h,
-                                                                   kmin_Vhat, dtmax, inv_dx, inv_dz);
+        k_update_varprop<<<bg.blocks(), band_block(), 0, st>>>(P, U, V, W, U_hat, V_hat, W_hat, P_now, P_old, rho,
+                                                               dPdy_ddy_hi, dPdy_ddy_lo, rho_min, h, bg, kmin_Vhat,
+                                                               dtmax, inv_dx, inv_dz);
     DNSP_LAUNCH_CHECK();
     if (P_per || !use_incompressible) {
-        k_pper_rotate<<<blocks_for(h.total()), kStencilThreads, 0, st>>>(P, P_per, P_now, P_old, h,
-                                                                        use_incompressible ? 0 : 1);
+        const BandGrid bp = band_grid(nx + 2, nz + 2, ny + 2);
+        k_pper_rotate<<<bp.blocks(), band_block(), 0, st>>>(P, P_per, P_now, P_old, h, bp,
+                                                            use_incompressible ? 0 : 1);
         DNSP_LAUNCH_CHECK();
     }
     const long nfin = 7 * h.sy() + 3 * (2L * (nz + 2) + 2L * nx) * ny;

@@ -333,6 +333,13 @@ class ProjectionStep:
         _check(_lib().dnsp_step_poisson(self._h, ctypes.byref(v)))
         return v
 
+    def autotune(self, reps=5, stream=None):
+        """Times the kernel variants of the step's Poisson solver (FP64 and,
+        for a mixed-precision step, mixed) on a temporary array and keeps the
+        fastest (dnsp_poisson_autotune); field contents are untouched."""
+        _check(lib().dnsp_poisson_autotune(self.poisson_handle(), _vp(None), int(reps), _vp(_stream_ptr(stream)),
+                                           ctypes.cast(None, _dp)))
+
     def enqueue(self, refinements=-1, stream=None):
         _check(_lib().dnsp_step_enqueue(self._h, int(refinements), _vp(_stream_ptr(stream))))
 

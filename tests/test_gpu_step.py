@@ -103,6 +103,28 @@ def test_step_fp64_matches_oracle_chain(cuda, nx, nz, ny):
     s.close()
 
 
+def test_step_autotune_keeps_oracle_parity(cuda):
+    """Tuning the step's solver (other kernel variants) leaves the fields and
+    the parity with the oracle chain as they were."""
+    say("step autotune")
+    c = W.projection_case(96, 64, 40, seed=5)
+    ref = oracle_step(c)
+    s = DP.ProjectionStep(c["nx"], c["nz"], c["ny"], c["yu"], c["inv_dx"], c["inv_dz"], c["dtmax"])
+    s.set_walls(c["walls"])
+    if c.get("ibm") is not None:
+        s.set_ibm(DP.IbmForcingSet(c["nx"], c["nz"], c["ny"], c["ibm"]))
+        s.set_columns(c["bot"], c["top"])
+    s.load(c["fields"])
+    before = {k: host(s.read(k)) for k in ("U", "P_now")}
+    s.autotune(reps=2)
+    for k, v in before.items():
+        assert np.array_equal(host(s.read(k)), v), k
+    s.enqueue(-1)
+    for k in OUT:
+        assert rel(host(s.read(k)), ref[k]) <= 1e-11, k
+    s.close()
+
+
 def test_step_incompressible_matches_oracle_chain(cuda):
     say("step incompressible")
     c = W.projection_case(64, 48, 32, seed=5, k_rough=0.0)
