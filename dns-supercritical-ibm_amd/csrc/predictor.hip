@@ -178,28 +178,20 @@ __global__ void k_div_finish(double* D, Grid g) {
 
 // --- momentum predictor ---------------------------------------------------------
 
-// One thread per interior point: the x and z momentum right-hand sides at
-// the U and W points of (i, j, k), the y one at the V point when
-// k >= kmin_Vhat; X_hat = X + dt (1.5 R - 0.5 R_old), R_old = R
-// (mod_cfd_subroutines.f90:147-354).  Blocks of 64 x 4 points of one
-// plane in band order (stencil_grid.h): rows j-1..j+1 of a block share
-// cache lines.
-template <bool VARPROP>
-__global__ __launch_bounds__(kThreads) void k_momentum(
-    const double* __restrict__ U, const double* __restrict__ V, const double* __restrict__ W,
-    const double* __restrict__ mu, const double* __restrict__ rho, const double* __restrict__ dvn,
-    const double* __restrict__ buoy, double* __restrict__ Ru_old, double* __restrict__ Rv_old,
-    double* __restrict__ Rw_old, double* __restrict__ U_hat, double* __restrict__ V_hat, double* __restrict__ W_hat,
-    Grid g, BandGrid bg, Geom G, Phys p, int kmin_Vhat) {
-    int i, j, k;
-    if (!band_point(bg, i, j, k)) return;
+// One interior point: the x and z momentum right-hand sides at the U and W
+// points of (i, j, k), the y one at the V point when k >= kmin_Vhat;
+// X_hat = X + dt (1.5 R - 0.5 R_old), R_old = R
+// (mod_cfd_subroutines.f90:147-354).  u, v, w, m(di, dj, dk) return U, V,
+// W, mu at (i + di, j + dj, k + dk).
+template <bool VARPROP, class AU, class AV, class AW, class AM>
+__device__ __forceinline__ void momentum_point(
+    const AU& u, const AV& v, const AW& w, const AM& m, int i, int j, int k, const double* __restrict__ rho,
+    const double* __restrict__ dvn, const double* __restrict__ buoy, double* __restrict__ Ru_old,
+    double* __restrict__ Rv_old, double* __restrict__ Rw_old, double* __restrict__ U_hat, double* __restrict__ V_hat,
+    double* __restrict__ W_hat, const Grid& g, const Geom& G, const Phys& p, int kmin_Vhat) {
     const long c = g.at(i, j, k);
     const long sz = g.sz(), sy = g.sy();
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * k);
-    auto u = [&](int di, int dj, int dk) { return U[c + di + dj * sz + dk * sy]; };
-    auto v = [&](int di, int dj, int dk) { return V[c + di + dj * sz + dk * sy]; };
-    auto w = [&](int di, int dj, int dk) { return W[c + di + dj * sz + dk * sy]; };
-    auto m = [&](int di, int dj, int dk) { return mu[c + di + dj * sz + dk * sy]; };
     auto r = [&](int di, int dj, int dk) { return rho[c + di + dj * sz + dk * sy]; };
     auto dv = [&](int di, int dj, int dk) { return dvn[c + di + dj * sz + dk * sy]; };
     auto b = [&](int di, int dj, int dk) { return buoy[c + di + dj * sz + dk * sy]; };
@@ -301,6 +293,53 @@ __global__ __launch_bounds__(kThreads) void k_momentum(
         }
         V_hat[c] = v(0, 0, 0) + p.dt * (1.5 * Rv - 0.5 * Rv_old[o]);
         Rv_old[o] = Rv;
+    }
+}
+
+// U, V, W (and mu) are read at up to 16 neighbouring points each, so a
+// workgroup stages them in LDS (PlaneRing, stencil_grid.h): it owns a 64 x 4
+// (i, j) tile and marches through kRunK planes with planes k-1, k, k+1 of
+// each field in a ring of three slots, the next plane's loads issued before
+// a step's arithmetic.  rho, div_now and buoyancy (four points each) are
+// read directly.  The per-thread version made 12.5 L1 and 2.9 L2 read
+// requests per point.
+template <bool VARPROP>
+__global__ __launch_bounds__(kThreads) void k_momentum(
+    const double* __restrict__ U, const double* __restrict__ V, const double* __restrict__ W,
+    const double* __restrict__ mu, const double* __restrict__ rho, const double* __restrict__ dvn,
+    const double* __restrict__ buoy, double* __restrict__ Ru_old, double* __restrict__ Rv_old,
+    double* __restrict__ Rw_old, double* __restrict__ U_hat, double* __restrict__ V_hat, double* __restrict__ W_hat,
+    Grid g, Geom G, Phys p, int kmin_Vhat) {
+    constexpr int NF = VARPROP ? 4 : 3;
+    __shared__ double sF[NF][3][kRingP];
+    const int i0 = (int)blockIdx.x * kTileX, j0 = (int)blockIdx.y * kTileZ;
+    const int k0 = (int)blockIdx.z * kRunK;
+    const int k1 = k0 + kRunK < g.ny ? k0 + kRunK : g.ny;
+    const int i = i0 + (int)threadIdx.x, j = j0 + (int)threadIdx.y;
+    const bool active = i < g.nx && j < g.nz;
+    const double* fields[4] = {U, V, W, mu};
+    PlaneRing<NF> ring(sF, fields, i0, j0, g.nx, g.nz);
+    ring.fetch(k0 - 1);
+    ring.put(k0 - 1);
+    ring.fetch(k0);
+    ring.put(k0);
+    ring.fetch(k0 + 1);
+    const int e0 = ring_point();
+    for (int k = k0; k < k1; ++k) {
+        ring.put(k + 1);  // the slot of plane k - 2, free since the last barrier
+        __syncthreads();
+        if (k + 2 <= k1) ring.fetch(k + 2);  // in flight during this step's arithmetic
+        if (active) {
+            const int sm = (k + 2) % 3, s0 = k % 3, sp = (k + 1) % 3;
+            auto slot = [&](int dk) { return dk < 0 ? sm : (dk > 0 ? sp : s0); };
+            auto lu = [&](int di, int dj, int dk) { return sF[0][slot(dk)][e0 + di + dj * kRingX]; };
+            auto lv = [&](int di, int dj, int dk) { return sF[1][slot(dk)][e0 + di + dj * kRingX]; };
+            auto lw = [&](int di, int dj, int dk) { return sF[2][slot(dk)][e0 + di + dj * kRingX]; };
+            auto lm = [&](int di, int dj, int dk) { return sF[NF - 1][slot(dk)][e0 + di + dj * kRingX]; };
+            momentum_point<VARPROP>(lu, lv, lw, lm, i, j, k, rho, dvn, buoy, Ru_old, Rv_old, Rw_old, U_hat, V_hat,
+                                    W_hat, g, G, p, kmin_Vhat);
+        }
+        __syncthreads();  // slot (k + 2) % 3 is restaged next
     }
 }
 
@@ -811,6 +850,9 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     const long nper = 9 * g.sy() + 4 * per_plane * g.ny;
     const BandGrid bg = band_grid(g.nx, g.nz, g.ny);
     const dim3 grid3((unsigned)bg.blocks());
+    // tiles x runs of planes, for the kernels that stage planes in LDS
+    const dim3 gridr((unsigned)((g.nx + kTileX - 1) / kTileX), (unsigned)((g.nz + kTileZ - 1) / kTileZ),
+                     (unsigned)((g.ny + kRunK - 1) / kRunK));
     const dim3 blk3 = band_block();
     const bool vp = !pr->incompressible;
     const bool rough = pr->d_bot || pr->ibm[0] || pr->ibm[1] || pr->ibm[2] || pr->ibm[3] || pr->ibm[4];
@@ -838,13 +880,13 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     double* mu = pr->f[DNSP_PRED_MU];
     double* buoy = pr->f[DNSP_PRED_BUOY];
     if (vp)
-        k_momentum<true><<<grid3, blk3, 0, st>>>(U, V, W, mu, rho, dvn, buoy, pr->f[DNSP_PRED_RU_OLD],
+        k_momentum<true><<<gridr, blk3, 0, st>>>(U, V, W, mu, rho, dvn, buoy, pr->f[DNSP_PRED_RU_OLD],
                                                      pr->f[DNSP_PRED_RV_OLD], pr->f[DNSP_PRED_RW_OLD], Uh, Vh, Wh, g,
-                                                     bg, pr->G, pr->p, pr->kmin_Vhat);
+                                                     pr->G, pr->p, pr->kmin_Vhat);
     else
-        k_momentum<false><<<grid3, blk3, 0, st>>>(U, V, W, nullptr, nullptr, nullptr, nullptr,
+        k_momentum<false><<<gridr, blk3, 0, st>>>(U, V, W, nullptr, nullptr, nullptr, nullptr,
                                                       pr->f[DNSP_PRED_RU_OLD], pr->f[DNSP_PRED_RV_OLD],
-                                                      pr->f[DNSP_PRED_RW_OLD], Uh, Vh, Wh, g, bg, pr->G, pr->p,
+                                                      pr->f[DNSP_PRED_RW_OLD], Uh, Vh, Wh, g, pr->G, pr->p,
                                                       pr->kmin_Vhat);
     // 4. temperature
     if (vp)

@@ -57,4 +57,59 @@ __device__ __forceinline__ bool band_point(const BandGrid& b, int& i, int& j, in
     return i < b.nx && j < b.nz;
 }
 
+// Ring of three planes of NF halo fields in LDS, for a workgroup of
+// kTileX x kTileZ threads that owns one (i, j) tile and marches through a run
+// of planes in k.  A slot holds one plane of the tile plus one halo point on
+// each side in x and z (kRingX x kRingZ values, point (i, j) of the tile at
+// (j + 1) * kRingX + i + 1); plane kp (-1..ny) lives in slot (kp + 3) % 3.
+// Each thread stages halo-tile points tid and tid + 256: fetch(kp) loads
+// plane kp into registers, put(kp) stores them to the slot, so the loads of
+// the next plane can be issued before a step's arithmetic.  Fields are the
+// reference's halo arrays ((nx+2)(nz+2)(ny+2), x fastest).
+constexpr int kRingX = kTileX + 2, kRingZ = kTileZ + 2, kRingP = kRingX * kRingZ;
+constexpr int kRunK = 16;  // planes per workgroup
+
+template <int NF>
+struct PlaneRing {
+    static constexpr int kNT = kTileX * kTileZ;
+    static_assert(kRingP <= 2 * kNT, "two staged points per thread");
+    double (*lds)[3][kRingP];
+    const double* f[NF];
+    long go[2], sy;
+    bool ok[2];
+    double r[NF][2];
+    int tid;
+
+    // Tile at (i0, j0) of an nx x nz x ny grid; lds points to NF x 3 slots.
+    __device__ __forceinline__ PlaneRing(double (*lds_)[3][kRingP], const double* const* fields, int i0, int j0,
+                                         int nx, int nz)
+        : lds(lds_) {
+        for (int q = 0; q < NF; ++q) f[q] = fields[q];
+        tid = (int)threadIdx.y * kTileX + (int)threadIdx.x;
+        const long sz = nx + 2;
+        sy = sz * (nz + 2);
+        for (int u = 0; u < 2; ++u) {
+            const int e = tid + u * kNT;
+            const int rr = e / kRingX, qq = e - rr * kRingX;
+            const int gi = i0 - 1 + qq, gj = j0 - 1 + rr;
+            ok[u] = e < kRingP && gi <= nx && gj <= nz;
+            go[u] = ok[u] ? (long)(gi + 1) + sz * (long)(gj + 1) + sy : 0;  // plane 0
+        }
+    }
+    __device__ __forceinline__ void fetch(int kp) {
+        for (int u = 0; u < 2; ++u)
+            if (ok[u])
+                for (int q = 0; q < NF; ++q) r[q][u] = f[q][go[u] + (long)kp * sy];
+    }
+    __device__ __forceinline__ void put(int kp) {
+        const int slot = (kp + 3) % 3;
+        for (int u = 0; u < 2; ++u)
+            if (ok[u])
+                for (int q = 0; q < NF; ++q) lds[q][slot][tid + u * kNT] = r[q][u];
+    }
+};
+
+// This thread's point in a ring slot.
+__device__ __forceinline__ int ring_point() { return ((int)threadIdx.y + 1) * kRingX + (int)threadIdx.x + 1; }
+
 }  // namespace dnsp
