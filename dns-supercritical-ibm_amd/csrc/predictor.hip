@@ -400,7 +400,13 @@ __global__ void k_temp_update(double* __restrict__ T, const double* __restrict__
 struct Law {
     int kind, nseg, off;
     double c, p;
+    // cubic: kBuckets equal buckets over [blo, blo + kBuckets / binv) map T
+    // to a starting segment (the law's bucket table), refined exactly by
+    // comparisons with the bounds
+    double blo, binv;
 };
+
+constexpr int kBuckets = 256;
 
 enum { Q_ENTH = 0, Q_RHO, Q_MU, Q_COND, Q_CP, Q_DRHO, Q_N };
 
@@ -412,7 +418,11 @@ struct PropSet {
     double b1, b0;
 };
 
-__device__ __forceinline__ double law_eval(const Law& L, const double* seg, double T) {
+// tab: the law's kBuckets starting segments.  The segment is the first one
+// with T <= hi (the last if none): the bucket's entry, moved down while the
+// previous bound still holds T and up while T exceeds the current bound,
+// which is that segment whatever bucket rounding picked.
+__device__ __forceinline__ double law_eval(const Law& L, const double* seg, const int* tab, double T) {
     if (L.kind == 0) return L.c;
     if (L.kind == 1) {
         if (L.p == -1.0) return L.c / T;
@@ -421,13 +431,13 @@ __device__ __forceinline__ double law_eval(const Law& L, const double* seg, doub
         if (L.p == 1.0) return L.c * T;
         return L.c * pow(T, L.p);
     }
-    int lo = 0, hi = L.nseg - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (T <= seg[6 * (L.off + mid)]) hi = mid;
-        else lo = mid + 1;
-    }
-    const double* s = seg + 6 * (L.off + lo);
+    const double* sg = seg + 6 * L.off;
+    const double x = (T - L.blo) * L.binv;
+    const int b = !(x >= 0.0) ? 0 : (x >= (double)kBuckets ? kBuckets - 1 : (int)x);
+    int lo = tab[b];
+    while (lo > 0 && T <= sg[6 * (lo - 1)]) --lo;
+    while (lo < L.nseg - 1 && T > sg[6 * lo]) ++lo;
+    const double* s = sg + 6 * lo;
     const double d = T - s[1];
     return ((s[2] + s[3] * d) + s[4] * (d * d)) + s[5] * ((d * d) * d);
 }
@@ -435,9 +445,10 @@ __device__ __forceinline__ double law_eval(const Law& L, const double* seg, doub
 // T_ext and the seven properties at every padded point (update_props): in
 // rough-wall columns (bot/top given) T_ext takes the wall values for
 // k <= bot(i, j) and k >= top(i, j), halo points those of their periodic
-// image.  Each workgroup stages the cubic laws' segments (nseg of them) in
-// LDS once and strides over the points, so the interval searches read LDS.
-constexpr int kMaxSegments = 1024;  // 48 KB of LDS
+// image.  Each workgroup stages the cubic laws' segments (nseg of them) and
+// bucket tables in LDS once and strides over the points, so the interval
+// searches read LDS.
+constexpr int kMaxSegments = 1024;  // 48 KB of LDS, plus 6 KB of bucket tables
 
 __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T, double* __restrict__ T_ext,
                                                     double* __restrict__ enth, double* __restrict__ rho,
@@ -445,9 +456,12 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
                                                     double* __restrict__ cp, double* __restrict__ divjac,
                                                     double* __restrict__ buoy, const int* __restrict__ bot,
                                                     const int* __restrict__ top, double t_bot, double t_top,
-                                                    const double* __restrict__ seg_g, int nseg, PropSet ps, Grid g) {
+                                                    const double* __restrict__ seg_g, int nseg,
+                                                    const int* __restrict__ tab_g, PropSet ps, Grid g) {
     extern __shared__ double seg[];
+    int* tab = reinterpret_cast<int*>(seg + 6 * nseg);
     for (int e = (int)threadIdx.x; e < 6 * nseg; e += kThreads) seg[e] = seg_g[e];
+    for (int e = (int)threadIdx.x; e < Q_N * kBuckets; e += kThreads) tab[e] = tab_g[e];
     __syncthreads();
     const long n = g.total();
     for (long idx = (long)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (long)gridDim.x * kThreads) {
@@ -466,11 +480,11 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
                 if (k >= top[col]) te = t_top;
             }
         }
-        const double h = law_eval(ps.law[Q_ENTH], seg, te);
-        const double r = law_eval(ps.law[Q_RHO], seg, te);
-        const double m = law_eval(ps.law[Q_MU], seg, te);
-        const double k = law_eval(ps.law[Q_COND], seg, te);
-        const double c = law_eval(ps.law[Q_CP], seg, te);
+        const double h = law_eval(ps.law[Q_ENTH], seg, tab + Q_ENTH * kBuckets, te);
+        const double r = law_eval(ps.law[Q_RHO], seg, tab + Q_RHO * kBuckets, te);
+        const double m = law_eval(ps.law[Q_MU], seg, tab + Q_MU * kBuckets, te);
+        const double k = law_eval(ps.law[Q_COND], seg, tab + Q_COND * kBuckets, te);
+        const double c = law_eval(ps.law[Q_CP], seg, tab + Q_CP * kBuckets, te);
         T_ext[idx] = te;
         enth[idx] = h;
         rho[idx] = r;
@@ -478,7 +492,7 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
         cond[idx] = k;
         cp[idx] = c;
         if (ps.divjac_formula) {
-            const double dr = law_eval(ps.law[Q_DRHO], seg, te);
+            const double dr = law_eval(ps.law[Q_DRHO], seg, tab + Q_DRHO * kBuckets, te);
             divjac[idx] = (-1.0 / ((r * r) * c)) * dr;
         } else {
             divjac[idx] = ps.divjac_c;
@@ -541,6 +555,8 @@ struct dnsp_predictor {
     std::vector<double> h_seg;  // host copy of every law's segments
     double* d_seg = nullptr;
     size_t d_seg_n = 0;
+    std::vector<int> h_tab = std::vector<int>(Q_N * kBuckets, 0);  // bucket tables, kBuckets per quantity
+    int* d_tab = nullptr;
     int* d_bot = nullptr;
     int* d_top = nullptr;
     dnsp_ibm* ibm[5] = {};  // U, V, W, T, div_now forcing (borrowed)
@@ -563,6 +579,7 @@ void release(dnsp_predictor* pr) {
     (void)hipFree(pr->d_rt_now);
     (void)hipFree(pr->d_geom);
     (void)hipFree(pr->d_seg);
+    (void)hipFree(pr->d_tab);
     (void)hipFree(pr->d_bot);
     (void)hipFree(pr->d_top);
     delete pr;
@@ -686,7 +703,10 @@ int dnsp_predictor_create(dnsp_step* s, const double* yu, const dnsp_pred_params
         bool ok = hipMalloc(&pr->d_geom, sizeof(double) * gh.size()) == hipSuccess &&
                   hipMemcpy(pr->d_geom, gh.data(), sizeof(double) * gh.size(), hipMemcpyHostToDevice) == hipSuccess &&
                   hipMalloc(&pr->d_rt_now, sizeof(double) * ic) == hipSuccess &&
-                  hipMemset(pr->d_rt_now, 0, sizeof(double) * ic) == hipSuccess;
+                  hipMemset(pr->d_rt_now, 0, sizeof(double) * ic) == hipSuccess &&
+                  hipMalloc(&pr->d_tab, sizeof(int) * pr->h_tab.size()) == hipSuccess &&
+                  hipMemcpy(pr->d_tab, pr->h_tab.data(), sizeof(int) * pr->h_tab.size(), hipMemcpyHostToDevice) ==
+                      hipSuccess;
         for (int i = 0; i < DNSP_PRED_NFIELDS && ok; ++i) {
             // the property fields only for the variable-property form
             if (inc && i >= DNSP_PRED_MU) continue;
@@ -746,7 +766,7 @@ int dnsp_predictor_set_law(dnsp_predictor* pr, int quantity, int kind, const dou
     DNSP_ARG_CHECK(pr && data, "NULL argument");
     DNSP_ARG_CHECK(quantity >= 0 && quantity < Q_N, "quantity %d out of range", quantity);
     DNSP_ARG_CHECK(!pr->incompressible, "property laws belong to the variable-property form");
-    Law L{kind, 0, 0, 0.0, 0.0};
+    Law L{kind, 0, 0, 0.0, 0.0, 0.0, 0.0};
     if (kind == DNSP_LAW_CONST) {
         L.c = data[0];
     } else if (kind == DNSP_LAW_POWER) {
@@ -771,6 +791,23 @@ int dnsp_predictor_set_law(dnsp_predictor* pr, int quantity, int kind, const dou
                        off + n);
         L.nseg = n;
         L.off = off;
+        // bucket table over [first bound, last bound]: entry b = first segment
+        // whose bound is >= the bucket's start (the device refines it exactly)
+        int* tb = pr->h_tab.data() + quantity * kBuckets;
+        if (n >= 3 && data[6 * (n - 2)] > data[0]) {
+            L.blo = data[0];
+            L.binv = kBuckets / (data[6 * (n - 2)] - data[0]);
+            for (int b = 0, sg = 0; b < kBuckets; ++b) {
+                const double start = L.blo + b / L.binv;
+                while (sg < n - 1 && data[6 * sg] < start) ++sg;
+                tb[b] = sg;
+            }
+        } else {
+            L.blo = 0.0;
+            L.binv = 0.0;
+            for (int b = 0; b < kBuckets; ++b) tb[b] = 0;
+        }
+        DNSP_HIP_TRY(hipMemcpy(pr->d_tab, pr->h_tab.data(), sizeof(int) * pr->h_tab.size(), hipMemcpyHostToDevice));
         keep.insert(keep.end(), data, data + 6 * (size_t)n);
         pr->h_seg.swap(keep);
         if (pr->h_seg.size() > pr->d_seg_n) {
@@ -914,9 +951,10 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     if (vp) {
         const int nseg = (int)(pr->h_seg.size() / 6);
         const int nbp = std::min(blocks_for(g.total()), 2048);
-        k_props<<<nbp, kThreads, sizeof(double) * 6 * nseg, st>>>(
+        k_props<<<nbp, kThreads, sizeof(double) * 6 * nseg + sizeof(int) * Q_N * kBuckets, st>>>(
             T, pr->f[DNSP_PRED_T_EXT], pr->f[DNSP_PRED_ENTH], rho, mu, pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_CP],
-            pr->f[DNSP_PRED_DIVJAC], buoy, pr->d_bot, pr->d_top, wl[3], wl[7], pr->d_seg, nseg, pr->ps, g);
+            pr->f[DNSP_PRED_DIVJAC], buoy, pr->d_bot, pr->d_top, wl[3], wl[7], pr->d_seg, nseg, pr->d_tab, pr->ps,
+            g);
         if (pr->ibm[3]) {
             // build_rhs_energy (:640-645): T forcing, then T's wall planes
             rc = dnsp_ibm_exec(pr->ibm[3], T, vs);
