@@ -309,12 +309,11 @@ __global__ __launch_bounds__(kThreads) void k_momentum(
     const double* __restrict__ mu, const double* __restrict__ rho, const double* __restrict__ dvn,
     const double* __restrict__ buoy, double* __restrict__ Ru_old, double* __restrict__ Rv_old,
     double* __restrict__ Rw_old, double* __restrict__ U_hat, double* __restrict__ V_hat, double* __restrict__ W_hat,
-    Grid g, Geom G, Phys p, int kmin_Vhat) {
+    Grid g, BandGrid bg, Geom G, Phys p, int kmin_Vhat) {
     constexpr int NF = VARPROP ? 4 : 3;
     __shared__ double sF[NF][3][kRingP];
-    const int i0 = (int)blockIdx.x * kTileX, j0 = (int)blockIdx.y * kTileZ;
-    const int k0 = (int)blockIdx.z * kRunK;
-    const int k1 = k0 + kRunK < g.ny ? k0 + kRunK : g.ny;
+    int i0, j0, k0, k1;
+    if (!ring_tile(bg, g.ny, i0, j0, k0, k1)) return;
     const int i = i0 + (int)threadIdx.x, j = j0 + (int)threadIdx.y;
     const bool active = i < g.nx && j < g.nz;
     const double* fields[4] = {U, V, W, mu};
@@ -888,8 +887,8 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     const BandGrid bg = band_grid(g.nx, g.nz, g.ny);
     const dim3 grid3((unsigned)bg.blocks());
     // tiles x runs of planes, for the kernels that stage planes in LDS
-    const dim3 gridr((unsigned)((g.nx + kTileX - 1) / kTileX), (unsigned)((g.nz + kTileZ - 1) / kTileZ),
-                     (unsigned)((g.ny + kRunK - 1) / kRunK));
+    const BandGrid br = ring_grid(g.nx, g.nz, g.ny);
+    const dim3 gridr((unsigned)br.blocks());
     const dim3 blk3 = band_block();
     const bool vp = !pr->incompressible;
     const bool rough = pr->d_bot || pr->ibm[0] || pr->ibm[1] || pr->ibm[2] || pr->ibm[3] || pr->ibm[4];
@@ -919,11 +918,11 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     if (vp)
         k_momentum<true><<<gridr, blk3, 0, st>>>(U, V, W, mu, rho, dvn, buoy, pr->f[DNSP_PRED_RU_OLD],
                                                      pr->f[DNSP_PRED_RV_OLD], pr->f[DNSP_PRED_RW_OLD], Uh, Vh, Wh, g,
-                                                     pr->G, pr->p, pr->kmin_Vhat);
+                                                     br, pr->G, pr->p, pr->kmin_Vhat);
     else
         k_momentum<false><<<gridr, blk3, 0, st>>>(U, V, W, nullptr, nullptr, nullptr, nullptr,
                                                       pr->f[DNSP_PRED_RU_OLD], pr->f[DNSP_PRED_RV_OLD],
-                                                      pr->f[DNSP_PRED_RW_OLD], Uh, Vh, Wh, g, pr->G, pr->p,
+                                                      pr->f[DNSP_PRED_RW_OLD], Uh, Vh, Wh, g, br, pr->G, pr->p,
                                                       pr->kmin_Vhat);
     // 4. temperature
     if (vp)

@@ -108,13 +108,12 @@ __global__ __launch_bounds__(kTileX * kTileZ) void k_rhs_varprop(
     double* P, const double* __restrict__ Uh, const double* __restrict__ Wh, const double* __restrict__ Vh,
     const double* __restrict__ Pn, const double* __restrict__ Po, const double* __restrict__ rho,
     const double* __restrict__ div_now, const double* __restrict__ inv_dy_p, const double* __restrict__ inv_dy_p_pad,
-    const double* __restrict__ rho_min_p, Halo h, double inv_dtmax, double inv_dx, double inv_dz, double inv_dx2,
-    double inv_dz2) {
+    const double* __restrict__ rho_min_p, Halo h, BandGrid bg, double inv_dtmax, double inv_dx, double inv_dz,
+    double inv_dx2, double inv_dz2) {
 #pragma clang fp contract(off)
     __shared__ double sF[3][3][kRingP];  // P_now, P_old, rho
-    const int i0 = (int)blockIdx.x * kTileX, j0 = (int)blockIdx.y * kTileZ;
-    const int k0 = (int)blockIdx.z * kRunK;
-    const int k1 = k0 + kRunK < h.ny ? k0 + kRunK : h.ny;
+    int i0, j0, k0, k1;
+    if (!ring_tile(bg, h.ny, i0, j0, k0, k1)) return;
     const int i = i0 + (int)threadIdx.x, j = j0 + (int)threadIdx.y;
     const bool active = i < h.nx && j < h.nz;
     const long sz = h.sz(), sy = h.sy();
@@ -392,10 +391,9 @@ int dnsp_build_p_rhs_varprop(double* P, const double* U_hat, const double* W_hat
                        rho_min && halo_ok(nx, nz, ny),
                    "invalid arguments");
     Halo h{nx, nz, ny};
-    const dim3 grid((unsigned)((nx + kTileX - 1) / kTileX), (unsigned)((nz + kTileZ - 1) / kTileZ),
-                    (unsigned)((ny + kRunK - 1) / kRunK));
-    k_rhs_varprop<<<grid, band_block(), 0, as_stream(stream)>>>(
-        P, U_hat, W_hat, V_hat, P_now, P_old, rho, div_now, inv_dy_p, inv_dy_p_pad, rho_min, h, inv_dtmax, inv_dx,
+    const BandGrid bg = ring_grid(nx, nz, ny);
+    k_rhs_varprop<<<bg.blocks(), band_block(), 0, as_stream(stream)>>>(
+        P, U_hat, W_hat, V_hat, P_now, P_old, rho, div_now, inv_dy_p, inv_dy_p_pad, rho_min, h, bg, inv_dtmax, inv_dx,
         inv_dz, inv_dx2, inv_dz2);
     DNSP_LAUNCH_CHECK();
     return DNSP_OK;

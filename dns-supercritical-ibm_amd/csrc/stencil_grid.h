@@ -40,18 +40,24 @@ inline BandGrid band_grid(int nx, int nz, int ny) {
 
 inline dim3 band_block() { return dim3(kTileX, kTileZ, 1); }
 
-// This thread's point (i, j, k), 0-based; false outside the grid (rows of
-// the last band past nz, columns past nx).  Launch: band_grid(..).blocks()
-// workgroups of band_block().
-__device__ __forceinline__ bool band_point(const BandGrid& b, int& i, int& j, int& k) {
+// This workgroup's tile (bx, bz) and plane (or run of planes) k.  Launch:
+// band_grid(..).blocks() workgroups of band_block().
+__device__ __forceinline__ void band_tile(const BandGrid& b, int& bx, int& bz, int& k) {
     const int L = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int band = L / b.per_band();
     const int r = L - band * b.per_band();
     const int tb = b.nbx * b.hb;
     k = r / tb;
     const int t = r - k * tb;
-    const int bz = band * b.hb + t / b.nbx;
-    const int bx = t - (t / b.nbx) * b.nbx;
+    bz = band * b.hb + t / b.nbx;
+    bx = t - (t / b.nbx) * b.nbx;
+}
+
+// This thread's point (i, j, k), 0-based; false outside the grid (rows of
+// the last band past nz, columns past nx).
+__device__ __forceinline__ bool band_point(const BandGrid& b, int& i, int& j, int& k) {
+    int bx, bz;
+    band_tile(b, bx, bz, k);
     i = bx * kTileX + (int)threadIdx.x;
     j = bz * kTileZ + (int)threadIdx.y;
     return i < b.nx && j < b.nz;
@@ -68,6 +74,23 @@ __device__ __forceinline__ bool band_point(const BandGrid& b, int& i, int& j, in
 // reference's halo arrays ((nx+2)(nz+2)(ny+2), x fastest).
 constexpr int kRingX = kTileX + 2, kRingZ = kTileZ + 2, kRingP = kRingX * kRingZ;
 constexpr int kRunK = 16;  // planes per workgroup
+
+// Band order over tiles and runs of kRunK planes (an XCD walks the tiles of
+// its band one run at a time, so neighbouring tiles' halo rows meet in its
+// L2): launch ring_grid(nx, nz, ny).blocks() workgroups of band_block().
+inline BandGrid ring_grid(int nx, int nz, int ny) { return band_grid(nx, nz, (ny + kRunK - 1) / kRunK); }
+
+// This workgroup's tile origin (i0, j0) and planes [k0, k1); false for a
+// workgroup past the last band's rows (uniform, before any barrier).
+__device__ __forceinline__ bool ring_tile(const BandGrid& b, int ny, int& i0, int& j0, int& k0, int& k1) {
+    int bx, bz, run;
+    band_tile(b, bx, bz, run);
+    i0 = bx * kTileX;
+    j0 = bz * kTileZ;
+    k0 = run * kRunK;
+    k1 = k0 + kRunK < ny ? k0 + kRunK : ny;
+    return j0 < b.nz;
+}
 
 template <int NF>
 struct PlaneRing {
