@@ -421,15 +421,7 @@ struct PropSet {
 // with T <= hi (the last if none): the bucket's entry, moved down while the
 // previous bound still holds T and up while T exceeds the current bound,
 // which is that segment whatever bucket rounding picked.
-__device__ __forceinline__ double law_eval(const Law& L, const double* seg, const int* tab, double T) {
-    if (L.kind == 0) return L.c;
-    if (L.kind == 1) {
-        if (L.p == -1.0) return L.c / T;
-        if (L.p == -0.5) return L.c / sqrt(T);
-        if (L.p == 0.5) return L.c * sqrt(T);
-        if (L.p == 1.0) return L.c * T;
-        return L.c * pow(T, L.p);
-    }
+__device__ __forceinline__ double cubic_eval(const Law& L, const double* seg, const int* tab, double T) {
     const double* sg = seg + 6 * L.off;
     const double x = (T - L.blo) * L.binv;
     const int b = !(x >= 0.0) ? 0 : (x >= (double)kBuckets ? kBuckets - 1 : (int)x);
@@ -441,6 +433,23 @@ __device__ __forceinline__ double law_eval(const Law& L, const double* seg, cons
     return ((s[2] + s[3] * d) + s[4] * (d * d)) + s[5] * ((d * d) * d);
 }
 
+// ALL_CUBIC: every law is known to be a piecewise cubic (no constant or
+// power-law code in the kernel).
+template <bool ALL_CUBIC>
+__device__ __forceinline__ double law_eval(const Law& L, const double* seg, const int* tab, double T) {
+    if constexpr (!ALL_CUBIC) {
+        if (L.kind == 0) return L.c;
+        if (L.kind == 1) {
+            if (L.p == -1.0) return L.c / T;
+            if (L.p == -0.5) return L.c / sqrt(T);
+            if (L.p == 0.5) return L.c * sqrt(T);
+            if (L.p == 1.0) return L.c * T;
+            return L.c * pow(T, L.p);
+        }
+    }
+    return cubic_eval(L, seg, tab, T);
+}
+
 // T_ext and the seven properties at every padded point (update_props): in
 // rough-wall columns (bot/top given) T_ext takes the wall values for
 // k <= bot(i, j) and k >= top(i, j), halo points those of their periodic
@@ -449,6 +458,8 @@ __device__ __forceinline__ double law_eval(const Law& L, const double* seg, cons
 // searches read LDS.
 constexpr int kMaxSegments = 1024;  // 48 KB of LDS, plus 6 KB of bucket tables
 
+// IDX32: the padded field has fewer than 2^31 points (32-bit index split).
+template <bool ALL_CUBIC, bool IDX32>
 __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T, double* __restrict__ T_ext,
                                                     double* __restrict__ enth, double* __restrict__ rho,
                                                     double* __restrict__ mu, double* __restrict__ cond,
@@ -466,11 +477,20 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
     for (long idx = (long)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (long)gridDim.x * kThreads) {
         double te = T[idx];
         if (bot != nullptr) {
-            const long pe = g.sy();
-            const int k = (int)(idx / pe) - 1;
-            const long r = idx - (long)(k + 1) * pe;
-            const int j = (int)(r / (g.nx + 2)) - 1;
-            const int i = (int)(r - (long)(j + 1) * (g.nx + 2)) - 1;
+            int i, j, k;
+            if constexpr (IDX32) {
+                const unsigned pe = (unsigned)g.sy(), px = (unsigned)(g.nx + 2), x = (unsigned)idx;
+                const unsigned kp = x / pe, r = x - kp * pe, jp = r / px;
+                k = (int)kp - 1;
+                j = (int)jp - 1;
+                i = (int)(r - jp * px) - 1;
+            } else {
+                const long pe = g.sy();
+                k = (int)(idx / pe) - 1;
+                const long r = idx - (long)(k + 1) * pe;
+                j = (int)(r / (g.nx + 2)) - 1;
+                i = (int)(r - (long)(j + 1) * (g.nx + 2)) - 1;
+            }
             const int wi = i < 0 ? g.nx - 1 : (i >= g.nx ? 0 : i);
             const int wj = j < 0 ? g.nz - 1 : (j >= g.nz ? 0 : j);
             if (k >= 0 && k < g.ny) {
@@ -479,11 +499,11 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
                 if (k >= top[col]) te = t_top;
             }
         }
-        const double h = law_eval(ps.law[Q_ENTH], seg, tab + Q_ENTH * kBuckets, te);
-        const double r = law_eval(ps.law[Q_RHO], seg, tab + Q_RHO * kBuckets, te);
-        const double m = law_eval(ps.law[Q_MU], seg, tab + Q_MU * kBuckets, te);
-        const double k = law_eval(ps.law[Q_COND], seg, tab + Q_COND * kBuckets, te);
-        const double c = law_eval(ps.law[Q_CP], seg, tab + Q_CP * kBuckets, te);
+        const double h = law_eval<ALL_CUBIC>(ps.law[Q_ENTH], seg, tab + Q_ENTH * kBuckets, te);
+        const double r = law_eval<ALL_CUBIC>(ps.law[Q_RHO], seg, tab + Q_RHO * kBuckets, te);
+        const double m = law_eval<ALL_CUBIC>(ps.law[Q_MU], seg, tab + Q_MU * kBuckets, te);
+        const double k = law_eval<ALL_CUBIC>(ps.law[Q_COND], seg, tab + Q_COND * kBuckets, te);
+        const double c = law_eval<ALL_CUBIC>(ps.law[Q_CP], seg, tab + Q_CP * kBuckets, te);
         T_ext[idx] = te;
         enth[idx] = h;
         rho[idx] = r;
@@ -491,7 +511,7 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
         cond[idx] = k;
         cp[idx] = c;
         if (ps.divjac_formula) {
-            const double dr = law_eval(ps.law[Q_DRHO], seg, tab + Q_DRHO * kBuckets, te);
+            const double dr = law_eval<ALL_CUBIC>(ps.law[Q_DRHO], seg, tab + Q_DRHO * kBuckets, te);
             divjac[idx] = (-1.0 / ((r * r) * c)) * dr;
         } else {
             divjac[idx] = ps.divjac_c;
@@ -950,7 +970,12 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     if (vp) {
         const int nseg = (int)(pr->h_seg.size() / 6);
         const int nbp = std::min(blocks_for(g.total()), 2048);
-        k_props<<<nbp, kThreads, sizeof(double) * 6 * nseg + sizeof(int) * Q_N * kBuckets, st>>>(
+        bool all_cubic = true;
+        for (int q = 0; q < Q_N; ++q) all_cubic = all_cubic && pr->ps.law[q].kind == DNSP_LAW_CUBIC;
+        const bool idx32 = g.total() < (1L << 31);
+        auto kp = all_cubic ? (idx32 ? k_props<true, true> : k_props<true, false>)
+                            : (idx32 ? k_props<false, true> : k_props<false, false>);
+        kp<<<nbp, kThreads, sizeof(double) * 6 * nseg + sizeof(int) * Q_N * kBuckets, st>>>(
             T, pr->f[DNSP_PRED_T_EXT], pr->f[DNSP_PRED_ENTH], rho, mu, pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_CP],
             pr->f[DNSP_PRED_DIVJAC], buoy, pr->d_bot, pr->d_top, wl[3], wl[7], pr->d_seg, nseg, pr->d_tab, pr->ps,
             g);
