@@ -38,4 +38,26 @@ void clear_error();
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// A rough-wall column-slab pair (bottom, top; n ints each) on the device:
+// copies the host slabs in (allocating on first use, *bytes grows by the
+// allocation when given), or frees the pair when both are NULL.
+inline int set_slab_pair(int** d_bot, int** d_top, const int* bot, const int* top, size_t n, size_t* bytes) {
+    DNSP_ARG_CHECK((bot == nullptr) == (top == nullptr), "give both column slabs or neither");
+    if (!bot) {
+        if (*d_bot && bytes) *bytes -= 2 * sizeof(int) * n;
+        (void)hipFree(*d_bot);
+        (void)hipFree(*d_top);
+        *d_bot = *d_top = nullptr;
+        return DNSP_OK;
+    }
+    if (!*d_bot) {
+        DNSP_HIP_TRY(hipMalloc(d_bot, sizeof(int) * n));
+        DNSP_HIP_TRY(hipMalloc(d_top, sizeof(int) * n));
+        if (bytes) *bytes += 2 * sizeof(int) * n;
+    }
+    DNSP_HIP_TRY(hipMemcpy(*d_bot, bot, sizeof(int) * n, hipMemcpyHostToDevice));
+    DNSP_HIP_TRY(hipMemcpy(*d_top, top, sizeof(int) * n, hipMemcpyHostToDevice));
+    return DNSP_OK;
+}
+
 }  // namespace dnsp

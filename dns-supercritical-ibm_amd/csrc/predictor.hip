@@ -576,8 +576,8 @@ struct dnsp_predictor {
     size_t d_seg_n = 0;
     std::vector<int> h_tab = std::vector<int>(Q_N * kBuckets, 0);  // bucket tables, kBuckets per quantity
     int* d_tab = nullptr;
-    int* d_bot = nullptr;
-    int* d_top = nullptr;
+    int* d_bot[4] = {};  // rough-wall column slabs of U, V, W, T (nx*nz each), or NULL
+    int* d_top[4] = {};
     dnsp_ibm* ibm[5] = {};  // U, V, W, T, div_now forcing (borrowed)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -599,8 +599,10 @@ void release(dnsp_predictor* pr) {
     (void)hipFree(pr->d_geom);
     (void)hipFree(pr->d_seg);
     (void)hipFree(pr->d_tab);
-    (void)hipFree(pr->d_bot);
-    (void)hipFree(pr->d_top);
+    for (int c = 0; c < 4; ++c) {
+        (void)hipFree(pr->d_bot[c]);
+        (void)hipFree(pr->d_top[c]);
+    }
     delete pr;
 }
 
@@ -857,23 +859,20 @@ int dnsp_predictor_set_derived(dnsp_predictor* pr, int divjac_formula, double di
     return DNSP_OK;
 }
 
+int dnsp_predictor_set_component_columns(dnsp_predictor* pr, int component, const int* bot, const int* top) {
+    clear_error();
+    DNSP_ARG_CHECK(pr && component >= 0 && component < 4, "NULL predictor or component %d", component);
+    return set_slab_pair(&pr->d_bot[component], &pr->d_top[component], bot, top, (size_t)pr->g.nx * pr->g.nz,
+                         nullptr);
+}
+
 int dnsp_predictor_set_columns(dnsp_predictor* pr, const int* bot, const int* top) {
     clear_error();
     DNSP_ARG_CHECK(pr, "NULL predictor");
-    DNSP_ARG_CHECK((bot == nullptr) == (top == nullptr), "give both column slabs or neither");
-    const size_t n = (size_t)pr->g.nx * pr->g.nz;
-    if (!bot) {
-        (void)hipFree(pr->d_bot);
-        (void)hipFree(pr->d_top);
-        pr->d_bot = pr->d_top = nullptr;
-        return DNSP_OK;
+    for (int c = 0; c < 4; ++c) {
+        const int rc = dnsp_predictor_set_component_columns(pr, c, bot, top);
+        if (rc) return rc;
     }
-    if (!pr->d_bot) {
-        DNSP_HIP_TRY(hipMalloc(&pr->d_bot, sizeof(int) * n));
-        DNSP_HIP_TRY(hipMalloc(&pr->d_top, sizeof(int) * n));
-    }
-    DNSP_HIP_TRY(hipMemcpy(pr->d_bot, bot, sizeof(int) * n, hipMemcpyHostToDevice));
-    DNSP_HIP_TRY(hipMemcpy(pr->d_top, top, sizeof(int) * n, hipMemcpyHostToDevice));
     return DNSP_OK;
 }
 
@@ -911,7 +910,8 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     const dim3 gridr((unsigned)br.blocks());
     const dim3 blk3 = band_block();
     const bool vp = !pr->incompressible;
-    const bool rough = pr->d_bot || pr->ibm[0] || pr->ibm[1] || pr->ibm[2] || pr->ibm[3] || pr->ibm[4];
+    const bool columns = pr->d_bot[0] || pr->d_bot[1] || pr->d_bot[2] || pr->d_bot[3];
+    const bool rough = columns || pr->ibm[0] || pr->ibm[1] || pr->ibm[2] || pr->ibm[3] || pr->ibm[4];
     void* vs = stream;
     if (!rough) {
         // 1. cfd_periodicity(U, V, W, T)
@@ -952,13 +952,15 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
         k_temp_rhs<false><<<grid3, blk3, 0, st>>>(T, U, V, W, nullptr, nullptr, nullptr, pr->d_rt_now, g, bg, pr->G,
                                                       pr->p);
     k_temp_update<<<grid3, blk3, 0, st>>>(T, pr->d_rt_now, pr->f[DNSP_PRED_RT_OLD], g, bg, pr->p.dt);
-    // 5. rough walls: columns of the hats and T (ibm_part_2, mod_cfd_ibm.f90:47-141)
-    if (rough && pr->d_bot) {
+    // 5. rough walls: columns of the hats and T, each from its own slabs
+    // (ibm_part_2, mod_cfd_ibm.f90:47-141)
+    if (columns) {
         double* H4[4] = {Uh, Vh, Wh, T};
         const double wb[4] = {wl[0], wl[1], wl[2], wl[3]}, wt[4] = {wl[4], wl[5], wl[6], wl[7]};
         for (int c = 0; c < 4 && !rc; ++c)
-            rc = dnsp_ibm_part2(H4[c], pr->d_bot, pr->d_top, g.nx, g.nz, g.ny, c == 1 ? g.ny : g.ny - 1, wb[c], wt[c],
-                                vs);
+            if (pr->d_bot[c])
+                rc = dnsp_ibm_part2(H4[c], pr->d_bot[c], pr->d_top[c], g.nx, g.nz, g.ny, c == 1 ? g.ny : g.ny - 1,
+                                    wb[c], wt[c], vs);
         if (rc) return rc;
     }
     // periodicity of the hats and T with their wall planes (with rough walls
@@ -977,7 +979,7 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
                             : (idx32 ? k_props<false, true> : k_props<false, false>);
         kp<<<nbp, kThreads, sizeof(double) * 6 * nseg + sizeof(int) * Q_N * kBuckets, st>>>(
             T, pr->f[DNSP_PRED_T_EXT], pr->f[DNSP_PRED_ENTH], rho, mu, pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_CP],
-            pr->f[DNSP_PRED_DIVJAC], buoy, pr->d_bot, pr->d_top, wl[3], wl[7], pr->d_seg, nseg, pr->d_tab, pr->ps,
+            pr->f[DNSP_PRED_DIVJAC], buoy, pr->d_bot[3], pr->d_top[3], wl[3], wl[7], pr->d_seg, nseg, pr->d_tab, pr->ps,
             g);
         if (pr->ibm[3]) {
             // build_rhs_energy (:640-645): T forcing, then T's wall planes
@@ -987,7 +989,7 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
             if (rc) return rc;
         }
         k_rhs_energy<<<grid3, blk3, 0, st>>>(T, pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_DIVJAC],
-                                                 pr->f[DNSP_PRED_RHS_ENERGY], dvn, pr->d_bot, pr->d_top, g, bg,
+                                                 pr->f[DNSP_PRED_RHS_ENERGY], dvn, pr->d_bot[3], pr->d_top[3], g, bg,
                                                  pr->G, pr->p);
         k_div_finish<<<blocks_for(2 * g.sy() + per_plane * g.ny), kThreads, 0, st>>>(dvn, g);
     }

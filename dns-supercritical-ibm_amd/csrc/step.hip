@@ -32,8 +32,8 @@ struct dnsp_step {
     double* d_walls = nullptr;          // 6: U_bot, V_bot, W_bot, U_top, V_top, W_top
     double walls[6] = {0, 0, 0, 0, 0, 0};
     dnsp_ibm* ibm[3] = {nullptr, nullptr, nullptr};  // U, V, W (borrowed)
-    int* d_bot = nullptr;               // rough-wall column slabs (nx*nz each), or NULL
-    int* d_top = nullptr;
+    int* d_bot[3] = {};                 // rough-wall column slabs of U, V, W (nx*nz each), or NULL
+    int* d_top[3] = {};
     size_t bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -51,8 +51,10 @@ void release(dnsp_step* s) {
     (void)hipFree(s->d_geom);
     (void)hipFree(s->d_rho_min);
     (void)hipFree(s->d_walls);
-    (void)hipFree(s->d_bot);
-    (void)hipFree(s->d_top);
+    for (int c = 0; c < 3; ++c) {
+        (void)hipFree(s->d_bot[c]);
+        (void)hipFree(s->d_top[c]);
+    }
     if (s->poisson) dnsp_poisson_free(s->poisson);
     delete s;
 }
@@ -93,12 +95,12 @@ int enqueue(dnsp_step* s, int refinements, hipStream_t st) {
     for (int c = 0; c < 3 && !rc; ++c)
         if (s->ibm[c]) rc = dnsp_ibm_exec(s->ibm[c], F[comp[c]], vs);
     if (rc) return rc;
-    if (s->d_bot) {
-        // kmax = ny-1 for U, W and ny for V (set_ibm_bands.f90 / mod_cfd_ibm.f90:47-141)
-        for (int c = 0; c < 3 && !rc; ++c)
-            rc = dnsp_ibm_part2(F[comp[c]], s->d_bot, s->d_top, nx, nz, ny, c == 1 ? ny : ny - 1, s->walls[c],
+    // each component's own columns; kmax = ny-1 for U, W and ny for V
+    // (set_ibm_bands.f90 / mod_cfd_ibm.f90:47-141)
+    for (int c = 0; c < 3 && !rc; ++c)
+        if (s->d_bot[c])
+            rc = dnsp_ibm_part2(F[comp[c]], s->d_bot[c], s->d_top[c], nx, nz, ny, c == 1 ? ny : ny - 1, s->walls[c],
                                 s->walls[3 + c], vs);
-    }
     return rc;
 }
 
@@ -234,23 +236,19 @@ int dnsp_step_set_ibm(dnsp_step* s, int component, dnsp_ibm* ibm) {
     return DNSP_OK;
 }
 
+int dnsp_step_set_component_columns(dnsp_step* s, int component, const int* bot, const int* top) {
+    clear_error();
+    DNSP_ARG_CHECK(s && component >= 0 && component < 3, "NULL step or component %d", component);
+    return set_slab_pair(&s->d_bot[component], &s->d_top[component], bot, top, (size_t)s->nx * s->nz, &s->bytes);
+}
+
 int dnsp_step_set_columns(dnsp_step* s, const int* bot, const int* top) {
     clear_error();
     DNSP_ARG_CHECK(s, "NULL step");
-    const size_t n = (size_t)s->nx * s->nz;
-    if (!bot || !top) {
-        (void)hipFree(s->d_bot);
-        (void)hipFree(s->d_top);
-        s->d_bot = s->d_top = nullptr;
-        return DNSP_OK;
+    for (int c = 0; c < 3; ++c) {
+        const int rc = dnsp_step_set_component_columns(s, c, bot, top);
+        if (rc) return rc;
     }
-    if (!s->d_bot) {
-        DNSP_HIP_TRY(hipMalloc(&s->d_bot, sizeof(int) * n));
-        DNSP_HIP_TRY(hipMalloc(&s->d_top, sizeof(int) * n));
-        s->bytes += 2 * sizeof(int) * n;
-    }
-    DNSP_HIP_TRY(hipMemcpy(s->d_bot, bot, sizeof(int) * n, hipMemcpyHostToDevice));
-    DNSP_HIP_TRY(hipMemcpy(s->d_top, top, sizeof(int) * n, hipMemcpyHostToDevice));
     return DNSP_OK;
 }
 

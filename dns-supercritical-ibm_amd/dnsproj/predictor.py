@@ -38,6 +38,7 @@ _SIGS = [
     ("dnsp_predictor_set_law", [_vp, _i, _i, _dp, _i]),
     ("dnsp_predictor_set_derived", [_vp, _i, _d, _i, _d, _d]),
     ("dnsp_predictor_set_columns", [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    ("dnsp_predictor_set_component_columns", [_vp, ctypes.c_int, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     ("dnsp_predictor_set_ibm", [_vp, _i, _vp]),
     ("dnsp_predictor_enqueue", [_vp, _vp]),
     ("dnsp_predictor_capture", [_vp, _i, _i, _vp]),
@@ -148,14 +149,21 @@ class Predictor:
         b1, b0 = (float(buoy[1]), float(buoy[2])) if lin else (0.0, 0.0)
         _check(_lib().dnsp_predictor_set_derived(self._h, f, c, lin, b1, b0))
 
-    def set_columns(self, bot, top):
+    def set_columns(self, bot, top, component=None):
+        """Column slabs (nz, nx) of U, V, W and T alike, or of one component
+        (0..3 or "U", "V", "W", "T"); None removes them."""
+        if component is None:
+            fn, args = _lib().dnsp_predictor_set_columns, ()
+        else:
+            k = "UVWT".index(component) if isinstance(component, str) else int(component)
+            fn, args = _lib().dnsp_predictor_set_component_columns, (k,)
         if bot is None:
-            _check(_lib().dnsp_predictor_set_columns(self._h, None, None))
+            _check(fn(self._h, *args, None, None))
             return
         b = np.ascontiguousarray(bot, dtype=np.int32)
         t = np.ascontiguousarray(top, dtype=np.int32)
         ip = ctypes.POINTER(_i)
-        _check(_lib().dnsp_predictor_set_columns(self._h, b.ctypes.data_as(ip), t.ctypes.data_as(ip)))
+        _check(fn(self._h, *args, b.ctypes.data_as(ip), t.ctypes.data_as(ip)))
 
     def set_ibm(self, forcings):
         """forcings: {"U"|"V"|"W"|"T"|"div": IbmForcing or None}; the handles must outlive the predictor."""

@@ -237,6 +237,7 @@ _STEP_SIGS = [
     ("dnsp_step_set_walls", [_vp, _dp]),
     ("dnsp_step_set_ibm", [_vp, _i, _vp]),
     ("dnsp_step_set_columns", [_vp, _ip, _ip]),
+    ("dnsp_step_set_component_columns", [_vp, ctypes.c_int, _ip, _ip]),
     ("dnsp_step_poisson", [_vp, ctypes.POINTER(_vp)]),
     ("dnsp_step_footprint", [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     ("dnsp_step_enqueue", [_vp, _i, _vp]),
@@ -317,10 +318,20 @@ class ProjectionStep:
             h = ibm_set.handles.get(comp) if ibm_set else None
             _check(_lib().dnsp_step_set_ibm(self._h, k, h._h if h is not None else _vp(None)))
 
-    def set_columns(self, bot, top):
-        b = np.ascontiguousarray(bot, dtype=np.int32)
-        t = np.ascontiguousarray(top, dtype=np.int32)
-        _check(_lib().dnsp_step_set_columns(self._h, b.ctypes.data_as(_ip), t.ctypes.data_as(_ip)))
+    def set_columns(self, bot, top, component=None):
+        """Column slabs (nz, nx) of U, V and W alike, or of one component
+        (0, 1, 2 or "U", "V", "W"); None removes them."""
+        if bot is None:
+            b = t = _ip()
+        else:
+            b = np.ascontiguousarray(bot, dtype=np.int32)
+            t = np.ascontiguousarray(top, dtype=np.int32)
+            b, t = b.ctypes.data_as(_ip), t.ctypes.data_as(_ip)
+        if component is None:
+            _check(_lib().dnsp_step_set_columns(self._h, b, t))
+        else:
+            k = "UVW".index(component) if isinstance(component, str) else int(component)
+            _check(_lib().dnsp_step_set_component_columns(self._h, k, b, t))
 
     @property
     def footprint(self):

@@ -447,9 +447,13 @@ int dnsp_step_set_walls(dnsp_step* s, const double* wall_bc);
 /* IBM forcing of component 0 (U), 1 (V), 2 (W): a handle created with
  * nhalo = 1, nskip = 0 (borrowed, must outlive the step); NULL removes it. */
 int dnsp_step_set_ibm(dnsp_step* s, int component, dnsp_ibm* ibm);
-/* Rough-wall column slabs (host nx*nz ints each, as dnsp_ibm_part2); NULL
- * removes them. */
+/* Rough-wall column slabs (host nx*nz ints each, as dnsp_ibm_part2) of U,
+ * V and W alike; NULL removes them. */
 int dnsp_step_set_columns(dnsp_step* s, const int* bot, const int* top);
+/* The same for one component 0 (U), 1 (V), 2 (W): the reference reads a
+ * slab pair per component (rough_U/V/W_bot/top_inds, set_ibm_bands.f90:
+ * 114-218) and ibm_part_2 fills each field from its own. */
+int dnsp_step_set_component_columns(dnsp_step* s, int component, const int* bot, const int* top);
 /* The embedded Poisson solver (for tuning; owned by the step). */
 int dnsp_step_poisson(dnsp_step* s, dnsp_poisson** h);
 /* Device bytes the step holds (fields, geometry, solver work arrays). */
@@ -536,18 +540,21 @@ int dnsp_predictor_set_law(dnsp_predictor* pr, int quantity, int kind, const dou
  * buoyancy: linear != 0: b1 (rho - b0), else 0. */
 int dnsp_predictor_set_derived(dnsp_predictor* pr, int divjac_formula, double divjac_const, int buoy_linear,
                                double b1, double b0);
-/* Rough-wall temperature columns (rough_T_bot/top_inds, nx*nz ints, i
- * fastest; both NULL: smooth walls): T_ext takes the wall values at
- * k <= bot and k >= top, div_now is zero there. */
+/* Rough-wall column slabs (nx*nz ints, i fastest; both NULL: none) of U, V,
+ * W and T alike.  ibm_part_2 fills the columns of each of U_hat, V_hat,
+ * W_hat and T; with T's slabs (rough_T_bot/top_inds) T_ext takes the wall
+ * values at k <= bot and k >= top and div_now is zero there. */
 int dnsp_predictor_set_columns(dnsp_predictor* pr, const int* bot, const int* top);
+/* The same for one component 0..3 = U, V, W, T (the reference's per-component
+ * rough_<X>_bot/top_inds, set_ibm_bands.f90:114-218). */
+int dnsp_predictor_set_component_columns(dnsp_predictor* pr, int component, const int* bot, const int* top);
 /* Rough-wall forcing (borrowed handles, NULL: none) for component 0..4 =
  * U, V, W, T, div_now (the reference's obj_ibm_U/V/W/T/div).  With any
  * forcing or columns set the predictor follows build_vars' rough-wall
  * branch (use_rough_surf without quady): ibm_part_1 (forcing of U, V, W, T,
  * wall planes, div_now forcing) instead of the opening periodicity,
- * ibm_part_2 on the hats and T after the predictor (the column slabs of
- * dnsp_predictor_set_columns for all four), and the T forcing at the start
- * of build_rhs_energy. */
+ * ibm_part_2 on the hats and T after the predictor (each with its column
+ * slabs), and the T forcing at the start of build_rhs_energy. */
 int dnsp_predictor_set_ibm(dnsp_predictor* pr, int component, dnsp_ibm* h);
 int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream);
 /* Records the predictor (and, with_step != 0, the step's sequence after

@@ -402,7 +402,11 @@ def predictor_step(st, g, p, laws=None, derived=None, incompressible=True, kmin_
     """st: dict of arrays (U, V, W, T, mu, rho, div_now, buoy, cp, cond, divjac,
     enth, T_ext halo arrays; Ru_old, Rv_old, Rw_old, Rt_old, RHS_energy interior
     arrays); updated in place; returns (U_hat, V_hat, W_hat).  bot/top: column
-    slabs (nx, nz); ibm: {"U"|"V"|"W"|"T"|"div": (ijk, c)} forcing records.
+    slabs (nx, nz), one pair for every component or dicts per component
+    {"U"|"V"|"W"|"T": slab} (the reference's rough_<X>_bot/top_inds:
+    ibm_part_2 fills each field's columns from its own pair, update_props and
+    build_rhs_energy use T's); ibm: {"U"|"V"|"W"|"T"|"div": (ijk, c)} forcing
+    records.
     With columns or forcing, build_vars' rough-wall branch (without quady):
     ibm_part_1 opens (mod_cfd_ibm.f90:10-45), ibm_part_2 on the hats and T
     follows the predictor (:47-141), T is forced again in build_rhs_energy
@@ -410,6 +414,10 @@ def predictor_step(st, g, p, laws=None, derived=None, incompressible=True, kmin_
     walls = p["walls"]
     ibm = ibm or {}
     rough = bot is not None or bool(ibm)
+
+    def comp(slab, name):
+        return slab.get(name) if isinstance(slab, dict) else slab
+
     ny2, nz2, nx2 = st["U"].shape
     nx, nz, ny = nx2 - 2, nz2 - 2, ny2 - 2
 
@@ -435,10 +443,13 @@ def predictor_step(st, g, p, laws=None, derived=None, incompressible=True, kmin_
     advance_T(st["T"], st["U"], st["V"], st["W"], st["rho"], st["cp"], st["RHS_energy"], st["Rt_old"], g, p,
               incompressible)
     if rough and bot is not None:
-        bj, tj = np.asarray(bot).T, np.asarray(top).T  # (nz, nx)
         for A, name, kmax in ((U_hat, "U", ny - 1), (V_hat, "V", ny), (W_hat, "W", ny - 1), (st["T"], "T", ny - 1)):
-            A[...] = ibm_part2(A, bj, tj, kmax, walls[name + "_bot"], walls[name + "_top"])
+            b, t = comp(bot, name), comp(top, name)
+            if b is not None:  # (nz, nx) for ibm_part2
+                A[...] = ibm_part2(A, np.asarray(b).T, np.asarray(t).T, kmax, walls[name + "_bot"],
+                                   walls[name + "_top"])
     periodicity(U_hat, V_hat, W_hat, st["T"], walls)
+    bot, top = (None, None) if bot is None else (comp(bot, "T"), comp(top, "T"))
     if not incompressible:
         update_props(st["T"], st, laws, derived, walls, bot, top)
         if "T" in ibm:

@@ -159,6 +159,35 @@ def test_predictor_rough_columns_match_oracle(cuda):
     s.close()
 
 
+def test_predictor_component_columns_match_oracle(cuda):
+    """A column-slab pair per component (U, V, W, T): ibm_part_2 fills each of
+    U_hat, V_hat, W_hat and T from its own, T_ext and div_now follow T's."""
+    say("predictor component columns")
+    nx, nz, ny = 20, 12, 16
+    c = make_case(nx, nz, ny, seed=13, kind="wan")
+    cols = {}
+    for n, comp in enumerate("UVWT"):
+        cols[comp] = columns(nx, nz, ny, seed=30 + n)
+    s, pr = build(c)
+    for comp in "UVWT":
+        pr.set_columns(*cols[comp], component=comp)
+    load(s, pr, c)
+    pr.enqueue()
+    st = {k: v.copy() for k, v in list(c["f"].items()) + list(c["pf"].items())}
+    shp = c["f"]["U"].shape
+    for k in ("mu", "buoy", "cp", "cond", "enth", "divjac", "T_ext"):
+        st.setdefault(k, np.zeros(shp))
+    st.setdefault("RHS_energy", np.zeros((ny, nz, nx)))
+    bot = {k: v[0].T for k, v in cols.items()}  # oracle slabs are (nx, nz)
+    top = {k: v[1].T for k, v in cols.items()}
+    Uh, Vh, Wh = PRO.predictor_step(st, PRO.predictor_geom(c["yu"]), c["p"], c["laws"], c["derived"],
+                                    c["incompressible"], 1, bot, top)
+    st.update(U_hat=Uh, V_hat=Vh, W_hat=Wh)
+    compare(s, pr, st, c)
+    pr.close()
+    s.close()
+
+
 @pytest.mark.parametrize("kind", ["incomp", "wan"])
 def test_predictor_rough_ibm_matches_oracle(cuda, kind):
     """The rough-wall branch with forcing of U, V, W, T and div_now plus columns."""

@@ -65,7 +65,8 @@ def oracle_step(c, incompressible=False, with_ibm=True):
             out[comp] = PJ.ibm_exec(out[comp], ijk.reshape(-1, 3, 3), cc, nx, nz, ny, 1, 0)
         kmax = {"U": ny - 1, "V": ny, "W": ny - 1}
         for n, comp in enumerate(("U", "V", "W")):
-            out[comp] = PJ.ibm_part2(out[comp], c["bot"], c["top"], kmax[comp], c["walls"][n], c["walls"][3 + n])
+            bot, top = c["cols"][comp] if "cols" in c else (c["bot"], c["top"])
+            out[comp] = PJ.ibm_part2(out[comp], bot, top, kmax[comp], c["walls"][n], c["walls"][3 + n])
     out.update({"P_per": P_per, "P": p, "rhs": rhs})
     if incompressible:
         out.update({"P_now": f["P_now"], "P_old": f["P_old"]})
@@ -119,6 +120,40 @@ def test_step_autotune_keeps_oracle_parity(cuda):
     s.autotune(reps=2)
     for k, v in before.items():
         assert np.array_equal(host(s.read(k)), v), k
+    s.enqueue(-1)
+    for k in OUT:
+        assert rel(host(s.read(k)), ref[k]) <= 1e-11, k
+    s.close()
+
+
+def component_columns(c, seed):
+    """Per-component column slabs: the case's pair moved by -1..1 (bottom)
+    and 0..1 (top) independently for each component."""
+    rng = np.random.default_rng(seed)
+    ny = c["ny"]
+    cols = {}
+    for comp in "UVWT":
+        b = np.clip(c["bot"] + rng.integers(-1, 2, size=c["bot"].shape), -1, ny - 2).astype(np.int32)
+        t = np.clip(c["top"] - rng.integers(0, 2, size=c["top"].shape), 1, ny + 1).astype(np.int32)
+        cols[comp] = (b, t)
+    return cols
+
+
+def test_step_component_columns_match_oracle_chain(cuda):
+    """A column-slab pair per velocity component (the reference's
+    rough_U/V/W_bot/top_inds): each field's columns from its own pair."""
+    say("step component columns")
+    c = W.projection_case(64, 32, 48, seed=9)
+    c["walls"] = np.array([0.25, -0.5, 0.125, 1.0, 0.0, -0.75])
+    c["cols"] = component_columns(c, 4)
+    assert not np.array_equal(c["cols"]["U"][0], c["cols"]["V"][0])
+    ref = oracle_step(c)
+    s = DP.ProjectionStep(c["nx"], c["nz"], c["ny"], c["yu"], c["inv_dx"], c["inv_dz"], c["dtmax"])
+    s.set_walls(c["walls"])
+    s.set_ibm(DP.IbmForcingSet(c["nx"], c["nz"], c["ny"], c["ibm"]))
+    for comp in "UVW":
+        s.set_columns(*c["cols"][comp], component=comp)
+    s.load(c["fields"])
     s.enqueue(-1)
     for k in OUT:
         assert rel(host(s.read(k)), ref[k]) <= 1e-11, k
