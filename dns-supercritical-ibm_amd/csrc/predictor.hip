@@ -564,6 +564,7 @@ struct dnsp_predictor {
     Phys p{};
     int incompressible = 1;
     int kmin_Vhat = 1;
+    int quady = 0;  // use_rough_surf_quady ordering of the rough-wall branch
     double walls[8] = {};  // U, V, W, T bottom; U, V, W, T top
     double extrap[4] = {};
     double* d_geom = nullptr;
@@ -876,6 +877,13 @@ int dnsp_predictor_set_columns(dnsp_predictor* pr, const int* bot, const int* to
     return DNSP_OK;
 }
 
+int dnsp_predictor_set_quady(dnsp_predictor* pr, int quady) {
+    clear_error();
+    DNSP_ARG_CHECK(pr, "NULL predictor");
+    pr->quady = quady ? 1 : 0;
+    return DNSP_OK;
+}
+
 int dnsp_predictor_set_ibm(dnsp_predictor* pr, int component, dnsp_ibm* h) {
     clear_error();
     DNSP_ARG_CHECK(pr, "NULL predictor");
@@ -913,19 +921,44 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     const bool columns = pr->d_bot[0] || pr->d_bot[1] || pr->d_bot[2] || pr->d_bot[3];
     const bool rough = columns || pr->ibm[0] || pr->ibm[1] || pr->ibm[2] || pr->ibm[3] || pr->ibm[4];
     void* vs = stream;
+    double* F4[4] = {U, V, W, T};
+    double* H4[4] = {Uh, Vh, Wh, T};
+    const double wb[4] = {wl[0], wl[1], wl[2], wl[3]}, wt[4] = {wl[4], wl[5], wl[6], wl[7]};
+    // ibm_part_2 (mod_cfd_ibm.f90:47-141) on four fields, each from its own slabs
+    auto fill_columns = [&](double* const* A4) {
+        int r = DNSP_OK;
+        for (int c = 0; c < 4 && !r; ++c)
+            if (pr->d_bot[c])
+                r = dnsp_ibm_part2(A4[c], pr->d_bot[c], pr->d_top[c], g.nx, g.nz, g.ny, c == 1 ? g.ny : g.ny - 1,
+                                   wb[c], wt[c], vs);
+        return r;
+    };
+    // ibm_part_1 (mod_cfd_ibm.f90:10-45): forcing of four fields, their wall
+    // planes, and (force_div) the forcing of div_now
+    auto part_1 = [&](double* const* A4, bool force_div) {
+        int r = DNSP_OK;
+        for (int c = 0; c < 4 && !r; ++c)
+            if (pr->ibm[c]) r = dnsp_ibm_exec(pr->ibm[c], A4[c], vs);
+        if (r) return r;
+        k_walls4<<<blocks_for(8 * g.sy()), kThreads, 0, st>>>(A4[0], A4[1], A4[2], A4[3], g, wl[0], wl[1], wl[2],
+                                                              wl[3], wl[4], wl[5], wl[6], wl[7]);
+        if (force_div && pr->ibm[4]) r = dnsp_ibm_exec(pr->ibm[4], dvn, vs);
+        return r;
+    };
     if (!rough) {
         // 1. cfd_periodicity(U, V, W, T)
         k_periodicity4<<<blocks_for(nper), kThreads, 0, st>>>(U, V, W, T, g, wl[0], wl[1], wl[2], wl[3], wl[4],
                                                               wl[5], wl[6], wl[7]);
     } else {
-        // 1. ibm_part_1 (mod_cfd_ibm.f90:10-45): forcing of U, V, W, T, wall planes, forcing of div_now
-        double* F4[4] = {U, V, W, T};
-        for (int c = 0; c < 4 && !rc; ++c)
-            if (pr->ibm[c]) rc = dnsp_ibm_exec(pr->ibm[c], F4[c], vs);
-        if (rc) return rc;
-        k_walls4<<<blocks_for(8 * g.sy()), kThreads, 0, st>>>(U, V, W, T, g, wl[0], wl[1], wl[2], wl[3], wl[4],
-                                                              wl[5], wl[6], wl[7]);
-        if (vp && pr->ibm[4]) rc = dnsp_ibm_exec(pr->ibm[4], dvn, vs);
+        // 1. rough walls (mod_cfd_subroutines.f90:98-108): with quady the columns
+        // of U, V, W, T and cfd_periodicity first; then ibm_part_1
+        if (pr->quady) {
+            rc = fill_columns(F4);
+            if (rc) return rc;
+            k_periodicity4<<<blocks_for(nper), kThreads, 0, st>>>(U, V, W, T, g, wl[0], wl[1], wl[2], wl[3], wl[4],
+                                                                  wl[5], wl[6], wl[7]);
+        }
+        rc = part_1(F4, vp);
         if (rc) return rc;
     }
     // 2. div_now ghost planes
@@ -953,14 +986,8 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
                                                       pr->p);
     k_temp_update<<<grid3, blk3, 0, st>>>(T, pr->d_rt_now, pr->f[DNSP_PRED_RT_OLD], g, bg, pr->p.dt);
     // 5. rough walls: columns of the hats and T, each from its own slabs
-    // (ibm_part_2, mod_cfd_ibm.f90:47-141)
     if (columns) {
-        double* H4[4] = {Uh, Vh, Wh, T};
-        const double wb[4] = {wl[0], wl[1], wl[2], wl[3]}, wt[4] = {wl[4], wl[5], wl[6], wl[7]};
-        for (int c = 0; c < 4 && !rc; ++c)
-            if (pr->d_bot[c])
-                rc = dnsp_ibm_part2(H4[c], pr->d_bot[c], pr->d_top[c], g.nx, g.nz, g.ny, c == 1 ? g.ny : g.ny - 1,
-                                    wb[c], wt[c], vs);
+        rc = fill_columns(H4);
         if (rc) return rc;
     }
     // periodicity of the hats and T with their wall planes (with rough walls
@@ -968,6 +995,13 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     // walls and T's; the extra halo and wall values are read by nothing)
     k_periodicity4<<<blocks_for(nper), kThreads, 0, st>>>(Uh, Vh, Wh, T, g, wl[0], wl[1], wl[2], wl[3], wl[4],
                                                           wl[5], wl[6], wl[7]);
+    if (rough && pr->quady) {
+        // quady (mod_cfd_subroutines.f90:417-427): ibm_part_1 on the hats and T;
+        // its last argument (.false.) leaves the div_now forcing on; then T's halos
+        rc = part_1(H4, true);
+        if (!rc) rc = dnsp_halo_periodic_xz(T, g.nx, g.nz, g.ny, vs);
+        if (rc) return rc;
+    }
     // 6. properties at the new temperature, energy right-hand side, div_now
     if (vp) {
         const int nseg = (int)(pr->h_seg.size() / 6);

@@ -39,6 +39,7 @@ _SIGS = [
     ("dnsp_predictor_set_derived", [_vp, _i, _d, _i, _d, _d]),
     ("dnsp_predictor_set_columns", [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     ("dnsp_predictor_set_component_columns", [_vp, ctypes.c_int, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    ("dnsp_predictor_set_quady", [_vp, ctypes.c_int]),
     ("dnsp_predictor_set_ibm", [_vp, _i, _vp]),
     ("dnsp_predictor_enqueue", [_vp, _vp]),
     ("dnsp_predictor_capture", [_vp, _i, _i, _vp]),
@@ -164,6 +165,10 @@ class Predictor:
         t = np.ascontiguousarray(top, dtype=np.int32)
         ip = ctypes.POINTER(_i)
         _check(fn(self._h, *args, b.ctypes.data_as(ip), t.ctypes.data_as(ip)))
+
+    def set_quady(self, quady=True):
+        """The quadratic-wall (use_rough_surf_quady) ordering of the rough-wall branch."""
+        _check(_lib().dnsp_predictor_set_quady(self._h, int(bool(quady))))
 
     def set_ibm(self, forcings):
         """forcings: {"U"|"V"|"W"|"T"|"div": IbmForcing or None}; the handles must outlive the predictor."""

@@ -551,11 +551,18 @@ int dnsp_predictor_set_component_columns(dnsp_predictor* pr, int component, cons
 /* Rough-wall forcing (borrowed handles, NULL: none) for component 0..4 =
  * U, V, W, T, div_now (the reference's obj_ibm_U/V/W/T/div).  With any
  * forcing or columns set the predictor follows build_vars' rough-wall
- * branch (use_rough_surf without quady): ibm_part_1 (forcing of U, V, W, T,
+ * branch (use_rough_surf; quady ordering: dnsp_predictor_set_quady): ibm_part_1 (forcing of U, V, W, T,
  * wall planes, div_now forcing) instead of the opening periodicity,
  * ibm_part_2 on the hats and T after the predictor (each with its column
  * slabs), and the T forcing at the start of build_rhs_energy. */
 int dnsp_predictor_set_ibm(dnsp_predictor* pr, int component, dnsp_ibm* h);
+/* The quadratic-wall ordering of the rough branch (use_rough_surf_quady,
+ * mod_cfd_subroutines.f90:98-108, :417-427): the opening fills the columns
+ * of U, V, W, T and applies cfd_periodicity before ibm_part_1, and after
+ * the predictor's column fill and cfd_periodicity the hats and T are forced
+ * again (ibm_part_1, div_now forcing included), then T's x/z halos.
+ * quady = 0 (default): the ghost-point ordering. */
+int dnsp_predictor_set_quady(dnsp_predictor* pr, int quady);
 int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream);
 /* Records the predictor (and, with_step != 0, the step's sequence after
  * it: one full time step) into a graph; dnsp_predictor_launch replays it. */

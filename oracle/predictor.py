@@ -398,7 +398,7 @@ def rhs_energy(T, cond, divjac, div_now, g, p, bot=None, top=None):
 # the predictor stage of one time step, in build_vars' order (smooth walls)
 # ---------------------------------------------------------------------------
 def predictor_step(st, g, p, laws=None, derived=None, incompressible=True, kmin_Vhat=1, bot=None, top=None,
-                   ibm=None):
+                   ibm=None, quady=False):
     """st: dict of arrays (U, V, W, T, mu, rho, div_now, buoy, cp, cond, divjac,
     enth, T_ext halo arrays; Ru_old, Rv_old, Rw_old, Rt_old, RHS_energy interior
     arrays); updated in place; returns (U_hat, V_hat, W_hat).  bot/top: column
@@ -406,7 +406,12 @@ def predictor_step(st, g, p, laws=None, derived=None, incompressible=True, kmin_
     {"U"|"V"|"W"|"T": slab} (the reference's rough_<X>_bot/top_inds:
     ibm_part_2 fills each field's columns from its own pair, update_props and
     build_rhs_energy use T's); ibm: {"U"|"V"|"W"|"T"|"div": (ijk, c)} forcing
-    records.
+    records.  quady (use_rough_surf_quady, mod_cfd_subroutines.f90:99-108 and
+    :417-427): with rough walls the opening is ibm_part_2 on U, V, W, T,
+    cfd_periodicity, then ibm_part_1; after the predictor ibm_part_2 on the
+    hats and T is followed by cfd_periodicity and ibm_part_1 on the hats and
+    T, whose last argument .false. leaves the div_now forcing on, then T's
+    halos.
     With columns or forcing, build_vars' rough-wall branch (without quady):
     ibm_part_1 opens (mod_cfd_ibm.f90:10-45), ibm_part_2 on the hats and T
     follows the predictor (:47-141), T is forced again in build_rhs_energy
@@ -425,30 +430,44 @@ def predictor_step(st, g, p, laws=None, derived=None, incompressible=True, kmin_
         if name in ibm:
             A[...] = ibm_exec(A, ibm[name][0], ibm[name][1], nx, nz, ny, 1, 0)
 
+    def columns(fields):
+        if bot is None:
+            return
+        for A, name, kmax in zip(fields, ("U", "V", "W", "T"), (ny - 1, ny, ny - 1, ny - 1)):
+            b, t = comp(bot, name), comp(top, name)
+            if b is not None:  # (nz, nx) for ibm_part2
+                A[...] = ibm_part2(A, np.asarray(b).T, np.asarray(t).T, kmax, walls[name + "_bot"],
+                                   walls[name + "_top"])
+
+    def part_1(fields, force_div):  # ibm_part_1 (mod_cfd_ibm.f90:10-45)
+        for A, name in zip(fields, ("U", "V", "W", "T")):
+            force(name, A)
+        for A, name in zip(fields, ("U", "V", "W", "T")):
+            set_val_slice(A, 0, 0, walls[name + "_bot"])
+        for A, name in zip(fields, ("U", "V", "W", "T")):
+            set_val_slice(A, -1, -1, walls[name + "_top"])
+        if force_div:
+            force("div", st["div_now"])
+
     if not rough:
         periodicity(st["U"], st["V"], st["W"], st["T"], walls)
     else:
-        for name in ("U", "V", "W", "T"):
-            force(name, st[name])
-        for name in ("U", "V", "W", "T"):
-            set_val_slice(st[name], 0, 0, walls[name + "_bot"])
-        for name in ("U", "V", "W", "T"):
-            set_val_slice(st[name], -1, -1, walls[name + "_top"])
-        if not incompressible:
-            force("div", st["div_now"])
+        if quady:
+            columns((st["U"], st["V"], st["W"], st["T"]))
+            periodicity(st["U"], st["V"], st["W"], st["T"], walls)
+        part_1((st["U"], st["V"], st["W"], st["T"]), not incompressible)
     if not incompressible:
         div_now_walls(st["div_now"], g["extrap"])
     U_hat, V_hat, W_hat = momentum(st["U"], st["V"], st["W"], st["mu"], st["rho"], st["div_now"], st["buoy"],
                                    st["Ru_old"], st["Rv_old"], st["Rw_old"], g, p, incompressible, kmin_Vhat)
     advance_T(st["T"], st["U"], st["V"], st["W"], st["rho"], st["cp"], st["RHS_energy"], st["Rt_old"], g, p,
               incompressible)
-    if rough and bot is not None:
-        for A, name, kmax in ((U_hat, "U", ny - 1), (V_hat, "V", ny), (W_hat, "W", ny - 1), (st["T"], "T", ny - 1)):
-            b, t = comp(bot, name), comp(top, name)
-            if b is not None:  # (nz, nx) for ibm_part2
-                A[...] = ibm_part2(A, np.asarray(b).T, np.asarray(t).T, kmax, walls[name + "_bot"],
-                                   walls[name + "_top"])
+    if rough:
+        columns((U_hat, V_hat, W_hat, st["T"]))
     periodicity(U_hat, V_hat, W_hat, st["T"], walls)
+    if rough and quady:
+        part_1((U_hat, V_hat, W_hat, st["T"]), True)
+        halo_periodic_xz(st["T"])
     bot, top = (None, None) if bot is None else (comp(bot, "T"), comp(top, "T"))
     if not incompressible:
         update_props(st["T"], st, laws, derived, walls, bot, top)

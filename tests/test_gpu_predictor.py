@@ -188,10 +188,12 @@ def test_predictor_component_columns_match_oracle(cuda):
     s.close()
 
 
+@pytest.mark.parametrize("quady", [False, True])
 @pytest.mark.parametrize("kind", ["incomp", "wan"])
-def test_predictor_rough_ibm_matches_oracle(cuda, kind):
-    """The rough-wall branch with forcing of U, V, W, T and div_now plus columns."""
-    say("predictor rough ibm", kind)
+def test_predictor_rough_ibm_matches_oracle(cuda, kind, quady):
+    """The rough-wall branch with forcing of U, V, W, T and div_now plus
+    columns, in the ghost-point and the quadratic-wall (quady) ordering."""
+    say("predictor rough ibm", kind, "quady" if quady else "")
     nx, nz, ny = 24, 12, 20
     c = make_case(nx, nz, ny, seed=21, kind=kind)
     h = W.rough_height_map(nx, nz, 0.08, 5)
@@ -202,6 +204,7 @@ def test_predictor_rough_ibm_matches_oracle(cuda, kind):
     s, pr = build(c, bot, top)
     handles = {k: DP.IbmForcing(nx, nz, ny, 1, 0, *recs[k]) for k in ("U", "V", "W", "T", "div")}
     pr.set_ibm(handles)
+    pr.set_quady(quady)
     load(s, pr, c)
     pr.enqueue()
     ibm = {k: (recs[k][0].reshape(-1, 3, 3), recs[k][1]) for k in recs}
@@ -211,7 +214,7 @@ def test_predictor_rough_ibm_matches_oracle(cuda, kind):
         st.setdefault(k, np.zeros(shp))
     st.setdefault("RHS_energy", np.zeros((ny, nz, nx)))
     Uh, Vh, Wh = PRO.predictor_step(st, PRO.predictor_geom(c["yu"]), c["p"], c["laws"], c["derived"],
-                                    c["incompressible"], 1, bot.T, top.T, ibm)
+                                    c["incompressible"], 1, bot.T, top.T, ibm, quady=quady)
     st.update(U_hat=Uh, V_hat=Vh, W_hat=Wh)
     compare(s, pr, st, c)
     pr.close()
