@@ -176,18 +176,28 @@ __global__ void k_min_init(double* out) {
         *reinterpret_cast<unsigned long long*>(out) = 0xffffffffffffffffull;
 }
 
-// Workgroups stride over the interior x rows (row -> (j, k) by one uniform
-// division); the threads of a workgroup over the row.
+// Workgroups stride over groups of kMinRows interior x rows (row -> (j, k)
+// by uniform divisions), the threads of a workgroup over the rows, with the
+// loads of all kMinRows rows issued before their keys are compared.
+constexpr int kMinRows = 4;
 __global__ void k_min_interior(const double* __restrict__ A, Halo h, double* out) {
     __shared__ unsigned long long red[kStencilThreads];
     const int rows = h.nz * h.ny;
     unsigned long long best = 0xffffffffffffffffull;
-    for (int row = (int)blockIdx.x; row < rows; row += (int)gridDim.x) {
-        const int k = row / h.nz, j = row - k * h.nz;
-        const double* a = A + h.at(0, j, k);
+    for (int row0 = (int)blockIdx.x * kMinRows; row0 < rows; row0 += (int)gridDim.x * kMinRows) {
         for (int i = (int)threadIdx.x; i < h.nx; i += kStencilThreads) {
-            const unsigned long long key = dkey(a[i]);
-            best = key < best ? key : best;
+            double v[kMinRows];
+#pragma unroll
+            for (int r = 0; r < kMinRows; ++r) {
+                const int row = row0 + r;
+                const int k = row / h.nz, j = row - k * h.nz;
+                v[r] = row < rows ? A[h.at(i, j, k)] : A[h.at(i, 0, 0)];  // past the last row: a repeat
+            }
+#pragma unroll
+            for (int r = 0; r < kMinRows; ++r) {
+                const unsigned long long key = dkey(v[r]);
+                best = key < best ? key : best;
+            }
         }
     }
     red[threadIdx.x] = best;
@@ -263,12 +273,26 @@ __global__ void k_update_varprop(const double* __restrict__ P, double* U, double
 }
 
 // P_per over the whole padded array, and (varprop) P_old <- P_now, P_now <- P_per
+// One thread per padded point, x fastest (a pure stream: no tiles, so no
+// idle lanes on the nx + 2 wide rows); IDX32: the padded array has fewer
+// than 2^31 points (32-bit index split).
+template <bool IDX32>
 __global__ void k_pper_rotate(const double* __restrict__ P, double* P_per, double* Pn, double* Po, Halo h,
-                              BandGrid bg, int rotate) {
-    int ip, jp, kp;  // padded coordinates (bg spans the padded array)
-    if (!band_point(bg, ip, jp, kp)) return;
-    const long idx = (long)ip + (long)(h.nx + 2) * ((long)jp + (long)(h.nz + 2) * kp);
-    const int i = ip - 1, j = jp - 1, k = kp - 1;
+                              int rotate) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= h.total()) return;
+    int i, j, k;
+    if constexpr (IDX32) {
+        const unsigned px = (unsigned)(h.nx + 2), pe = (unsigned)h.sy(), x = (unsigned)idx;
+        const unsigned kp = x / pe, r = x - kp * pe, jp = r / px;
+        i = (int)(r - jp * px) - 1;
+        j = (int)jp - 1;
+        k = (int)kp - 1;
+    } else {
+        i = (int)(idx % (h.nx + 2)) - 1;
+        j = (int)((idx / (h.nx + 2)) % (h.nz + 2)) - 1;
+        k = (int)(idx / h.sy()) - 1;
+    }
     const double v = P[pidx(h, i, j, k)];
     if (P_per) P_per[idx] = v;
     if (rotate) {
@@ -404,7 +428,7 @@ int dnsp_rho_min(const double* rho, int nx, int nz, int ny, double* out, void* s
     DNSP_ARG_CHECK(rho && out && halo_ok(nx, nz, ny), "invalid arguments");
     Halo h{nx, nz, ny};
     hipStream_t st = as_stream(stream);
-    const int nb = std::min(2048, nz * ny);
+    const int nb = std::min(2048, (nz * ny + kMinRows - 1) / kMinRows);
     k_min_init<<<1, 1, 0, st>>>(out);
     k_min_interior<<<nb, kStencilThreads, 0, st>>>(rho, h, out);
     k_min_decode<<<1, 1, 0, st>>>(out);
@@ -436,9 +460,8 @@ int dnsp_update_vars(const double* P, double* P_per, double* P_now, double* P_ol
                                                                dtmax, inv_dx, inv_dz);
     DNSP_LAUNCH_CHECK();
     if (P_per || !use_incompressible) {
-        const BandGrid bp = band_grid(nx + 2, nz + 2, ny + 2);
-        k_pper_rotate<<<bp.blocks(), band_block(), 0, st>>>(P, P_per, P_now, P_old, h, bp,
-                                                            use_incompressible ? 0 : 1);
+        auto kr = h.total() < (1L << 31) ? k_pper_rotate<true> : k_pper_rotate<false>;
+        kr<<<blocks_for(h.total()), kStencilThreads, 0, st>>>(P, P_per, P_now, P_old, h, use_incompressible ? 0 : 1);
         DNSP_LAUNCH_CHECK();
     }
     const long nfin = 7 * h.sy() + 3 * (2L * (nz + 2) + 2L * nx) * ny;
