@@ -261,5 +261,12 @@ def test_predictor_errors(cuda):
         pr.read("mu")
     with pytest.raises(D.DnspError):
         pr.launch()  # nothing captured
+    bot, top = columns(12, 8, 6, seed=2)
+    with pytest.raises(D.DnspError):
+        pr.set_columns(bot, top, component=4)  # components are U, V, W, T
+    with pytest.raises(D.DnspError):
+        s.set_columns(bot, top, component=3)  # the step's are U, V, W
+    pr.set_columns(bot, top, component="T")
+    pr.set_columns(None, None, component="T")  # removes them again
     pr.close()
     s.close()
