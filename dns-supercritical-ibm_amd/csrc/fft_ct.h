@@ -63,7 +63,10 @@ __device__ __forceinline__ void ct_stage(V* buf, const V* __restrict__ tw) {
             base[q] = g * LP;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = buf[base[q] + ct_pad(j + r * m)];
-            if (NS > 1 && k != 0) {
+            // no branch on k != 0: tw[0] = (1, 0) exactly, so k = 0 multiplies
+            // by one, and the twiddle loads of every q issue together (a load
+            // under a lane branch waits for its round trip before the next)
+            if constexpr (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
                     const V w = tw[r * k * tstep];

@@ -396,11 +396,28 @@ __device__ __forceinline__ void tri_load_rows(double (&d)[MR], const T* S, long 
                                               const double* __restrict__ msum) {
     const bool valid = mode < plane;
     const bool mean = msum != nullptr && mode == 0;
+    // Branch-free: every row load goes to a valid address (row and mode
+    // clamped) and the masking follows all of them, so the MR loads are in
+    // flight together (a load under a branch waits for its own round trip
+    // before the next one issues).  The mask is a bitwise AND, not a select
+    // the compiler could turn back into a branch, nor a 0/1 factor it could
+    // fuse into the elimination's first multiply-add (which would change the
+    // rounding); masked rows are +0.
+    const long md = valid ? mode : 0;
+    T raw[MR];
 #pragma unroll
     for (int r = 0; r < MR; ++r) {
-        const int row = s + r;
-        if (mean) d[r] = row < ny ? msum[row] : 0.0;
-        else d[r] = (valid && row < ny) ? (double)S[(size_t)row * plane + mode] : 0.0;
+        const int row = s + r < ny ? s + r : ny - 1;
+        raw[r] = S[(size_t)row * plane + md];
+    }
+#pragma unroll
+    for (int r = 0; r < MR; ++r) {
+        const long long keep = (valid && !mean && s + r < ny) ? -1LL : 0LL;
+        d[r] = __longlong_as_double(__double_as_longlong((double)raw[r]) & keep);
+    }
+    if (mean) {
+#pragma unroll
+        for (int r = 0; r < MR; ++r) d[r] = s + r < ny ? msum[s + r] : 0.0;
     }
 }
 

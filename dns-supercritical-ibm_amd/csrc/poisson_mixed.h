@@ -253,16 +253,141 @@ __device__ __forceinline__ void xfwd_mr_body(const TP* __restrict__ p, const dou
     xfwd_store_f32<N, G, NT>(sm, S, l0, nlines);
 }
 
+// Staged residual tile, for nz % 2G == 0: a tile is then 2G consecutive z
+// lines of one plane k0 (strip_tile keeps tiles whole), and every tile is
+// full.  The tile's lines and the z line on either side of it (2G + 2 lines,
+// periodic in z) are staged in LDS, so the x and z neighbours are LDS reads
+// and a p line is read from L2/HBM once per tile instead of three times; the
+// y neighbours and f are read directly.  The band values of plane k0 are
+// workgroup-uniform.  Term for term the arithmetic of xfwd_mr_body.  stg
+// (which aliases sm) holds (2G + 2) N values of TP.
+template <int N, int G, typename TP>
+constexpr bool mr_staged() {
+    return (2 * G + 2) * N * (int)sizeof(TP) <= 48 * 1024;
+}
+
+// float2 slots of the residual kernel's LDS tile (staging area or FFT lines)
+template <int N, int G, typename TP>
+constexpr int mr_lds_f2() {
+    constexpr int a = G * ct_pitch(N);
+    constexpr int b = (2 * G + 2) * N * (int)sizeof(TP) / 8;
+    return (mr_staged<N, G, TP>() && b > a) ? b : a;
+}
+
+template <int N, int G, int NT, typename TP>
+__device__ __forceinline__ void xfwd_mr_staged(const TP* __restrict__ p, const double* __restrict__ f,
+                                               const double* __restrict__ qk, const double* __restrict__ bands,
+                                               float* __restrict__ S, double* __restrict__ lsum, int nz, int ny,
+                                               double ix2, double iz2, long nlines,
+                                               const float2* __restrict__ tw, long tile, float2* sm,
+                                               double* slot) {
+    using P2 = typename Cx<TP>::V;
+    constexpr int LP = ct_pitch(N);
+    constexpr int NL = (G * N + NT - 1) / NT;
+    constexpr int NPS = (2 * G + 2) * N / 2;  // staged pairs
+    constexpr int NS = (NPS + NT - 1) / NT;
+    TP* stg = reinterpret_cast<TP*>(sm);
+    float* smf = reinterpret_cast<float*>(sm);
+    const long l0 = strip_tile<G>(tile, nz, ny) * 2 * G;
+    const int k0 = (int)(l0 / nz);
+    const int j0 = (int)(l0 - (long)k0 * nz);
+    const long pl = (long)k0 * nz;
+    // loads first (clamped, unconditional), LDS writes after them
+    P2 sv[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        int e = (int)threadIdx.x + q * NT;
+        e = (NPS % NT == 0 || e < NPS) ? e : 0;
+        const int u = (2 * e) / N;
+        const int t = 2 * e - u * N;
+        int jj = j0 - 1 + u;
+        jj = jj < 0 ? jj + nz : (jj >= nz ? jj - nz : jj);
+        sv[q] = *reinterpret_cast<const P2*>(p + (pl + jj) * N + t);
+    }
+    const long dn = k0 > 0 ? -(long)nz : 0;
+    const long up = k0 < ny - 1 ? (long)nz : 0;
+    P2 yd[NL], yu[NL];
+    double2 fv[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const bool in_tile = (G * N % NT == 0 || e < G * N);
+        const int ee = in_tile ? e : 0;
+        const int r = (2 * ee) / N;
+        const int t0 = 2 * ee - r * N;
+        const long line = l0 + r;
+        yd[q] = *reinterpret_cast<const P2*>(p + (line + dn) * N + t0);
+        yu[q] = *reinterpret_cast<const P2*>(p + (line + up) * N + t0);
+        fv[q] = *reinterpret_cast<const double2*>(f + line * N + t0);
+    }
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        if (NPS % NT == 0 || e < NPS) *reinterpret_cast<P2*>(stg + 2 * e) = sv[q];
+    }
+    const double av = bands[k0], bk = bands[ny + k0], cv = bands[2 * ny + k0], qv = qk[k0];
+    const double ak = k0 > 0 ? av : 0.0;
+    const double ck = k0 < ny - 1 ? cv : 0.0;
+    __syncthreads();
+    double r0s[NL], r1s[NL], ps[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const bool in_tile = (G * N % NT == 0 || e < G * N);
+        const int ee = in_tile ? e : 0;
+        const int r = (2 * ee) / N;
+        const int t0 = 2 * ee - r * N;
+        const TP* ln = stg + (r + 1) * N;
+        const P2 c = *reinterpret_cast<const P2*>(ln + t0);
+        const TP xl = ln[t0 == 0 ? N - 1 : t0 - 1];
+        const TP xr = ln[t0 + 2 == N ? 0 : t0 + 2];
+        const P2 zm = *reinterpret_cast<const P2*>(ln - N + t0);
+        const P2 zp = *reinterpret_cast<const P2*>(ln + N + t0);
+        const double cx = (double)c.x, cy = (double)c.y;
+        double v0 = ix2 * (cy - 2 * cx + (double)xl) + iz2 * ((double)zp.x - 2 * cx + (double)zm.x) + bk * cx;
+        double v1 = ix2 * ((double)xr - 2 * cy + cx) + iz2 * ((double)zp.y - 2 * cy + (double)zm.y) + bk * cy;
+        v0 += ak * (double)yd[q].x;
+        v1 += ak * (double)yd[q].y;
+        v0 += ck * (double)yu[q].x;
+        v1 += ck * (double)yu[q].y;
+        const double okf = in_tile ? 1.0 : 0.0;
+        r0s[q] = (fv[q].x - (v0 + qv)) * okf;
+        r1s[q] = (fv[q].y - (v1 + qv)) * okf;
+        ps[q] = r0s[q] + r1s[q];
+    }
+    __syncthreads();  // staged lines read; sm becomes the FFT tile
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        if (G * N % NT == 0 || e < G * N) {
+            const int r = (2 * e) / N;
+            const int t0 = 2 * e - r * N;
+            const int g = r >> 1, cc = r & 1;
+            smf[(g * LP + ct_pad(t0)) * 2 + cc] = (float)r0s[q];
+            smf[(g * LP + ct_pad(t0 + 1)) * 2 + cc] = (float)r1s[q];
+        }
+    }
+    tile_line_sums<N, G, NT, NL>(ps, slot, lsum, l0, nlines);
+    fft_ct<N, G, NT, false>(sm, tw);
+    xfwd_store_f32<N, G, NT>(sm, S, l0, nlines);
+}
+
 template <int N, int G, int NT, typename TP>
 __global__ __launch_bounds__(NT) void kt_xfwd_mr(const TP* __restrict__ p, const double* __restrict__ f,
                                                  const double* __restrict__ qk, const double* __restrict__ bands,
                                                  float* __restrict__ S, double* __restrict__ lsum, int nz,
                                                  int ny, double ix2, double iz2, long nlines,
                                                  const float2* __restrict__ tw) {
-    __shared__ float2 sm[G * ct_pitch(N)];
+    __shared__ __align__(16) float2 sm[mr_lds_f2<N, G, TP>()];
     __shared__ double slot[line_slots<N, G>()];
-    xfwd_mr_body<N, G, NT, TP>(p, f, qk, bands, S, lsum, nz, ny, ix2, iz2, nlines, tw,
-                               xcd_tile(blockIdx.x, gridDim.x), sm, slot);
+    const long tile = xcd_tile(blockIdx.x, gridDim.x);
+    if constexpr (mr_staged<N, G, TP>()) {
+        if (nz % (2 * G) == 0) {
+            xfwd_mr_staged<N, G, NT, TP>(p, f, qk, bands, S, lsum, nz, ny, ix2, iz2, nlines, tw, tile, sm, slot);
+            return;
+        }
+    }
+    xfwd_mr_body<N, G, NT, TP>(p, f, qk, bands, S, lsum, nz, ny, ix2, iz2, nlines, tw, tile, sm, slot);
 }
 
 // Halfcomplex pairs of lines -> LDS complex lines ready for the inverse FFT

@@ -46,6 +46,20 @@ template <typename T>
 void part_launch(int variant, T* S, const double* ax, const double* az, const double* bands, int nx, int nz,
                  int ny, double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st) {
 #define DNSP_PV(P, M, OCC, TP) part_launch_pm<P, M, T, OCC, TP>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st)
+    if constexpr (sizeof(T) == 4) {
+        // FP32 rows: 32 modes make a 128-byte row segment
+        switch (variant) {
+            case 0: DNSP_PV(32, 16, 4, false); break;
+            case 1: DNSP_PV(32, 8, 1, false); break;
+            case 2: DNSP_PV(32, 32, 1, false); break;
+            case 3: DNSP_PV(16, 32, 1, false); break;
+            case 4: DNSP_PV(32, 16, 4, true); break;
+            case 5: DNSP_PV(32, 8, 1, true); break;
+            case 6: DNSP_PV(32, 32, 1, true); break;
+            default: DNSP_PV(16, 32, 1, true); break;
+        }
+        return;
+    }
     switch (variant) {
         case 0: DNSP_PV(32, 16, 4, false); break;
         case 1: DNSP_PV(32, 8, 1, false); break;
