@@ -288,9 +288,41 @@ __device__ __forceinline__ void zbwd_body(T* S, int nx, int ntc, const typename 
     const int k = tile / ntc;
     const int r0 = (tile - k * ntc) * W;
     T* Sk = S + (size_t)k * nx * N;
-    // (element-wise loads here: measured faster than pairwise ones for this
-    // gather of rows q and N - q)
+    // FP64: element-wise loads (measured faster than pairwise ones for this
+    // gather of rows q and N - q).  FP32 with even nx and an aligned array:
+    // one 8-byte (u, v) pair per row, from clamped addresses (unconditional
+    // loads issue together), masked by exact 0/1 factors afterwards.
     T ur[NI], ui[NI], vr[NI], vi[NI];
+    const bool pairs = sizeof(T) == 4 && (nx & 1) == 0 && ((size_t)S & (2 * sizeof(T) - 1)) == 0;
+    if (pairs) {
+        using V = typename Cx<T>::V;
+        V pu[NI], pi[NI];
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            const bool ok = e < H * TA;
+            const int ee = ok ? e : 0;
+            const int tt = ee % TA;
+            const int qq = ee / TA;
+            const int ru = r0 + 2 * tt < nx ? r0 + 2 * tt : r0;
+            const int qm = (qq == 0 || 2 * qq == N) ? qq : N - qq;
+            pu[q] = *reinterpret_cast<const V*>(Sk + (size_t)qq * nx + ru);
+            pi[q] = *reinterpret_cast<const V*>(Sk + (size_t)qm * nx + ru);
+        }
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            const int tt = e % TA;
+            const int qq = e / TA;
+            const bool single = (qq == 0) || (2 * qq == N);
+            const T m = (e < H * TA && r0 + 2 * tt < nx) ? T(1) : T(0);
+            const T mi = single ? T(0) : m;
+            ur[q] = pu[q].x * m;
+            vr[q] = pu[q].y * m;
+            ui[q] = pi[q].x * mi;
+            vi[q] = pi[q].y * mi;
+        }
+    } else {
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
         const int e = (int)threadIdx.x + q * NT;
@@ -304,6 +336,7 @@ __device__ __forceinline__ void zbwd_body(T* S, int nx, int ntc, const typename 
         vr[q] = hv ? Sk[(size_t)qq * nx + rv] : T(0);
         ui[q] = (hu && !single) ? Sk[(size_t)(N - qq) * nx + ru] : T(0);
         vi[q] = (hv && !single) ? Sk[(size_t)(N - qq) * nx + rv] : T(0);
+    }
     }
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
