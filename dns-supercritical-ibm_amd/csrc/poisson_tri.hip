@@ -47,27 +47,37 @@ void part_launch(int variant, T* S, const double* ax, const double* az, const do
                  int ny, double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st) {
 #define DNSP_PV(P, M, OCC, TP) part_launch_pm<P, M, T, OCC, TP>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st)
     if constexpr (sizeof(T) == 4) {
-        // FP32 rows: 32 modes make a 128-byte row segment
+        // FP32 spectra: (16, 32) in place of (64, 8), whose 8-mode FP32 rows
+        // are 32-byte segments (32 modes make a 128-byte row).  16 chunks of
+        // at most 32 rows hold ny <= 512; above that (ny <= 32 * kPartChunks)
+        // the slot runs (32, 16).
+        const bool c16 = ny <= 16 * 32;
         switch (variant) {
             case 0: DNSP_PV(32, 16, 4, false); break;
             case 1: DNSP_PV(32, 8, 1, false); break;
-            case 2: DNSP_PV(32, 32, 1, false); break;
-            case 3: DNSP_PV(16, 32, 1, false); break;
+            case 2: DNSP_PV(48, 16, 1, false); break;
+            case 3:
+                if (c16) DNSP_PV(16, 32, 1, false);
+                else DNSP_PV(32, 16, 4, false);
+                break;
             case 4: DNSP_PV(32, 16, 4, true); break;
             case 5: DNSP_PV(32, 8, 1, true); break;
-            case 6: DNSP_PV(32, 32, 1, true); break;
-            default: DNSP_PV(16, 32, 1, true); break;
+            case 6: DNSP_PV(48, 16, 1, true); break;
+            default:
+                if (c16) DNSP_PV(16, 32, 1, true);
+                else DNSP_PV(32, 16, 4, true);
+                break;
         }
         return;
     }
     switch (variant) {
         case 0: DNSP_PV(32, 16, 4, false); break;
         case 1: DNSP_PV(32, 8, 1, false); break;
-        case 2: DNSP_PV(64, 4, 1, false); break;
+        case 2: DNSP_PV(48, 16, 1, false); break;
         case 3: DNSP_PV(64, 8, 1, false); break;
         case 4: DNSP_PV(32, 16, 4, true); break;
         case 5: DNSP_PV(32, 8, 1, true); break;
-        case 6: DNSP_PV(64, 4, 1, true); break;
+        case 6: DNSP_PV(48, 16, 1, true); break;
         default: DNSP_PV(64, 8, 1, true); break;
     }
 #undef DNSP_PV

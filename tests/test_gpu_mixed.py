@@ -156,6 +156,27 @@ def test_every_mixed_variant_and_the_tuner(cuda):
     s.close()
 
 
+@pytest.mark.parametrize("ny", [600, 1024])
+def test_every_tridiag_variant_tall_columns(cuda, ny):
+    """Every tridiagonal workgroup shape of both forms on columns taller than
+    16 chunks of 32 rows hold (the FP32 16-chunk slots run another shape
+    there), up to the partitioned solver's limit of 1024 rows."""
+    say("tall variants", ny)
+    c = case_grid(32, 32, ny, ny)
+    s, bands = make_solver(c)
+    f = to_dev(c["rhs"])
+    for tv in range(8):
+        s.set_variants(0, 0, tv)
+        P = to_dev(np.zeros_like(c["rhs"]))
+        s.solve(f, P)
+        assert oracle_rel_resid(to_host(P), c, bands) <= 1e-10, ("fp64", tv)
+        s.set_mixed_variants(0, 0, tv)
+        P = to_dev(np.zeros_like(c["rhs"]))
+        s.solve_mixed(f, P, 1)
+        assert oracle_rel_resid(to_host(P), c, bands) <= 1e-10, ("mixed", tv)
+    s.close()
+
+
 @pytest.mark.parametrize("nx,nz,ny", [(48, 20, 10), (7, 5, 3), (1, 1, 2), (64, 32, 64)])
 def test_device_residual_matches_oracle(cuda, nx, nz, ny):
     say("residual", nx, nz, ny)
