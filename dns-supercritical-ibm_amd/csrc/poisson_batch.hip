@@ -76,7 +76,8 @@ __global__ __launch_bounds__(ct_var_threads(V)) void kb_x(const BInst* __restric
     constexpr int NT = ct_var_threads(V);
     constexpr bool F64 = KIND == BX_F64_FWD || KIND == BX_F64_BWD;
     using VT = typename std::conditional<F64, double2, float2>::type;
-    __shared__ VT sm[G * ct_pitch(N)];
+    // the residual kind may stage 2G + 2 lines of its instance (mr_lds_f2)
+    __shared__ __align__(16) VT sm[KIND == BX_MX_RES ? mr_lds_f2<N, G, float>() : G * ct_pitch(N)];
     __shared__ double slot[F64 ? 1 : line_slots<N, G>()];
     const long task = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const int u = find_inst(start, cnt, task);
@@ -89,8 +90,18 @@ __global__ __launch_bounds__(ct_var_threads(V)) void kb_x(const BInst* __restric
     } else if constexpr (KIND == BX_MX_FWD) {
         xfwd_mf_body<N, G, NT>(b.f, b.A, b.lsA, b.nlines, b.twx32, tile, sm, slot);
     } else if constexpr (KIND == BX_MX_RES) {
-        xfwd_mr_body<N, G, NT, float>(b.A, b.f, b.small + 3 * b.ny, b.bands, b.B, b.lsB, b.nz, b.ny, b.ix2, b.iz2,
-                                      b.nlines, b.twx32, tile, sm, slot);
+        // same dispatch as kt_xfwd_mr (the instance's nz is workgroup-uniform)
+        bool done = false;
+        if constexpr (mr_staged<N, G, float>()) {
+            if (b.nz % (2 * G) == 0) {
+                xfwd_mr_staged<N, G, NT, float>(b.A, b.f, b.small + 3 * b.ny, b.bands, b.B, b.lsB, b.nz, b.ny, b.ix2,
+                                                b.iz2, b.nlines, b.twx32, tile, sm, slot);
+                done = true;
+            }
+        }
+        if (!done)
+            xfwd_mr_body<N, G, NT, float>(b.A, b.f, b.small + 3 * b.ny, b.bands, b.B, b.lsB, b.nz, b.ny, b.ix2,
+                                          b.iz2, b.nlines, b.twx32, tile, sm, slot);
     } else if constexpr (KIND == BX_MX_BWD) {
         xbwd_mf_body<N, G, NT>(b.A, b.A, b.lsA, b.nlines, b.twx32, (float)b.inv_nxnz, tile, sm, slot);
     } else if constexpr (KIND == BX_MX_ACC) {
