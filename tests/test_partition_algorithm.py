@@ -14,7 +14,8 @@ Cyclic reduction can lose digits on nearly singular (low |lambda|) modes, so
 the kernel switches to the serial reduced solve below
 |lambda| = min_k(a_k + c_k).  This test shows, on the channel grids the
 benchmarks use, that the hybrid keeps the residual within a small factor of a
-serial Thomas solve of the whole column, for 32 and 64 chunks.
+serial Thomas solve of the whole column, for 16, 32, 48 and 64 chunks (48:
+cyclic reduction on a chunk count that is not a power of two).
 """
 import numpy as np
 import pytest
@@ -126,7 +127,7 @@ def partitioned(a, b, c, d, MR, P, reduced, two_pass=False):
 
 
 @pytest.mark.parametrize("two_pass", [False, True])
-@pytest.mark.parametrize("chunks", [32, 64])
+@pytest.mark.parametrize("chunks", [16, 32, 48, 64])
 @pytest.mark.parametrize("grid", ["config1", "config2"])
 def test_hybrid_reduced_solve_matches_serial_thomas(grid, chunks, two_pass):
     yu = W.config1()["yu"] if grid == "config1" else W.config2_grid()["yu"]
@@ -152,7 +153,8 @@ def test_partitioned_solution_matches_dense_solve():
     # rows, the reduced solve) against a dense LU solve on a small system,
     # including padded chunks (ny not a multiple of the chunk count)
     rng = np.random.default_rng(3)
-    for ny, MR, P in [(37, 2, 32), (64, 2, 32), (100, 4, 32), (97, 2, 64), (384, 12, 32)]:
+    for ny, MR, P in [(37, 2, 32), (64, 2, 32), (100, 4, 32), (97, 2, 64), (384, 12, 32), (384, 8, 48),
+                      (300, 8, 48), (41, 2, 48), (384, 24, 16), (500, 32, 16)]:
         a = rng.uniform(0.5, 1.5, ny)
         c = rng.uniform(0.5, 1.5, ny)
         a[0] = 0.0
