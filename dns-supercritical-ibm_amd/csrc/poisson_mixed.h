@@ -401,8 +401,8 @@ __device__ __forceinline__ void xbwd_load_f32(float2* sm, const float* S, long l
     float xr[NI], xi[NI], yr[NI], yi[NI];
     // 16-byte aligned tile (every supported length is a multiple of 4): the
     // 2G lines are staged in LDS (inside sm, which they fit) with contiguous
-    // 16-byte loads from clamped lines, and the t / N - t gather reads LDS;
-    // 4-byte global gathers ran the FP32 kernels at 3.5 TB/s.
+    // 16-byte loads from clamped lines, and the t / N - t gather reads LDS
+    // (a few percent faster than 4-byte global gathers at N = 768).
     bool staged = false;
     if constexpr (N % 4 == 0) {
         if ((((size_t)src) & 15) == 0) {
