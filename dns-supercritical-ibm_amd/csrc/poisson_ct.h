@@ -288,12 +288,12 @@ __device__ __forceinline__ void zbwd_body(T* S, int nx, int ntc, const typename 
     const int k = tile / ntc;
     const int r0 = (tile - k * ntc) * W;
     T* Sk = S + (size_t)k * nx * N;
-    // FP64: element-wise loads (measured faster than pairwise ones for this
-    // gather of rows q and N - q).  FP32 with even nx and an aligned array:
-    // one 8-byte (u, v) pair per row, from clamped addresses (unconditional
-    // loads issue together), masked by exact 0/1 factors afterwards.
+    // Even nx and an aligned array: one (u, v) pair per row (8 or 16 bytes)
+    // from clamped addresses (unconditional loads issue together), masked by
+    // exact 0/1 factors afterwards; else element-wise loads.  (Paired loads
+    // under lane branches had measured slower than element-wise ones.)
     T ur[NI], ui[NI], vr[NI], vi[NI];
-    const bool pairs = sizeof(T) == 4 && (nx & 1) == 0 && ((size_t)S & (2 * sizeof(T) - 1)) == 0;
+    const bool pairs = (nx & 1) == 0 && ((size_t)S & (2 * sizeof(T) - 1)) == 0;
     if (pairs) {
         using V = typename Cx<T>::V;
         V pu[NI], pi[NI];
