@@ -236,3 +236,152 @@ def test_ibm_part2_columns():
             assert np.all(col[:bot[j, i] + 1] == 1.5)
             assert np.all(col[top[j, i]:] == -2.5)
             assert np.array_equal(col[bot[j, i] + 1:top[j, i]], A[1:-1, j + 1, i + 1][bot[j, i] + 1:top[j, i]])
+
+
+# ---------------------------------------------------------------------------
+# Variable-density projection identity
+# ---------------------------------------------------------------------------
+def _periodic_neumann(A):
+    """Periodic x/z halos and Neumann y ghosts (every padded plane)."""
+    PJ.halo_periodic_xz(A)
+    return PJ.neumann_ghosts_y(A)
+
+
+def _varprop_case(nx, nz, ny, seed, h=0.2, rho_lo=0.1):
+    rng = np.random.default_rng(seed)
+    yu = uniform_faces(ny, h)
+    g = PJ.geom_coeffs(yu)
+    Uh, Wh, Vh, div_now = (rand_halo(rng, nx, nz, ny) for _ in range(4))
+    for A in (Uh, Wh, Vh):
+        PJ.halo_periodic_xz(A)
+    PJ.set_val_slice(Vh, 1, 1, 0.0)    # V_hat(:,:,0) = wall
+    PJ.set_val_slice(Vh, -1, -1, 0.0)  # V_hat(:,:,ny) = wall
+    rho = _periodic_neumann(rng.uniform(rho_lo, 1.0, (ny + 2, nz + 2, nx + 2)))
+    Pn = _periodic_neumann(rand_halo(rng, nx, nz, ny))
+    Po = _periodic_neumann(rand_halo(rng, nx, nz, ny))
+    return dict(yu=yu, g=g, Uh=Uh, Wh=Wh, Vh=Vh, div_now=div_now, rho=rho, Pn=Pn, Po=Po)
+
+
+def _projected_divergence(c, inv_dx, inv_dz, dt, rhs_rho=None, corr_rho=None):
+    """build_p_rhs_varprop -> solve (mean removed) -> update_vars, and the
+    divergence of the corrected velocity; rhs_rho / corr_rho replace rho in one
+    of the two stencils (planted mistakes)."""
+    g = c["g"]
+    rho = c["rho"]
+    rm = PJ.rho_min(rho)
+    f = PJ.build_p_rhs_varprop(c["Uh"], c["Wh"], c["Vh"], c["Pn"], c["Po"],
+                               rho if rhs_rho is None else rhs_rho, c["div_now"], g["inv_dy_p"],
+                               g["inv_dy_p_pad"], 1 / dt, inv_dx, inv_dz, inv_dx ** 2, inv_dz ** 2, rm)
+    fbar = f.mean()  # uniform y grid: the compatibility weights are uniform
+    p = O.solve(f - fbar, inv_dx, inv_dz, O.bands_from_yu(c["yu"]))
+    z = np.zeros_like(c["Uh"])
+    U, V, W_, _, _, _ = PJ.update_vars(p, z, z, z, c["Uh"], c["Vh"], c["Wh"], g["dPdy_ddy_hi"],
+                                       g["dPdy_ddy_lo"], dt, inv_dx, inv_dz, 1, walls0(),
+                                       use_incompressible=False, P_now=c["Pn"], P_old=c["Po"],
+                                       rho=rho if corr_rho is None else corr_rho, rho_min_v=rm)
+    div = PJ.build_p_rhs_incomp(U, W_, V, g["inv_dy_p"], 1.0, inv_dx, inv_dz)
+    return div, fbar, rm
+
+
+@pytest.mark.parametrize("nx,nz,ny,seed", [(12, 10, 9, 0), (8, 6, 14, 1), (16, 4, 6, 2)])
+def test_variable_density_projection_identity(nx, nz, ny, seed):
+    """With rho varying by 10x, the variable-density splitting makes the
+    projected velocity satisfy div(u) = div_now + dt*mean(f)/rho_min exactly:
+    the RHS face factors 1 - rho_min*(1/(2 rho_nb) + 1/(2 rho))
+    (mod_cfd_subroutines.f90:552-569) cancel the correction's
+    (<1/rho> - 1/rho_min)(2 P_now - P_old) term (:1154-1197) face by face, and
+    div(grad p) is the Neumann band operator the solve inverts."""
+    inv_dx, inv_dz, dt = 4.0, 3.0, 0.01
+    c = _varprop_case(nx, nz, ny, seed)
+    assert c["rho"][1:-1, 1:-1, 1:-1].max() / PJ.rho_min(c["rho"]) > 5
+    div, fbar, rm = _projected_divergence(c, inv_dx, inv_dz, dt)
+    want = c["div_now"][1:-1, 1:-1, 1:-1] + dt * fbar / rm
+    scale = np.abs(want).max()
+    err = np.abs(div - want).max() / scale
+    assert err <= 1e-12, err
+    # planted mistakes: rho one cell off in the RHS only, or in the correction
+    # only, must break the identity by orders of magnitude
+    for axis in (0, 1, 2):
+        shifted = _periodic_neumann(np.roll(c["rho"], 1, axis=axis).copy())
+        for kw in ({"rhs_rho": shifted}, {"corr_rho": shifted}):
+            d2, _, _ = _projected_divergence(c, inv_dx, inv_dz, dt, **kw)
+            assert np.abs(d2 - want).max() / scale > 1e-3, (axis, list(kw))
+
+
+def test_variable_density_identity_needs_both_terms():
+    """Dropping the (2 P_now - P_old) history from both stencils (P_now = P_old
+    = 0) still satisfies the identity, while a nonzero history that only one
+    stencil sees does not: the two terms are each other's exact counterpart."""
+    inv_dx, inv_dz, dt = 5.0, 2.0, 0.02
+    c = _varprop_case(10, 8, 7, 3)
+    c0 = dict(c, Pn=np.zeros_like(c["Pn"]), Po=np.zeros_like(c["Po"]))
+    div, fbar, rm = _projected_divergence(c0, inv_dx, inv_dz, dt)
+    want = c["div_now"][1:-1, 1:-1, 1:-1] + dt * fbar / rm
+    assert np.abs(div - want).max() <= 1e-12 * np.abs(want).max()
+    # history in the correction only
+    g = c["g"]
+    f = PJ.build_p_rhs_varprop(c["Uh"], c["Wh"], c["Vh"], c0["Pn"], c0["Po"], c["rho"], c["div_now"],
+                               g["inv_dy_p"], g["inv_dy_p_pad"], 1 / dt, inv_dx, inv_dz, inv_dx ** 2,
+                               inv_dz ** 2, rm)
+    p = O.solve(f - f.mean(), inv_dx, inv_dz, O.bands_from_yu(c["yu"]))
+    z = np.zeros_like(c["Uh"])
+    U, V, W_, _, _, _ = PJ.update_vars(p, z, z, z, c["Uh"], c["Vh"], c["Wh"], g["dPdy_ddy_hi"],
+                                       g["dPdy_ddy_lo"], dt, inv_dx, inv_dz, 1, walls0(),
+                                       use_incompressible=False, P_now=c["Pn"], P_old=c["Po"], rho=c["rho"],
+                                       rho_min_v=rm)
+    d2 = PJ.build_p_rhs_incomp(U, W_, V, g["inv_dy_p"], 1.0, inv_dx, inv_dz)
+    want2 = c["div_now"][1:-1, 1:-1, 1:-1] + dt * f.mean() / rm
+    assert np.abs(d2 - want2).max() > 1e-3 * np.abs(want2).max()
+
+
+# ---------------------------------------------------------------------------
+# IBM div coefficients: linear extrapolation along the image line
+# ---------------------------------------------------------------------------
+def _collinear_records(nx, nz, ny):
+    """(target, a, b) triples, collinear in physical space with a between t and
+    b, including periodic wraps in x and z: vertical lines on any y grid, and
+    lines along x, z and diagonals in (x, z).  Sources lie within half a period
+    of the target, where the nearest periodic image is the one on the line."""
+    recs = []
+    for (i, j) in [(0, 0), (3, 2), (nx - 1, nz - 1)]:
+        recs.append(((i, j, 0), (i, j, 1), (i, j, 3)))        # upwards from the bottom
+        recs.append(((i, j, ny - 1), (i, j, ny - 2), (i, j, ny - 4)))
+    for k in (1, ny // 2):
+        recs.append(((nx - 1, 1, k), (0, 1, k), (1, 1, k)))                # x wrap
+        recs.append(((2, nz - 1, k), (2, 0, k), (2, 1, k)))                # z wrap
+        recs.append(((0, 0, k), (nx - 1, nz - 1, k), (nx - 2, nz - 2, k)))  # diagonal, both wrap
+        recs.append(((4, 1, k), (5, 3, k), (6, 5, k)))                    # slope 2 in (x, z)
+    return np.array(recs, dtype=np.int64)
+
+
+def test_ibm_div_coeffs_extrapolate_linear_fields():
+    """c1 f(a) + c2 f(b) = f(t) for every linear f when t, a, b are collinear
+    with a between t and b (set_ibm_bands.f90:57-77: c1 = 1 + d_ta/d_ab,
+    c2 = -d_ta/d_ab).  Swapping c1 and c2, or the distances, fails."""
+    nx, nz, ny = 9, 12, 12
+    inv_dx, inv_dz = 3.0, 2.5
+    yp = W.tanh_faces(ny, 1.6)
+    yp = 0.5 * (yp[1:] + yp[:-1])
+    ijk = _collinear_records(nx, nz, ny)
+    coeffs = {"oracle": PJ.ibm_div_coeffs(ijk, yp, nx, nz, inv_dx, inv_dz),
+              "library": DP.ibm_div_coeffs(ijk.reshape(-1, 9), yp, nx, nz, inv_dx, inv_dz)}
+    rng = np.random.default_rng(12)
+    for _ in range(4):
+        al, be, ga, de = rng.standard_normal(4)
+
+        def f(i, j, k):
+            return al * i / inv_dx + be * j / inv_dz + ga * yp[k] + de
+
+        for name, c in coeffs.items():
+            assert np.all(c[:, 0] == 0)
+            for r, rec in enumerate(ijk):
+                t, a, b = (tuple(int(v) for v in p) for p in rec)
+                # unwrap a and b to the periodic image nearest the target
+                ua = [a[0] + nx * round((t[0] - a[0]) / nx), a[1] + nz * round((t[1] - a[1]) / nz), a[2]]
+                ub = [b[0] + nx * round((t[0] - b[0]) / nx), b[1] + nz * round((t[1] - b[1]) / nz), b[2]]
+                ft, fa, fb = f(*t), f(*ua), f(*ub)
+                got = c[r, 1] * fa + c[r, 2] * fb
+                assert abs(got - ft) <= 1e-12 * (1 + abs(ft)), (name, r)
+                # planted mistake: c1 and c2 swapped
+                if abs(fa - fb) > 1e-6:
+                    assert abs(c[r, 2] * fa + c[r, 1] * fb - ft) > 1e-6
