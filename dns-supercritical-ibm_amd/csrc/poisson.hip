@@ -734,6 +734,12 @@ int dnsp_poisson_run_host(dnsp_poisson* h, double* P_host) {
     if (rc) return rc;
     DNSP_HIP_TRY(hipMemcpy(P_host, h->d_host, bytes, hipMemcpyDeviceToHost));
     DNSP_HIP_TRY(hipDeviceSynchronize());
+    const size_t n = bytes / sizeof(double);
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(P_host[i])) {
+            set_error("the solution holds NaN or Inf (first at %zu): non-finite right-hand side?", i);
+            return DNSP_ERR_NONFINITE;
+        }
     return DNSP_OK;
 }
 

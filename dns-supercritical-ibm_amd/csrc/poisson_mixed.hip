@@ -71,6 +71,20 @@ __global__ __launch_bounds__(256) void k_line_sums(const double* __restrict__ P,
     if (lane == 0) lsum[line] = acc;
 }
 
+// Per-workgroup counts of the NaN and Inf values of a (grid kResBlocks x 256).
+__global__ __launch_bounds__(256) void k_count_nonfinite(const double* __restrict__ a, long n,
+                                                         double* __restrict__ part) {
+    __shared__ unsigned int sc[4];
+    unsigned int c = 0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        c += isfinite(a[i]) ? 0u : 1u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0) sc[threadIdx.x / 64] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (double)((sc[0] + sc[1]) + (sc[2] + sc[3]));
+}
+
 // Partial sums of (f - L p)^2 and f^2: grid kResBlocks x 256, fixed order.
 // L p = periodic x, z second differences + plain y bands + q(k).
 __global__ __launch_bounds__(256) void k_resid_norm(const double* __restrict__ p, const double* __restrict__ f,
@@ -419,6 +433,15 @@ int dnsp_poisson_solve_refine(dnsp_poisson* h, const double* rhs, double* P, dou
     if (rc) return rc;
     if (refinements_done) *refinements_done = done;
     if (rel_resid) *rel_resid = rel;
+    if (!std::isfinite(rel)) {
+        set_error("mixed solve: the relative residual is %g after %d refinements (non-finite right-hand side?)", rel,
+                  done);
+        return DNSP_ERR_NONFINITE;
+    }
+    if (rel > tol) {
+        set_error("mixed solve: relative residual %.3e above tol %.3e after %d refinements", rel, tol, done);
+        return DNSP_ERR_NOT_CONVERGED;
+    }
     return DNSP_OK;
 }
 
@@ -428,7 +451,34 @@ int dnsp_poisson_residual(dnsp_poisson* h, const double* rhs, const double* P, d
     hipStream_t st = as_stream(stream);
     const long nlines = (long)h->nz * h->ny;
     k_line_sums<<<(int)((nlines + 3) / 4), 256, 0, st>>>(P, h->d_lsB, h->nx, nlines);
-    return resid_norm(h, rhs, P, h->d_lsB, rel_resid, st);
+    const int rc = resid_norm(h, rhs, P, h->d_lsB, rel_resid, st);
+    if (rc) return rc;
+    if (!std::isfinite(*rel_resid)) {
+        set_error("residual is %g: the solution or the right-hand side holds a NaN or Inf", *rel_resid);
+        return DNSP_ERR_NONFINITE;
+    }
+    return DNSP_OK;
+}
+
+int dnsp_poisson_check(dnsp_poisson* h, const double* P, long long* n_nonfinite, void* stream) {
+    clear_error();
+    DNSP_ARG_CHECK(h && P, "NULL handle or array");
+    hipStream_t st = as_stream(stream);
+    const Small sm = small_of(h);
+    const long n = (long)h->nx * h->nz * h->ny;
+    k_count_nonfinite<<<kResBlocks, 256, 0, st>>>(P, n, sm.part);
+    DNSP_LAUNCH_CHECK();
+    std::vector<double> part((size_t)kResBlocks);
+    DNSP_HIP_TRY(hipMemcpyAsync(part.data(), sm.part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+    DNSP_HIP_TRY(hipStreamSynchronize(st));
+    long long cnt = 0;
+    for (double v : part) cnt += (long long)v;
+    if (n_nonfinite) *n_nonfinite = cnt;
+    if (cnt) {
+        set_error("%lld of %ld values of the solution are NaN or Inf (non-finite right-hand side?)", cnt, n);
+        return DNSP_ERR_NONFINITE;
+    }
+    return DNSP_OK;
 }
 
 int dnsp_poisson_set_mixed_variants(dnsp_poisson* h, int xvar, int zvar, int tvar) {

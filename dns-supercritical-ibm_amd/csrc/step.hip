@@ -2,9 +2,11 @@
 // between the predictor and the next predictor (src/main.f90 time loop):
 //   rho_min (mod_cfd_subroutines.f90:470-499), Poisson right-hand side
 //   (build_vars :527-574), pressure Poisson solve (run_spec_poisson_multigpu),
-//   gradient correction update_vars (:1048-1245), IBM forcing of U, V, W
-//   (ibm_part_1, src/ibm_setup/mod_cfd_ibm.f90:10-45) and the rough-wall
-//   columns (ibm_part_2 :47-141).
+//   gradient correction update_vars (:1048-1245), then on rough walls the
+//   part of the next build_vars' opening that acts on U, V, W (:98-113): the
+//   rough-wall columns (ibm_part_2, src/ibm_setup/mod_cfd_ibm.f90:47-141)
+//   with cfd_periodicity where column slabs are set, then the IBM forcing
+//   and wall planes (ibm_part_1 :10-45).
 // The object owns every field at its reference layout on the device and
 // enqueues the kernels of the existing entry points on one stream; it can
 // record the sequence once into a graph and replay it.  Nothing in the
@@ -91,16 +93,41 @@ int enqueue(dnsp_step* s, int refinements, hipStream_t st) {
                           F[DNSP_STEP_RHO], hi, lo, s->d_rho_min, s->d_walls, nx, nz, ny, 1, s->dtmax, s->inv_dx,
                           s->inv_dz, s->incompressible, vs);
     if (rc) return rc;
+    // Rough walls: the opening of the next build_vars on U, V, W
+    // (mod_cfd_subroutines.f90:98-113), so the step ends on the velocities the
+    // next predictor reads.  With column slabs (the quadratic-wall ordering,
+    // use_rough_surf_quady) ibm_part_2 fills each component's columns and
+    // cfd_periodicity restores halos and walls first; then ibm_part_1 forces
+    // U, V, W and sets their wall ghost planes (mod_cfd_ibm.f90:26-38).  The
+    // forcing comes last, so no column fill overwrites a forced value.
     const int comp[3] = {DNSP_STEP_U, DNSP_STEP_V, DNSP_STEP_W};
-    for (int c = 0; c < 3 && !rc; ++c)
-        if (s->ibm[c]) rc = dnsp_ibm_exec(s->ibm[c], F[comp[c]], vs);
-    if (rc) return rc;
-    // each component's own columns; kmax = ny-1 for U, W and ny for V
-    // (set_ibm_bands.f90 / mod_cfd_ibm.f90:47-141)
-    for (int c = 0; c < 3 && !rc; ++c)
-        if (s->d_bot[c])
-            rc = dnsp_ibm_part2(F[comp[c]], s->d_bot[c], s->d_top[c], nx, nz, ny, c == 1 ? ny : ny - 1, s->walls[c],
-                                s->walls[3 + c], vs);
+    const bool columns = s->d_bot[0] || s->d_bot[1] || s->d_bot[2];
+    const bool forcing = s->ibm[0] || s->ibm[1] || s->ibm[2];
+    if (columns) {
+        // each component's own columns; kmax = ny-1 for U, W and ny for V
+        // (mod_cfd_ibm.f90:47-141)
+        for (int c = 0; c < 3 && !rc; ++c)
+            if (s->d_bot[c])
+                rc = dnsp_ibm_part2(F[comp[c]], s->d_bot[c], s->d_top[c], nx, nz, ny, c == 1 ? ny : ny - 1,
+                                    s->walls[c], s->walls[3 + c], vs);
+        // cfd_periodicity (:860-899): halos, then U, W at k = -1, V at k = -1
+        // and 0, and all three at k = ny take the wall values
+        for (int c = 0; c < 3 && !rc; ++c) rc = dnsp_halo_periodic_xz(F[comp[c]], nx, nz, ny, vs);
+        for (int c = 0; c < 3 && !rc; ++c) {
+            rc = dnsp_set_val_slice(F[comp[c]], nx, nz, ny, 0, c == 1 ? 1 : 0, s->walls[c], vs);
+            if (!rc) rc = dnsp_set_val_slice(F[comp[c]], nx, nz, ny, -1, -1, s->walls[3 + c], vs);
+        }
+        if (rc) return rc;
+    }
+    if (forcing) {
+        for (int c = 0; c < 3 && !rc; ++c)
+            if (s->ibm[c]) rc = dnsp_ibm_exec(s->ibm[c], F[comp[c]], vs);
+        // ibm_part_1's wall planes: k = -1 and k = ny of U, V, W
+        for (int c = 0; c < 3 && !rc; ++c) {
+            rc = dnsp_set_val_slice(F[comp[c]], nx, nz, ny, 0, 0, s->walls[c], vs);
+            if (!rc) rc = dnsp_set_val_slice(F[comp[c]], nx, nz, ny, -1, -1, s->walls[3 + c], vs);
+        }
+    }
     return rc;
 }
 

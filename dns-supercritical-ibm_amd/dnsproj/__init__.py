@@ -25,6 +25,8 @@ DNSP_ERR_HIP = 2
 DNSP_ERR_NOMEM = 3
 DNSP_ERR_SINGULAR = 4
 DNSP_ERR_UNSUPPORTED = 5
+DNSP_ERR_NOT_CONVERGED = 6
+DNSP_ERR_NONFINITE = 7
 
 FFT_AUTO, FFT_GENERIC, FFT_FIXED = 0, 1, 2
 TRIDIAG_AUTO, TRIDIAG_SERIAL, TRIDIAG_PARTITION = 0, 1, 2
@@ -77,6 +79,7 @@ _SIGNATURES = [
     ("dnsp_poisson_run_mixed", [_vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     ("dnsp_poisson_solve_refine", [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _ip, _dp, _vp], ctypes.c_int),
     ("dnsp_poisson_residual", [_vp, _vp, _vp, _dp, _vp], ctypes.c_int),
+    ("dnsp_poisson_check", [_vp, _vp, ctypes.POINTER(ctypes.c_longlong), _vp], ctypes.c_int),
     ("dnsp_batch_create", [ctypes.c_int, _ip, _ip, _ip, _dp, _dp, _dp, ctypes.c_int, ctypes.POINTER(_vp)],
      ctypes.c_int),
     ("dnsp_batch_bind", [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], ctypes.c_int),
@@ -255,7 +258,9 @@ class PoissonSolver:
         return self.solve_mixed(P, P, refinements, stream)
 
     def solve_refine(self, rhs, P, tol=1e-10, max_refinements=4, stream=None):
-        """Refines until ||rhs - L P||/||rhs|| <= tol; returns (refinements done, relative residual)."""
+        """Refines until ||rhs - L P||/||rhs|| <= tol; returns (refinements done, relative residual).
+        Raises DnspError with code DNSP_ERR_NOT_CONVERGED when tol is not met
+        after max_refinements, DNSP_ERR_NONFINITE for a NaN or Inf residual."""
         done, rel = ctypes.c_int(), ctypes.c_double()
         _check(lib().dnsp_poisson_solve_refine(self._h, _vp(_device_ptr(rhs, self.numel)),
                                                _vp(_device_ptr(P, self.numel)), float(tol),
@@ -270,6 +275,14 @@ class PoissonSolver:
                                            _vp(_device_ptr(P, self.numel)), ctypes.byref(rel),
                                            _vp(_stream_ptr(stream))))
         return rel.value
+
+    def check(self, P, stream=None):
+        """Number of NaN/Inf values in the device solution P (0, or raises
+        DnspError with code DNSP_ERR_NONFINITE); synchronises the stream."""
+        cnt = ctypes.c_longlong()
+        _check(lib().dnsp_poisson_check(self._h, _vp(_device_ptr(P, self.numel)), ctypes.byref(cnt),
+                                        _vp(_stream_ptr(stream))))
+        return cnt.value
 
     @property
     def min_pivot(self):

@@ -26,6 +26,9 @@ extern "C" {
 #define DNSP_ERR_NOMEM        3  /* device allocation failed */
 #define DNSP_ERR_SINGULAR     4  /* a zero or non-finite tridiagonal pivot */
 #define DNSP_ERR_UNSUPPORTED  5  /* FFT length with a prime factor above 13, or a form's size limits */
+#define DNSP_ERR_NOT_CONVERGED 6 /* refinement stopped above its tolerance (outputs still written) */
+#define DNSP_ERR_NONFINITE    7  /* NaN or Inf in a solution or residual: the right-hand side held one
+                                  * (the reference stops on a NaN bulk value, src/mod_get_bulk.f90:180-186) */
 
 typedef struct dnsp_poisson dnsp_poisson;
 
@@ -80,8 +83,18 @@ int dnsp_poisson_solve(dnsp_poisson* h, const double* rhs, double* P, void* stre
 
 /* Same as dnsp_poisson_run for a host array: copies in, solves, copies out,
  * synchronises.  For callers whose P lives in host memory (the reference's
- * CPU test driver). */
+ * CPU test driver).  DNSP_ERR_NONFINITE when the solution holds a NaN or
+ * Inf (P_host is still written). */
 int dnsp_poisson_run_host(dnsp_poisson* h, double* P_host);
+
+/* Counts the non-finite values of a solution P (nx*nz*ny device doubles)
+ * after the work on `stream`: DNSP_OK when there are none, else
+ * DNSP_ERR_NONFINITE with the count in *n_nonfinite (may be NULL).  One read
+ * of P; synchronises `stream`.  A NaN or Inf anywhere in a right-hand side
+ * reaches every point of the solution (the x and z transforms spread it over
+ * a plane's modes, each mode's y solve over the column), so a solve's input
+ * need not be checked separately.  No counterpart in the reference's solver. */
+int dnsp_poisson_check(dnsp_poisson* h, const double* P, long long* n_nonfinite, void* stream);
 
 /* Stage entry points of the same solve (device arrays, in place):
  *   forward  = x R2HC, z R2HC          (poisson_solver_multigpu.f90:360-393)
@@ -171,6 +184,10 @@ int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream,
  *   dnsp_poisson_solve_refine: refines until the relative residual is <= tol
  *     or max_refinements sweeps are done; returns the sweeps done and the
  *     residual (each check reads P and rhs once more).  Synchronises.
+ *     Status DNSP_ERR_NONFINITE when the residual is NaN or Inf, and
+ *     DNSP_ERR_NOT_CONVERGED when it is still above tol after
+ *     max_refinements sweeps; *refinements_done and *rel_resid are written
+ *     in both cases.
  */
 #define DNSP_MAX_REFINEMENTS 8
 /* The form dnsp_poisson_run (and dnsp_poisson_run_host) uses: -1 = FP64
@@ -222,7 +239,8 @@ int dnsp_batch_free(dnsp_batch* b);
 /* Relative residual ||rhs - L P|| / ||rhs|| (||rhs - L P|| if rhs = 0) of the
  * discrete operator the solve inverts: periodic second differences in x and
  * z, the plain y bands, and the (0,0)-mode bands on the xz-plane means.  FP64,
- * any precision of handle.  Synchronises `stream`. */
+ * any precision of handle.  Synchronises `stream`.  DNSP_ERR_NONFINITE (with
+ * *rel_resid written) when the residual is NaN or Inf. */
 int dnsp_poisson_residual(dnsp_poisson* h, const double* rhs, const double* P, double* rel_resid, void* stream);
 
 /* min |pivot| over all modes and rows: the init diagnostic min_abs_div_bacp
@@ -413,8 +431,10 @@ int dnsp_io_read_faces(const char* path, int* ny, double* yu);
 /* fields of one channel case on the device and enqueues, on one stream:     */
 /* rho_min + the variable-density right-hand side (or the incompressible     */
 /* one), the Poisson solve (FP64, or the mixed form with 0 or 1 refinement), */
-/* update_vars (kmin_Vhat = 1), the IBM forcing of U, V, W (where a handle   */
-/* is set) and the rough-wall columns (where slabs are set).                 */
+/* update_vars (kmin_Vhat = 1), then on rough walls the opening of the next  */
+/* build_vars on U, V, W (mod_cfd_subroutines.f90:98-113): where slabs are   */
+/* set the column fill (ibm_part_2) and cfd_periodicity, then where handles  */
+/* are set the forcing and wall planes of ibm_part_1 (mod_cfd_ibm.f90:10-45). */
 /* ------------------------------------------------------------------------ */
 typedef struct dnsp_step dnsp_step;
 

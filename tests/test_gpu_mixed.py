@@ -214,6 +214,44 @@ def test_mixed_errors(cuda):
     assert e.value.code == D.DNSP_ERR_UNSUPPORTED
 
 
+def test_refine_reports_unmet_tolerance_and_nonfinite(cuda):
+    """Failure statuses of the refinement loop: a tolerance no sweep count can
+    reach gives DNSP_ERR_NOT_CONVERGED, a NaN in the right-hand side gives
+    DNSP_ERR_NONFINITE (the reference stops on NaN, mod_get_bulk.f90:180-186),
+    never DNSP_OK; the FP64 solve of the same NaN field is reported by the
+    check entry point and by the residual."""
+    say("refine failure statuses")
+    c = case_grid(64, 32, 16, 5)
+    s, _ = make_solver(c)
+    f = to_dev(c["rhs"])
+    P = to_dev(np.zeros_like(c["rhs"]))
+    with pytest.raises(D.DnspError) as e:
+        s.solve_refine(f, P, tol=0.0, max_refinements=1)
+    assert e.value.code == D.DNSP_ERR_NOT_CONVERGED
+    done, rel = s.solve_refine(f, P, tol=1e-10, max_refinements=3)
+    assert rel <= 1e-10 and s.check(P) == 0
+    bad = c["rhs"].copy()
+    bad[3, 5, 7] = np.nan
+    fb = to_dev(bad)
+    with pytest.raises(D.DnspError) as e:
+        s.solve_refine(fb, P, tol=1e-10, max_refinements=2)
+    assert e.value.code == D.DNSP_ERR_NONFINITE
+    Q = to_dev(np.zeros_like(bad))
+    s.solve(fb, Q)
+    with pytest.raises(D.DnspError) as e:
+        s.check(Q)
+    assert e.value.code == D.DNSP_ERR_NONFINITE
+    with pytest.raises(D.DnspError) as e:
+        s.residual(fb, Q)
+    assert e.value.code == D.DNSP_ERR_NONFINITE
+    inf = c["rhs"].copy()
+    inf[0, 0, 0] = np.inf
+    with pytest.raises(D.DnspError) as e:
+        s.run_host(inf)
+    assert e.value.code == D.DNSP_ERR_NONFINITE
+    s.close()
+
+
 @pytest.mark.slow
 def test_config2_mixed_full_size(cuda):
     """768 x 384 x 384: one refinement reaches 1e-10; agreement with the FP64 form."""

@@ -100,3 +100,92 @@ def test_serving_errors(cuda):
     with pytest.raises(D.DnspError):
         s.add_case(cases[0], seed=1)
     s.close()
+
+
+def config5_requests(n, seed=4):
+    """n Poisson arrivals at 200/s over the 15 config-5 cases (seed 4), the
+    largest case (768 x 384 x 384) forced into the first request."""
+    cases = W.config5_cases()
+    big = max(range(len(cases)), key=lambda i: cases[i]["nx"] * cases[i]["nz"] * cases[i]["ny"])
+    t, cid, sd = W.poisson_arrivals(n, 200.0, seed, len(cases))
+    cid[0] = big
+    return cases, big, t, cid, sd
+
+
+@pytest.mark.slow
+def test_config5_ensemble_alone_and_in_the_mix(cuda):
+    """The 15-case Ri_b x k/delta ensemble at its real grids (256x192x128 to
+    768x384x384): 24 Poisson-arrival requests, 8 slots.  Each request's
+    checksums are bit-identical alone and in the mix (tolerance zero), and the
+    mixed-precision results agree with the FP64 ones to 1e-9 relative."""
+    import torch
+    cases, big, t, cid, sd = config5_requests(24)
+    assert big in cid and len(set(cid.tolist())) >= 8
+    out = {}
+    for name, ref in (("mixed", 1), ("fp64", -1)):
+        say("config5 mix", name)
+        s = SV.Server(200 << 30, max_slots=8, precision=D.PRECISION_MIXED, refinements=ref)
+        for n, c in enumerate(cases):
+            s.add_case(c, seed=5000 + n)
+        out[name] = s.run(cid, t, sd)
+        assert np.all(np.isfinite(out[name]["checksum"]))
+        if name == "mixed":
+            say("config5 alone")
+            one = SV.Server(200 << 30, max_slots=1, precision=D.PRECISION_MIXED, refinements=ref)
+            for n, c in enumerate(cases):
+                one.add_case(c, seed=5000 + n)
+            for r in range(len(cid)):
+                a = one.run(cid[r:r + 1], [0.0], sd[r:r + 1])
+                assert np.array_equal(a["checksum"][0], out[name]["checksum"][r]), r
+            one.close()
+        s.close()
+        torch.cuda.empty_cache()
+    a, b = out["mixed"]["checksum"], out["fp64"]["checksum"]
+    assert np.all(np.abs(a - b) <= 1e-9 * np.maximum(np.abs(b), 1.0))
+
+
+@pytest.mark.slow
+def test_config5_largest_case_residual(cuda):
+    """768 x 384 x 384 with the rough wall, the mixed-precision projection
+    step (one refinement): the Poisson solution it leaves has relative
+    residual <= 1e-10 against the right-hand side the step built, evaluated
+    by dnsp_poisson_residual of the step's own solver."""
+    import ctypes
+    import torch
+    from dnsproj import projection as DP
+    cases, big, _, _, _ = config5_requests(1)
+    c = cases[big]
+    nx, nz, ny = c["nx"], c["nz"], c["ny"]
+    say("config5 largest residual", nx, nz, ny)
+    s = DP.ProjectionStep.from_case(c, precision=D.PRECISION_MIXED)
+    g = torch.Generator(device="cuda").manual_seed(77)
+    hs = s.shape("U")
+    yu = np.asarray(c["yu"])
+    yp = 0.5 * (yu[:-1] + yu[1:])
+    ypad = torch.from_numpy(np.concatenate([[2 * yu[0] - yp[0]], yp, [2 * yu[-1] - yp[-1]]])).cuda()
+    rnd = lambda: torch.randn(hs, generator=g, device="cuda", dtype=torch.float64)
+    T = (ypad / W.LY)[:, None, None] + 0.05 * rnd()
+    f = {"U_hat": 1.5 * (1.0 - (ypad - 1.0) ** 2)[:, None, None] + 0.1 * rnd(), "V_hat": 0.1 * rnd(),
+         "W_hat": 0.1 * rnd(), "rho": 1.0 - 0.45 * (1.0 + torch.tanh((T - 0.5) / 0.05)),
+         "P_now": 0.01 * rnd(), "P_old": 0.01 * rnd(), "div_now": 0.01 * rnd()}
+    del T
+    for k, v in f.items():
+        s.write(k, v)
+    gc = DP.geom_coeffs(yu)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    rm = torch.zeros(1, dtype=torch.float64, device="cuda")
+    DP.rho_min(f["rho"], rm)
+    rhs = torch.empty((ny, nz, nx), dtype=torch.float64, device="cuda")
+    DP.build_p_rhs_varprop(rhs, f["U_hat"], f["W_hat"], f["V_hat"], f["P_now"], f["P_old"], f["rho"], f["div_now"],
+                           dev(gc["inv_dy_p"]), dev(gc["inv_dy_p_pad"]), rm, 1 / c["dtmax"], c["inv_dx"], c["inv_dz"],
+                           c["inv_dx"] ** 2, c["inv_dz"] ** 2)
+    s.enqueue(1)
+    P = s.read("P")
+    torch.cuda.synchronize()
+    rel = ctypes.c_double()
+    rc = D.lib().dnsp_poisson_residual(s.poisson_handle(), ctypes.c_void_p(rhs.data_ptr()),
+                                       ctypes.c_void_p(P.data_ptr()), ctypes.byref(rel), None)
+    assert rc == D.DNSP_OK
+    say("config5 largest residual", rel.value)
+    assert rel.value <= 1e-10
+    s.close()

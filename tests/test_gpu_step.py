@@ -60,18 +60,37 @@ def oracle_step(c, incompressible=False, with_ibm=True):
         P_now=f["P_now"], P_old=f["P_old"], rho=f["rho"], rho_min_v=rm)
     out = {"U": U, "V": V, "W": Wv}
     if with_ibm and c.get("ibm") is not None:
-        for comp in ("U", "V", "W"):
-            ijk, cc = c["ibm"][comp]
-            out[comp] = PJ.ibm_exec(out[comp], ijk.reshape(-1, 3, 3), cc, nx, nz, ny, 1, 0)
-        kmax = {"U": ny - 1, "V": ny, "W": ny - 1}
-        for n, comp in enumerate(("U", "V", "W")):
-            bot, top = c["cols"][comp] if "cols" in c else (c["bot"], c["top"])
-            out[comp] = PJ.ibm_part2(out[comp], bot, top, kmax[comp], c["walls"][n], c["walls"][3 + n])
+        out.update(oracle_rough_opening(c, out))
     out.update({"P_per": P_per, "P": p, "rhs": rhs})
     if incompressible:
         out.update({"P_now": f["P_now"], "P_old": f["P_old"]})
     else:
         out.update({"P_now": P_now, "P_old": P_old})
+    return out
+
+
+def oracle_rough_opening(c, vel):
+    """The opening of build_vars on U, V, W with rough walls
+    (mod_cfd_subroutines.f90:98-113) in the quadratic-wall ordering: column
+    fill (ibm_part_2), cfd_periodicity (halos, then wall planes), then
+    ibm_part_1 (forcing, then the k = -1 and ny wall planes)."""
+    nx, nz, ny = c["nx"], c["nz"], c["ny"]
+    w = c["walls"]
+    out = dict(vel)
+    kmax = {"U": ny - 1, "V": ny, "W": ny - 1}
+    for n, comp in enumerate(("U", "V", "W")):
+        bot, top = c["cols"][comp] if "cols" in c else (c["bot"], c["top"])
+        A = PJ.ibm_part2(out[comp], bot, top, kmax[comp], w[n], w[3 + n])
+        PJ.halo_periodic_xz(A)
+        PJ.set_val_slice(A, 0, 1 if comp == "V" else 0, w[n])
+        PJ.set_val_slice(A, -1, -1, w[3 + n])
+        out[comp] = A
+    for n, comp in enumerate(("U", "V", "W")):
+        ijk, cc = c["ibm"][comp]
+        A = PJ.ibm_exec(out[comp], ijk.reshape(-1, 3, 3), cc, nx, nz, ny, 1, 0)
+        PJ.set_val_slice(A, 0, 0, w[n])
+        PJ.set_val_slice(A, -1, -1, w[3 + n])
+        out[comp] = A
     return out
 
 
@@ -157,6 +176,63 @@ def test_step_component_columns_match_oracle_chain(cuda):
     s.enqueue(-1)
     for k in OUT:
         assert rel(host(s.read(k)), ref[k]) <= 1e-11, k
+    s.close()
+
+
+def generated_rough_wall(c, seed):
+    """IBM records and column slabs of every velocity component from the
+    coefficient generator (ghost points, both walls) on the case's grid: its
+    slabs put each column's bottom at the highest target, so targets lie
+    inside the columns."""
+    nx, nz, ny = c["nx"], c["nz"], c["ny"]
+    dx, dz = W.LX / nx, W.LZ / nz
+    yu = np.asarray(c["yu"])
+    yp = 0.5 * (yu[:-1] + yu[1:])
+    xu, xp = np.arange(nx) * dx, (np.arange(nx) + 0.5) * dx
+    zu, zp = np.arange(nz) * dz, (np.arange(nz) + 0.5) * dz
+    rng = np.random.default_rng(seed)
+    mnab = [(0, 0, 0.15, 0.0)] + [(int(rng.integers(0, 2)), int(rng.integers(0, 2)), 0.004 * rng.standard_normal(),
+                                   0.004 * rng.standard_normal()) for _ in range(5)]
+    coords = {"U": (xu, zp, yp), "V": (xp, zp, yu), "W": (xp, zu, yp)}
+    ibm, cols = {}, {}
+    for n, comp in enumerate("UVW"):
+        x, z, y = coords[comp]
+        ijk, cc, sb, st = DP.ibm_generate(np.array(mnab), W.LX, W.LZ, W.LY, x, z, y,
+                                          bc_bot=c["walls"][n], bc_top=c["walls"][3 + n])
+        ibm[comp] = (ijk, cc)
+        cols[comp] = (sb, st)
+    return ibm, cols
+
+
+def test_step_generated_slabs_keep_forced_targets(cuda):
+    """Slabs and records from dnsp_ibm_generate: the step fills the columns
+    before it forces, so the forced values of targets inside the columns
+    survive and the step equals the oracle chain of build_vars' opening."""
+    say("step generated slabs")
+    c = W.projection_case(64, 32, 48, seed=21, k_rough=0.0)
+    c["walls"] = np.array([0.25, -0.5, 0.125, 1.0, 0.0, -0.75])
+    c["ibm"], c["cols"] = generated_rough_wall(c, 3)
+    ref = oracle_step(c)
+    s = DP.ProjectionStep(c["nx"], c["nz"], c["ny"], c["yu"], c["inv_dx"], c["inv_dz"], c["dtmax"])
+    s.set_walls(c["walls"])
+    s.set_ibm(DP.IbmForcingSet(c["nx"], c["nz"], c["ny"], c["ibm"]))
+    for comp in "UVW":
+        s.set_columns(*c["cols"][comp], component=comp)
+    s.load(c["fields"])
+    s.enqueue(-1)
+    got = {k: host(s.read(k)) for k in OUT}
+    for k in OUT:
+        assert rel(got[k], ref[k]) <= 1e-11, k
+    inside = forced = 0
+    for n, comp in enumerate("UVW"):
+        ijk, _ = c["ibm"][comp]
+        bot = c["cols"][comp][0]
+        for i, j, k in ijk[:, :3]:
+            if 0 <= k < c["ny"] and k <= bot[j, i]:
+                inside += 1
+                forced += got[comp][k + 1, j + 1, i + 1] != c["walls"][n]
+    assert inside > 100
+    assert forced >= 0.9 * inside, (forced, inside)
     s.close()
 
 
