@@ -34,8 +34,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-solves", type=int, default=3, help="timed oracle solves for cpu_baseline (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--cpu-solves", type=int, default=5,
+                    help="timed config-2 solves per thread count for cpu_baseline (0 = skip the CPU leg)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the all-core runs (0 = OMP_NUM_THREADS, else the CPUs this process may use)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_fp64_768x384x384.json"),
                     help="rocprofv3 FETCH/WRITE summary to report as roofline.traffic")
     ap.add_argument("--mode", default="bench", choices=["bench", "counters"],
@@ -216,6 +218,130 @@ def bench_serve(torch, D, W, n_req):
             "peak_resident_GB": round(st["peak_resident_bytes"] / 1e9, 3)}
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _stats(ts):
+    ts = np.asarray(ts, dtype=np.float64)
+    return {"median_s": round(float(np.median(ts)), 5), "p99_s": round(float(np.percentile(ts, 99)), 5),
+            "n": int(ts.size)}
+
+
+def cpu_baselines(args, c2, bands2, O):
+    """The CPU baseline: the oracle's source built for speed
+    (oracle/libdnsp_baseline.so: -O3 -march=x86-64-v3, two real lines per
+    complex FFT, blocked z and tridiagonal loops; OpenMP), timed on the host
+    cores after the GPU work.  Config 2 on all cores and on one core (1
+    warm-up + cpu_solves timed each), config 1 (1000 / 200 solves), config 3
+    (the 48 instances one after another), config 4 (the projection step:
+    numpy stencils on one thread + the C solve)."""
+    B = O.baseline_lib()
+    cores = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
+
+    def solves(f, inv_dx, inv_dz, bands, threads, n, warm=1):
+        O.set_threads(threads, B)
+        w = f.copy()
+        ts = []
+        for it in range(warm + n):
+            w[...] = f
+            a = time.perf_counter()
+            O.solve_inplace(w, inv_dx, inv_dz, bands, B)
+            if it >= warm:
+                ts.append(time.perf_counter() - a)
+        return ts, w
+
+    out = {"host": {"model": _cpu_model(), "os_cpu_count": os.cpu_count(),
+                    "affinity_cpus": len(os.sched_getaffinity(0))}}
+    ts, w = solves(c2["rhs"], c2["inv_dx"], c2["inv_dz"], bands2, cores, args.cpu_solves)
+    r = O.apply_operator(w, c2["inv_dx"], c2["inv_dz"], bands2) - c2["rhs"]
+    st = _stats(ts)
+    out.update({"value": round(1.0 / st["median_s"], 4), "unit": "solves/s", "cores": cores, "kind": "port",
+                "sample": f"{args.cpu_solves} full 768x384x384 FP64 solves of the bench RHS after 1 warm-up, "
+                          f"median (oracle source built -O3 -march=x86-64-v3, OpenMP {cores} threads)",
+                "config2": dict(st, rel_residual_maxabs=float(np.abs(r).max() / np.abs(c2["rhs"]).max()))})
+    del r, w
+    ts1, _ = solves(c2["rhs"], c2["inv_dx"], c2["inv_dz"], bands2, 1, args.cpu_solves, warm=0)
+    out["config2_one_core"] = dict(_stats(ts1), solves_per_s=round(1.0 / float(np.median(ts1)), 4))
+    from dnsproj import workloads as W
+    c1 = W.config1()
+    b1 = O.bands_from_yu(c1["yu"])
+    for name, th, n in (("config1", cores, 1000), ("config1_one_core", 1, 200)):
+        t1, _ = solves(c1["rhs"], c1["inv_dx"], c1["inv_dz"], b1, th, n)
+        out[name] = dict(_stats(t1), solves_per_s=round(1.0 / float(np.median(t1)), 2), threads=th)
+    cases = W.config3()
+    O.set_threads(cores, B)
+    for cc in cases:
+        cc["bands"] = O.bands_from_yu(cc["yu"])
+    passes = []
+    for it in range(3):
+        works = [cc["rhs"].copy() for cc in cases]
+        a = time.perf_counter()
+        for cc, wk in zip(cases, works):
+            O.solve_inplace(wk, cc["inv_dx"], cc["inv_dz"], cc["bands"], B)
+        if it:
+            passes.append(time.perf_counter() - a)
+    st3 = _stats(passes)
+    out["config3"] = dict(st3, instances=len(cases), solves_per_s=round(len(cases) / st3["median_s"], 3),
+                          note="48 instances solved one after another per pass; 1 warm-up pass + 2 timed")
+    del cases, works
+    out["config4"] = cpu_projection_step(O, B, cores)
+    return out
+
+
+def cpu_projection_step(O, B, cores):
+    """384x256x192 variable-density projection step with the rough wall on the
+    CPU: the oracle's numpy stencils (one thread) and the baseline C solve."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import projection as PJ
+    from dnsproj import workloads as W
+    c = W.config4()
+    f = c["fields"]
+    nx, nz, ny = c["nx"], c["nz"], c["ny"]
+    g = PJ.geom_coeffs(c["yu"])
+    bands = O.bands_from_yu(c["yu"])
+    dt, idx, idz = c["dtmax"], c["inv_dx"], c["inv_dz"]
+    zeros = np.zeros((ny + 2, nz + 2, nx + 2))
+    walls = dict(zip(PJ.WALL_KEYS, c["walls"]))
+    O.set_threads(cores, B)
+    parts = {"rhs": [], "solve": [], "update_vars": [], "rough_wall": []}
+    for _ in range(2):
+        t0 = time.perf_counter()
+        rm = PJ.rho_min(f["rho"])
+        rhs = PJ.build_p_rhs_varprop(f["U_hat"], f["W_hat"], f["V_hat"], f["P_now"], f["P_old"], f["rho"],
+                                     f["div_now"], g["inv_dy_p"], g["inv_dy_p_pad"], 1 / dt, idx, idz, idx ** 2,
+                                     idz ** 2, rm)
+        t1 = time.perf_counter()
+        O.solve_inplace(rhs, idx, idz, bands, B)
+        t2 = time.perf_counter()
+        vel = PJ.update_vars(rhs, zeros, zeros, zeros, f["U_hat"], f["V_hat"], f["W_hat"], g["dPdy_ddy_hi"],
+                             g["dPdy_ddy_lo"], dt, idx, idz, 1, walls, use_incompressible=False,
+                             P_now=f["P_now"], P_old=f["P_old"], rho=f["rho"], rho_min_v=rm)
+        t3 = time.perf_counter()
+        out = {"U": vel[0], "V": vel[1], "W": vel[2]}
+        for n, comp in enumerate(("U", "V", "W")):  # build_vars' opening, quadratic-wall order
+            A = PJ.ibm_part2(out[comp], c["bot"], c["top"], ny if comp == "V" else ny - 1, walls[PJ.WALL_KEYS[n]],
+                             walls[PJ.WALL_KEYS[3 + n]])
+            PJ.halo_periodic_xz(A)
+            ijk, cc = c["ibm"][comp]
+            A = PJ.ibm_exec(A, ijk, cc, nx, nz, ny, 1, 0)
+            PJ.set_val_slice(A, 0, 0, walls[PJ.WALL_KEYS[n]])
+            out[comp] = PJ.set_val_slice(A, -1, -1, walls[PJ.WALL_KEYS[3 + n]])
+        t4 = time.perf_counter()
+        for k, v in zip(parts, (t1 - t0, t2 - t1, t3 - t2, t4 - t3)):
+            parts[k].append(v)
+    tot = [sum(v[i] for v in parts.values()) for i in range(2)]
+    return {"step_s": round(float(np.median(tot)), 4), "steps": 2, "threads_solve": cores, "threads_stencils": 1,
+            "parts_s": {k: round(float(np.median(v)), 4) for k, v in parts.items()},
+            "note": "numpy stencils of the oracle (one thread) + the baseline C solve; median of 2 steps"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -334,21 +460,7 @@ def main():
 
     cpu = None
     if args.cpu_solves > 0:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        O.set_threads(threads)
-        work = f_host.copy()
-        O.solve_inplace(work, c["inv_dx"], c["inv_dz"], bands)  # warm-up (page faults, plans)
-        ts = []
-        for _ in range(args.cpu_solves):
-            work[...] = f_host
-            a = time.perf_counter()
-            O.solve_inplace(work, c["inv_dx"], c["inv_dz"], bands)
-            ts.append(time.perf_counter() - a)
-        cpu = {"value": round(1.0 / float(np.median(ts)), 4), "unit": "solves/s", "cores": threads,
-               "kind": "port",
-               "sample": f"{args.cpu_solves} full 768x384x384 FP64 solves of the same RHS (C oracle, "
-                         f"OpenMP {threads} threads), median"}
-
+        cpu = cpu_baselines(args, c, bands, O)
     forms = {
         "fp64": {"solves_per_s": round(K / wall, 3), "ms_per_step": round(wall / K * 1e3, 4),
                  "p50_ms": round(float(np.percentile(lat, 50)), 4),

@@ -86,8 +86,10 @@ static void orc_cfft(const orc_plan* pl, cpx* x, cpx* work, int inverse) {
         const int tstep = n / (Ns * R);
         cpx v[R];
         cpx o[R];
-        for (int j = 0; j < m; ++j) {
-            const int k = j % Ns;
+        /* j = jb Ns + k in increasing order, k = j mod Ns */
+        for (int jb = 0; jb < m / Ns; ++jb)
+        for (int k = 0; k < Ns; ++k) {
+            const int j = jb * Ns + k;
             for (int r = 0; r < R; ++r) {
                 cpx a = src[j + r * m];
                 int t = r * k * tstep; /* < n */
@@ -113,16 +115,19 @@ static void orc_cfft(const orc_plan* pl, cpx* x, cpx* work, int inverse) {
                 const int rs = n / R;
                 for (int q = 0; q < R; ++q) {
                     double sr = 0.0, si = 0.0;
+                    int qr = 0; /* (q r) mod R */
                     for (int r = 0; r < R; ++r) {
-                        int t = ((q * r) % R) * rs;
+                        int t = qr * rs;
                         double wr = pl->tw[t].re, wi = sg * pl->tw[t].im;
                         sr += v[r].re * wr - v[r].im * wi;
                         si += v[r].re * wi + v[r].im * wr;
+                        qr += q;
+                        if (qr >= R) qr -= R;
                     }
                     o[q].re = sr; o[q].im = si;
                 }
             }
-            const int idxD = (j / Ns) * Ns * R + k;
+            const int idxD = jb * Ns * R + k;
             for (int q = 0; q < R; ++q) dst[idxD + q * Ns] = o[q];
         }
         cpx* tmp = src; src = dst; dst = tmp;
@@ -152,6 +157,42 @@ static void orc_hc2r_plan(const orc_plan* pl, const double* in, double* out, cpx
     orc_cfft(pl, buf, work, 1);
     for (int t = 0; t < n; ++t) out[t] = buf[t].re;
 }
+
+#ifdef ORC_FAST_REAL
+/* Two real transforms per complex one (baseline build only; same results up
+ * to rounding): z = in0 + i in1, Z = DFT(z), X0(k) = (Z(k) + conj Z(n-k))/2,
+ * X1(k) = (Z(k) - conj Z(n-k))/(2i); halfcomplex out as orc_r2hc_plan. */
+static void orc_r2hc_pair(const orc_plan* pl, const double* in0, const double* in1, double* out0, double* out1,
+                          cpx* buf, cpx* work) {
+    const int n = pl->n;
+    for (int t = 0; t < n; ++t) { buf[t].re = in0[t]; buf[t].im = in1[t]; }
+    orc_cfft(pl, buf, work, 0);
+    for (int k = 0; k <= n / 2; ++k) {
+        const cpx a = buf[k], b = buf[(n - k) % n];
+        out0[k] = 0.5 * (a.re + b.re);
+        out1[k] = 0.5 * (a.im + b.im);
+        if (k > 0 && k < (n + 1) / 2) {
+            out0[n - k] = 0.5 * (a.im - b.im);
+            out1[n - k] = 0.5 * (b.re - a.re);
+        }
+    }
+}
+
+/* inverse: z(t) = x0(t) + i x1(t) from the two halfcomplex spectra */
+static void orc_hc2r_pair(const orc_plan* pl, const double* in0, const double* in1, double* out0, double* out1,
+                          cpx* buf, cpx* work) {
+    const int n = pl->n;
+    buf[0].re = in0[0]; buf[0].im = in1[0];
+    for (int k = 1; k < (n + 1) / 2; ++k) {
+        /* X0(k) = in0[k] + i in0[n-k], X1(k) = in1[k] + i in1[n-k]; Z = X0 + i X1 */
+        buf[k].re = in0[k] - in1[n - k];     buf[k].im = in0[n - k] + in1[k];
+        buf[n - k].re = in0[k] + in1[n - k]; buf[n - k].im = in1[k] - in0[n - k];
+    }
+    if (n % 2 == 0 && n > 0) { buf[n / 2].re = in0[n / 2]; buf[n / 2].im = in1[n / 2]; }
+    orc_cfft(pl, buf, work, 1);
+    for (int t = 0; t < n; ++t) { out0[t] = buf[t].re; out1[t] = buf[t].im; }
+}
+#endif
 
 void orc_r2hc(int n, const double* in, double* out) {
     orc_plan pl; orc_plan_init(&pl, n);
@@ -253,6 +294,51 @@ static int orc_tid(void) {
 #endif
 }
 
+/* z transforms of every (a, k) column, R2HC (inverse = 0) or HC2R (1), in
+ * place.  Columns are gathered ORC_ZB at a time (adjacent a), so each row j
+ * is read and written as one contiguous run; every column goes through the
+ * same 1-D transform as alone. */
+#define ORC_ZB 8
+static void orc_zcols(const orc_plan* pz, int nx, int nz, int ny, double* P, int inverse) {
+    const int nt = orc_nthreads();
+    const size_t per = (size_t)ORC_ZB * nz * 2 + (size_t)nz * 5;
+    double* scratch = (double*)malloc(sizeof(double) * (size_t)nt * per);
+    const int nblk = (nx + ORC_ZB - 1) / ORC_ZB;
+    const long nwork = (long)nblk * ny;
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < nwork; ++q) {
+        double* s = scratch + (size_t)orc_tid() * per;
+        double* in = s;                              /* ORC_ZB columns */
+        double* out = s + (size_t)ORC_ZB * nz;      /* ORC_ZB columns */
+        double* w = s + (size_t)ORC_ZB * nz * 2;    /* buf (2 nz) + work (2 nz) */
+        const int a0 = (int)(q % nblk) * ORC_ZB;
+        const long k = q / nblk;
+        const int nb = nx - a0 < ORC_ZB ? nx - a0 : ORC_ZB;
+        double* base = P + (size_t)k * nx * nz + a0;
+        for (int j = 0; j < nz; ++j)
+            for (int u = 0; u < nb; ++u) in[(size_t)u * nz + j] = base[(size_t)j * nx + u];
+        int u = 0;
+#ifdef ORC_FAST_REAL
+        for (; u + 1 < nb; u += 2) {
+            double *i0 = in + (size_t)u * nz, *o0 = out + (size_t)u * nz;
+            if (inverse)
+                orc_hc2r_pair(pz, i0, i0 + nz, o0, o0 + nz, (cpx*)w, (cpx*)(w + 2 * nz));
+            else
+                orc_r2hc_pair(pz, i0, i0 + nz, o0, o0 + nz, (cpx*)w, (cpx*)(w + 2 * nz));
+        }
+#endif
+        for (; u < nb; ++u) {
+            if (inverse)
+                orc_hc2r_plan(pz, in + (size_t)u * nz, out + (size_t)u * nz, (cpx*)w, (cpx*)(w + 2 * nz));
+            else
+                orc_r2hc_plan(pz, in + (size_t)u * nz, out + (size_t)u * nz, (cpx*)w, (cpx*)(w + 2 * nz));
+        }
+        for (int j = 0; j < nz; ++j)
+            for (int u = 0; u < nb; ++u) base[(size_t)j * nx + u] = out[(size_t)u * nz + j];
+    }
+    free(scratch);
+}
+
 /* S1-S3: x R2HC of every (j,k) line, then z R2HC of every (a,k) column. In place. */
 void orc_poisson_forward(int nx, int nz, int ny, double* P) {
     orc_plan px, pz;
@@ -262,25 +348,28 @@ void orc_poisson_forward(int nx, int nz, int ny, double* P) {
     const int nt = orc_nthreads();
     double* scratch = (double*)malloc(sizeof(double) * (size_t)nt * (size_t)nmax * 6);
     const long nlines_x = (long)nz * ny;
+#ifdef ORC_FAST_REAL
+    /* line pairs; the scratch holds both lines (2 nmax), buf and work (4 nmax) */
+    double* scratch2 = (double*)malloc(sizeof(double) * (size_t)nt * (size_t)nmax * 6);
+#pragma omp parallel for schedule(static)
+    for (long l = 0; l < nlines_x - 1; l += 2) {
+        double* s = scratch2 + (size_t)orc_tid() * nmax * 6;
+        double* line = P + (size_t)l * nx;
+        memcpy(s, line, sizeof(double) * (size_t)nx * 2);
+        orc_r2hc_pair(&px, s, s + nx, line, line + nx, (cpx*)(s + 2 * nmax), (cpx*)(s + 4 * nmax));
+    }
+    free(scratch2);
+    for (long l = nlines_x & ~1L; l < nlines_x; ++l) {
+#else
 #pragma omp parallel for schedule(static)
     for (long l = 0; l < nlines_x; ++l) {
+#endif
         double* s = scratch + (size_t)orc_tid() * nmax * 6;
         double* line = P + (size_t)l * nx;
         memcpy(s, line, sizeof(double) * (size_t)nx);
         orc_r2hc_plan(&px, s, line, (cpx*)(s + nmax), (cpx*)(s + 3 * nmax));
     }
-    const long ncols = (long)nx * ny;
-#pragma omp parallel for schedule(static)
-    for (long q = 0; q < ncols; ++q) {
-        double* s = scratch + (size_t)orc_tid() * nmax * 6;
-        const int a = (int)(q % nx);
-        const long k = q / nx;
-        double* col = P + (size_t)k * nx * nz + a;
-        double* tmp = s + 5 * nmax; /* holds nmax reals */
-        for (int j = 0; j < nz; ++j) s[j] = col[(size_t)j * nx];
-        orc_r2hc_plan(&pz, s, tmp, (cpx*)(s + nmax), (cpx*)(s + 3 * nmax));
-        for (int j = 0; j < nz; ++j) col[(size_t)j * nx] = tmp[j];
-    }
+    orc_zcols(&pz, nx, nz, ny, P, 0);
     free(scratch);
     orc_plan_free(&px);
     orc_plan_free(&pz);
@@ -294,22 +383,25 @@ void orc_poisson_backward(int nx, int nz, int ny, double* P) {
     const int nmax = nx > nz ? nx : nz;
     const int nt = orc_nthreads();
     double* scratch = (double*)malloc(sizeof(double) * (size_t)nt * (size_t)nmax * 6);
-    const long ncols = (long)nx * ny;
-#pragma omp parallel for schedule(static)
-    for (long q = 0; q < ncols; ++q) {
-        double* s = scratch + (size_t)orc_tid() * nmax * 6;
-        const int a = (int)(q % nx);
-        const long k = q / nx;
-        double* col = P + (size_t)k * nx * nz + a;
-        double* tmp = s + 5 * nmax;
-        for (int j = 0; j < nz; ++j) s[j] = col[(size_t)j * nx];
-        orc_hc2r_plan(&pz, s, tmp, (cpx*)(s + nmax), (cpx*)(s + 3 * nmax));
-        for (int j = 0; j < nz; ++j) col[(size_t)j * nx] = tmp[j];
-    }
+    orc_zcols(&pz, nx, nz, ny, P, 1);
     const long nlines_x = (long)nz * ny;
     const double inv_nxnz = 1.0 / ((double)nx * nz + 0.0);
+#ifdef ORC_FAST_REAL
+    double* scratch2 = (double*)malloc(sizeof(double) * (size_t)nt * (size_t)nmax * 6);
+#pragma omp parallel for schedule(static)
+    for (long l = 0; l < nlines_x - 1; l += 2) {
+        double* s = scratch2 + (size_t)orc_tid() * nmax * 6;
+        double* line = P + (size_t)l * nx;
+        memcpy(s, line, sizeof(double) * (size_t)nx * 2);
+        orc_hc2r_pair(&px, s, s + nx, line, line + nx, (cpx*)(s + 2 * nmax), (cpx*)(s + 4 * nmax));
+        for (int i = 0; i < 2 * nx; ++i) line[i] = line[i] * inv_nxnz;
+    }
+    free(scratch2);
+    for (long l = nlines_x & ~1L; l < nlines_x; ++l) {
+#else
 #pragma omp parallel for schedule(static)
     for (long l = 0; l < nlines_x; ++l) {
+#endif
         double* s = scratch + (size_t)orc_tid() * nmax * 6;
         double* line = P + (size_t)l * nx;
         memcpy(s, line, sizeof(double) * (size_t)nx);
@@ -321,6 +413,7 @@ void orc_poisson_backward(int nx, int nz, int ny, double* P) {
     orc_plan_free(&pz);
 }
 
+#define ORC_MB 8
 /* S4: per-mode Thomas, poisson_solver_multigpu.f90:162-179 and :395-422.
  * Bands are selected per mode: the *_00 set for the (0,0) mode, else the plain set. */
 void orc_poisson_tridiag(int nx, int nz, int ny, double inv_dx, double inv_dz,
@@ -335,28 +428,51 @@ void orc_poisson_tridiag(int nx, int nz, int ny, double inv_dx, double inv_dz,
     orc_eigen(nz, inv_dz, az, is0z);
     const long plane = (long)nx * nz;
     const int nt = orc_nthreads();
-    double* cpall = (double*)malloc(sizeof(double) * (size_t)nt * (size_t)ny);
+    /* modes go ORC_MB at a time (adjacent i of one j), so each row k is one
+     * contiguous run; per mode the operations and their order are those of
+     * the reference's sweep */
+    double* cpall = (double*)malloc(sizeof(double) * (size_t)nt * (size_t)ny * ORC_MB);
+    const long nblk = (plane + ORC_MB - 1) / ORC_MB;
 #pragma omp parallel for schedule(static)
-    for (long m = 0; m < plane; ++m) {
-        double* cp = cpall + (size_t)orc_tid() * ny;
-        const int i = (int)(m % nx);
-        const int j = (int)(m / nx);
-        const double a_xz = ax[i] + az[j];
-        const int is00 = is0x[i] && is0z[j];
-        const double* A = is00 ? ba00 : ba;
-        const double* B = is00 ? bb00 : bb;
-        const double* C = is00 ? bc00 : bc;
-        double* d = S + m;
+    for (long q = 0; q < nblk; ++q) {
+        double* cp = cpall + (size_t)orc_tid() * ny * ORC_MB;
+        const long m0 = q * ORC_MB;
+        const int nb = plane - m0 < ORC_MB ? (int)(plane - m0) : ORC_MB;
+        double axz[ORC_MB];
+        const double* Au[ORC_MB];
+        const double* Bu[ORC_MB];
+        const double* Cu[ORC_MB];
+        for (int u = 0; u < nb; ++u) {
+            const long m = m0 + u;
+            const int i = (int)(m % nx);
+            const int j = (int)(m / nx);
+            axz[u] = ax[i] + az[j];
+            const int is00 = is0x[i] && is0z[j];
+            Au[u] = is00 ? ba00 : ba;
+            Bu[u] = is00 ? bb00 : bb;
+            Cu[u] = is00 ? bc00 : bc;
+        }
+        double* d = S + m0;
         /* poi_C: init_spec_poisson_multigpu :167-176 */
-        cp[0] = C[0] / (B[0] + a_xz);
-        for (int k = 1; k < ny; ++k) cp[k] = C[k] / (B[k] + a_xz - A[k] * cp[k - 1]);
-        /* run_spec_poisson_multigpu :396-421 */
-        d[0] = d[0] / (B[0] + a_xz);
+        for (int u = 0; u < nb; ++u) cp[u] = Cu[u][0] / (Bu[u][0] + axz[u]);
         for (int k = 1; k < ny; ++k)
-            d[(size_t)k * plane] = (d[(size_t)k * plane] - A[k] * d[(size_t)(k - 1) * plane]) /
-                                   (B[k] + a_xz - A[k] * cp[k - 1]);
-        for (int k = ny - 2; k >= 0; --k)
-            d[(size_t)k * plane] = d[(size_t)k * plane] - d[(size_t)(k + 1) * plane] * cp[k];
+            for (int u = 0; u < nb; ++u)
+                cp[(size_t)k * ORC_MB + u] =
+                    Cu[u][k] / (Bu[u][k] + axz[u] - Au[u][k] * cp[(size_t)(k - 1) * ORC_MB + u]);
+        /* run_spec_poisson_multigpu :396-421 */
+        for (int u = 0; u < nb; ++u) d[u] = d[u] / (Bu[u][0] + axz[u]);
+        for (int k = 1; k < ny; ++k) {
+            double* dk = d + (size_t)k * plane;
+            const double* dm = d + (size_t)(k - 1) * plane;
+            for (int u = 0; u < nb; ++u)
+                dk[u] = (dk[u] - Au[u][k] * dm[u]) /
+                        (Bu[u][k] + axz[u] - Au[u][k] * cp[(size_t)(k - 1) * ORC_MB + u]);
+        }
+        for (int k = ny - 2; k >= 0; --k) {
+            double* dk = d + (size_t)k * plane;
+            const double* dp = d + (size_t)(k + 1) * plane;
+            for (int u = 0; u < nb; ++u) dk[u] = dk[u] - dp[u] * cp[(size_t)k * ORC_MB + u];
+        }
     }
     free(cpall); free(ax); free(az); free(is0x); free(is0z);
 }

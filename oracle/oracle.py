@@ -26,32 +26,51 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+_BASELINE_PATH = os.path.join(_HERE, "libdnsp_baseline.so")
+_baseline = None
+
+
+def baseline_lib():
+    """The same source built for speed (-O3 -march=x86-64-v3, contraction
+    allowed): the CPU baseline bench.py times.  Never the correctness oracle."""
+    global _baseline
+    if _baseline is None:
+        if not os.path.exists(_BASELINE_PATH):
+            build()
+        _baseline = _bind(ctypes.CDLL(_BASELINE_PATH))
+    return _baseline
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        L.orc_r2hc.argtypes = [ctypes.c_int, _dp, _dp]
-        L.orc_hc2r.argtypes = [ctypes.c_int, _dp, _dp]
-        L.orc_cfft_inplace.argtypes = [ctypes.c_int, _dp, ctypes.c_int]
-        L.orc_yp_ext.argtypes = [ctypes.c_int, _dp, _dp]
-        L.orc_mk_P_abc.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp]
-        L.orc_eigen.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _ip]
-        L.orc_get_lo_n_bounds_1d.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip]
-        L.orc_poisson_forward.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
-        L.orc_poisson_backward.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
-        bands = [_dp] * 6
-        L.orc_poisson_tridiag.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 2 + bands + [_dp]
-        L.orc_poisson_solve.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 2 + bands + [_dp]
-        L.orc_poisson_solve.restype = ctypes.c_int
-        L.orc_min_pivot.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 2 + bands
-        L.orc_min_pivot.restype = ctypes.c_double
-        L.orc_apply_operator.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 2 + bands + [_dp, _dp]
-        L.orc_set_threads.argtypes = [ctypes.c_int]
-        L.orc_get_threads.restype = ctypes.c_int
-        _lib = L
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+def _bind(L):
+    """Argument and result types of the exported functions."""
+    L.orc_r2hc.argtypes = [ctypes.c_int, _dp, _dp]
+    L.orc_hc2r.argtypes = [ctypes.c_int, _dp, _dp]
+    L.orc_cfft_inplace.argtypes = [ctypes.c_int, _dp, ctypes.c_int]
+    L.orc_yp_ext.argtypes = [ctypes.c_int, _dp, _dp]
+    L.orc_mk_P_abc.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp]
+    L.orc_eigen.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _ip]
+    L.orc_get_lo_n_bounds_1d.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip]
+    L.orc_poisson_forward.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+    L.orc_poisson_backward.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+    bands = [_dp] * 6
+    L.orc_poisson_tridiag.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 2 + bands + [_dp]
+    L.orc_poisson_solve.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 2 + bands + [_dp]
+    L.orc_poisson_solve.restype = ctypes.c_int
+    L.orc_min_pivot.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 2 + bands
+    L.orc_min_pivot.restype = ctypes.c_double
+    L.orc_apply_operator.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 2 + bands + [_dp, _dp]
+    L.orc_set_threads.argtypes = [ctypes.c_int]
+    L.orc_get_threads.restype = ctypes.c_int
+    return L
 
 
 def _p(a):
@@ -64,8 +83,8 @@ def _pi(a):
     return a.ctypes.data_as(_ip)
 
 
-def set_threads(n):
-    lib().orc_set_threads(int(n))
+def set_threads(n, L=None):
+    (L or lib()).orc_set_threads(int(n))
 
 
 def get_threads():
@@ -169,10 +188,11 @@ def solve(rhs, inv_dx, inv_dz, bands):
     return P
 
 
-def solve_inplace(P, inv_dx, inv_dz, bands):
+def solve_inplace(P, inv_dx, inv_dz, bands, L=None):
+    """In-place solve; L: the library to run (default the oracle)."""
     ny, nz, nx = P.shape
     bs = [np.ascontiguousarray(b, dtype=np.float64) for b in bands]
-    rc = lib().orc_poisson_solve(nx, nz, ny, float(inv_dx), float(inv_dz), *_bands(bs), _p(P))
+    rc = (L or lib()).orc_poisson_solve(nx, nz, ny, float(inv_dx), float(inv_dz), *_bands(bs), _p(P))
     if rc != 0:
         raise ValueError("oracle solve rejected the sizes")
 
