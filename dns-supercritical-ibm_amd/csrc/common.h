@@ -38,6 +38,11 @@ void clear_error();
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Device bytes a handle holds: every allocation it owns at the moment
+// (poisson.hip, ibm.hip).
+size_t poisson_device_bytes(const dnsp_poisson* h);
+size_t ibm_device_bytes(const dnsp_ibm* h);
+
 // A rough-wall column-slab pair (bottom, top; n ints each) on the device:
 // copies the host slabs in (allocating on first use, *bytes grows by the
 // allocation when given), or frees the pair when both are NULL.

@@ -96,6 +96,20 @@ static void ibm_release(dnsp_ibm* h) {
     delete h;
 }
 
+namespace dnsp {
+
+size_t ibm_device_bytes(const dnsp_ibm* h) {
+    if (!h) return 0;
+    size_t b = 0;
+    if (h->d_pos) b += sizeof(long long) * 3 * (size_t)h->n;
+    if (h->d_c) b += sizeof(double) * 3 * (size_t)h->n;
+    if (h->d_img) b += sizeof(long long) * 2 * (size_t)h->nimg;
+    if (h->d_vals) b += sizeof(double) * (size_t)h->n;
+    return b;
+}
+
+}  // namespace dnsp
+
 extern "C" {
 
 int dnsp_ibm_create(int nx, int nz, int ny, int nhalo, int nskip, int n, const int* ijk, const double* c,

@@ -54,6 +54,10 @@ struct BGroup {
     long total = 0;
     int* ids = nullptr;     // device, cnt
     long* start = nullptr;  // device, cnt
+    // tridiagonal groups of the mixed form: (instance slot u, mode group)
+    // pairs solved in FP64 ([0]) and in FP32 ([1]) arithmetic
+    int2* tl[2] = {nullptr, nullptr};
+    long ntl[2] = {0, 0};
 };
 
 __device__ __forceinline__ int find_inst(const long* __restrict__ start, int cnt, long task) {
@@ -169,6 +173,35 @@ __global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri(const BInst
     __syncthreads();
     tri_group<MR, P, M, T, TP>(d, S, b.ax, b.az, smb, b.nx, b.ny, b.plane, b.pcr_min, mode, msum, pbar, b.inv_nxnz,
                                red);
+}
+
+// The mixed form's tridiagonal from a task list: workgroup b solves mode
+// group tl[b].y of instance slot tl[b].x, in arithmetic Ar.  KIND 1 or 2.
+template <int MR, int KIND, int OCC, bool TP, typename Ar>
+__global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri_list(const BInst* __restrict__ inst,
+                                                                           const int* __restrict__ ids,
+                                                                           const int2* __restrict__ tl) {
+    constexpr int M = kBatchM;
+    constexpr int P = kPartChunks;
+    constexpr int NT = M * P;
+    __shared__ TriRed<P, M, Ar> red;
+    extern __shared__ double smb_raw[];
+    Ar* smb = reinterpret_cast<Ar*>(smb_raw);
+    const int2 t = tl[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
+    const BInst& b = inst[ids[t.x]];
+    const long grp = t.y;
+    const int mi = (int)threadIdx.x % M;
+    const int s = ((int)threadIdx.x / M) * MR;
+    const long mode = tri_group_mode(grp, (b.plane + M - 1) / M, M, mi);
+    tri_stage_bands<MR, P, Ar>(smb, b.bands + (grp == 0 ? 3 * b.ny : 0), b.ny, NT);
+    float* S = KIND == 1 ? b.A : b.B;
+    const double* msum = b.small;
+    double* pbar = b.small + (KIND == 1 ? 1 : 2) * b.ny;
+    Ar d[MR];
+    tri_load_rows<MR, P, M, float, Ar>(d, S, mode, b.plane, b.ny, s, msum);
+    __syncthreads();
+    tri_group<MR, P, M, float, TP, Ar>(d, S, b.ax, b.az, smb, b.nx, b.ny, b.plane, b.pcr_min, mode, msum, pbar,
+                                       b.inv_nxnz, red);
 }
 
 // q(k) of the first refinement for every plane of every instance (one
@@ -288,8 +321,24 @@ static void bt_launch_m(int kind, const BInst* inst, const BGroup& g, hipStream_
     const int nb = (int)g.total;
     constexpr int NT = kBatchM * kPartChunks;
     constexpr size_t lds = 3 * sizeof(double) * MR * kPartChunks;
-    if (kind == 0) kb_tri<MR, 0, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
-    else if (kind == 1) kb_tri<MR, 1, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
+    if (kind == 0) {
+        kb_tri<MR, 0, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
+        return;
+    }
+    if (g.tl[0]) {
+        // the mixed form's split: FP64 groups, then FP32 ones
+        constexpr size_t lds32 = 3 * sizeof(float) * MR * kPartChunks;
+        if (g.ntl[0] > 0) {
+            if (kind == 1) kb_tri_list<MR, 1, OCC, TP, double><<<(int)g.ntl[0], NT, lds, st>>>(inst, g.ids, g.tl[0]);
+            else kb_tri_list<MR, 2, OCC, TP, double><<<(int)g.ntl[0], NT, lds, st>>>(inst, g.ids, g.tl[0]);
+        }
+        if (g.ntl[1] > 0) {
+            if (kind == 1) kb_tri_list<MR, 1, OCC, TP, float><<<(int)g.ntl[1], NT, lds32, st>>>(inst, g.ids, g.tl[1]);
+            else kb_tri_list<MR, 2, OCC, TP, float><<<(int)g.ntl[1], NT, lds32, st>>>(inst, g.ids, g.tl[1]);
+        }
+        return;
+    }
+    if (kind == 1) kb_tri<MR, 1, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
     else kb_tri<MR, 2, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
 }
 
@@ -319,6 +368,9 @@ struct dnsp_batch {
     int device = 0;
     std::vector<int> nx, nz, ny, mr;
     std::vector<double> pcr_min;
+    // host copies for the mixed form's FP32/FP64 split of the tridiagonal
+    std::vector<std::vector<double>> hax, haz;
+    std::vector<double> f32_min;  // |lambda| from which a mode runs in FP32
     BInst* d_inst = nullptr;
     std::vector<BInst> h_inst;
     std::vector<void*> allocs;  // everything hipMalloc'd, freed at the end
@@ -447,6 +499,11 @@ int setup(dnsp_batch* b, const double* bands_cat) {
         const int k0 = ny > 2 ? 1 : 0, k1 = ny > 2 ? ny - 1 : ny;
         for (int k = k0; k < k1; ++k) mn = std::min(mn, std::fabs(bi[k]) + std::fabs(bi[2 * ny + k]));
         I.pcr_min = mn;
+        // the single-instance rule (poisson_mixed.hip): FP32 sweeps where
+        // |lambda| >= max_k(|a_k| + |c_k|) / 128
+        double cmax = 0.0;
+        for (int k = 0; k < ny; ++k) cmax = std::max(cmax, std::fabs(bi[k]) + std::fabs(bi[2 * ny + k]));
+        b->f32_min.push_back(cmax / 128.0);
         int rc = dev_copy(b, const_cast<double**>(&I.bands), bi, 6 * (size_t)ny);
         if (rc) return rc;
         for (int dir = 0; dir < 2; ++dir) {
@@ -488,6 +545,8 @@ int setup_eig(dnsp_batch* b, const double* inv_dx, const double* inv_dz) {
         std::vector<double> ax, az;
         eig(I.nx, inv_dx[i], ax);
         eig(I.nz, inv_dz[i], az);
+        b->hax.push_back(ax);
+        b->haz.push_back(az);
         I.ix2 = inv_dx[i] * inv_dx[i];
         I.iz2 = inv_dz[i] * inv_dz[i];
         int rc = dev_copy(b, const_cast<double**>(&I.ax), ax.data(), ax.size());
@@ -522,6 +581,38 @@ int setup_groups(dnsp_batch* b) {
     }
     int rc = make_groups(b, b->mr, tt, b->gt);
     if (rc) return rc;
+    if (b->precision == DNSP_PRECISION_MIXED) {
+        // per row-count launch group: (slot, mode group) pairs with a mode of
+        // |lambda| < f32_min (FP64; mode 0 and modes 1..M-1 among them, so
+        // tl[0] is never empty) and the rest (FP32); slots follow
+        // make_groups' ascending instance order
+        for (BGroup& g : b->gt) {
+            std::vector<int2> tl[2];
+            int u = 0;
+            for (int i = 0; i < n; ++i) {
+                if (b->mr[(size_t)i] != g.key) continue;
+                const BInst& I = b->h_inst[(size_t)i];
+                const long ng = (I.plane + kBatchM - 1) / kBatchM;
+                tl[0].push_back(make_int2(u, 0));
+                tl[0].push_back(make_int2(u, (int)ng));
+                for (long q = 1; q < ng; ++q) {
+                    bool f32 = true;
+                    for (long m = q * kBatchM; m < std::min(I.plane, (q + 1) * kBatchM) && f32; ++m)
+                        f32 = std::fabs(b->hax[(size_t)i][(size_t)(m % I.nx)] + b->haz[(size_t)i][(size_t)(m / I.nx)]) >=
+                              b->f32_min[(size_t)i];
+                    tl[f32 ? 1 : 0].push_back(make_int2(u, (int)q));
+                }
+                ++u;
+            }
+            for (int w = 0; w < 2; ++w) {
+                g.ntl[w] = (long)tl[w].size();
+                if (!tl[w].empty()) {
+                    rc = dev_copy(b, &g.tl[w], tl[w].data(), tl[w].size());
+                    if (rc) return rc;
+                }
+            }
+        }
+    }
     std::vector<BGroup> gq;
     rc = make_groups(b, zero, tq, gq);
     if (rc) return rc;

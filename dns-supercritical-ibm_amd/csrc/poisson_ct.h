@@ -418,14 +418,31 @@ __device__ __forceinline__ double rcp64(double b) {
     return r;
 }
 
+// FP32 arithmetic of the mixed form's well-conditioned modes: v_rcp_f32
+// (1 ulp) refined by one Newton step.
+__device__ __forceinline__ float rcp64(float b) {
+    float r = __builtin_amdgcn_rcpf(b);
+    r = fmaf(r, fmaf(-b, r, 1.0f), r);
+    return r;
+}
+
+// Bitwise mask of a value: all bits kept (keep) or +0.
+__device__ __forceinline__ double keep_or_zero(double v, bool keep) {
+    return __longlong_as_double(__double_as_longlong(v) & (keep ? -1LL : 0LL));
+}
+__device__ __forceinline__ float keep_or_zero(float v, bool keep) {
+    return __int_as_float(__float_as_int(v) & (keep ? -1 : 0));
+}
+
 // A workgroup solves one mode group of M modes.  OCC is the minimum number
 // of resident waves per SIMD the register allocation must allow.  T is the
 // storage type of S (double, or float for the mixed-precision form); the
-// arithmetic is FP64.  When msum is not NULL the (0,0) mode is solved from the
-// FP64 plane sums msum[k] instead of S, its solution times inv_nxnz goes to
-// pbar[k] and its S entries are set to zero.
-template <int MR, int P, int M, typename T>
-__device__ __forceinline__ void tri_load_rows(double (&d)[MR], const T* S, long mode, long plane, int ny, int s,
+// arithmetic type R is FP64, or FP32 in the mixed form's workgroups whose
+// modes are all well conditioned.  When msum is not NULL the (0,0) mode is
+// solved from the FP64 plane sums msum[k] instead of S, its solution times
+// inv_nxnz goes to pbar[k] and its S entries are set to zero.
+template <int MR, int P, int M, typename T, typename R = double>
+__device__ __forceinline__ void tri_load_rows(R (&d)[MR], const T* S, long mode, long plane, int ny, int s,
                                               const double* __restrict__ msum) {
     const bool valid = mode < plane;
     const bool mean = msum != nullptr && mode == 0;
@@ -444,22 +461,19 @@ __device__ __forceinline__ void tri_load_rows(double (&d)[MR], const T* S, long 
         raw[r] = S[(size_t)row * plane + md];
     }
 #pragma unroll
-    for (int r = 0; r < MR; ++r) {
-        const long long keep = (valid && !mean && s + r < ny) ? -1LL : 0LL;
-        d[r] = __longlong_as_double(__double_as_longlong((double)raw[r]) & keep);
-    }
+    for (int r = 0; r < MR; ++r) d[r] = keep_or_zero((R)raw[r], valid && !mean && s + r < ny);
     if (mean) {
 #pragma unroll
-        for (int r = 0; r < MR; ++r) d[r] = s + r < ny ? msum[s + r] : 0.0;
+        for (int r = 0; r < MR; ++r) d[r] = s + r < ny ? (R)msum[s + r] : R(0);
     }
 }
 
 // LDS of the reduced system: two buffers of P rows per mode, mode fastest.
 // b holds the diagonal for the serial sweep and its reciprocal for cyclic
 // reduction.  Buffer 1 first carries the s rows between neighbouring chunks.
-template <int P, int M>
+template <int P, int M, typename R = double>
 struct TriRed {
-    double a[2][P][M], b[2][P][M], c[2][P][M], d[2][P][M];
+    R a[2][P][M], b[2][P][M], c[2][P][M], d[2][P][M];
 };
 
 // Stages one band set (a, b, c of ny rows each) into LDS as three arrays of
@@ -467,12 +481,12 @@ struct TriRed {
 // above and below, with the diagonal -1 + lambda nonzero for every mode
 // (lambda <= 0), so that they solve to x = 0 from d = 0 and the band reads
 // need no bounds test.
-template <int MR, int P>
-__device__ __forceinline__ void tri_stage_bands(double* smb, const double* __restrict__ set, int ny, int nt) {
+template <int MR, int P, typename Ar = double>
+__device__ __forceinline__ void tri_stage_bands(Ar* smb, const double* __restrict__ set, int ny, int nt) {
     constexpr int NP = MR * P;
     for (int e = (int)threadIdx.x; e < 3 * NP; e += nt) {
         const int which = e / NP, row = e - which * NP;
-        smb[e] = row < ny ? set[which * ny + row] : (which == 1 ? -1.0 : 0.0);
+        smb[e] = row < ny ? (Ar)set[which * ny + row] : (which == 1 ? Ar(-1) : Ar(0));
     }
 }
 
@@ -480,7 +494,7 @@ __device__ __forceinline__ void tri_stage_bands(double* smb, const double* __res
 // and writes its solution rows (mode >= plane: an idle lane).  smb: the band
 // set of every mode of the group, staged in LDS by tri_stage_bands; R: the
 // LDS reduced system.  Contains workgroup barriers; all threads call it with
-// the same group.
+// the same group.  Ar: the arithmetic type (bands in smb, d[], R).
 //
 // TP = false: the elimination keeps the normalised rows (A^, C^, D^) of the
 // chunk interior in registers and back-substitutes them once x_s and x_e are
@@ -489,11 +503,11 @@ __device__ __forceinline__ void tri_stage_bands(double* smb, const double* __res
 // known the interior rows are solved again as a tridiagonal system with those
 // two values as boundary data.  About a third fewer values stay live across
 // the reduced solve, for one more elimination of the chunk.
-template <int MR, int P, int M, typename T, bool TP = false>
-__device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* __restrict__ ax,
-                                          const double* __restrict__ az, const double* smb, int nx, int ny,
+template <int MR, int P, int M, typename T, bool TP = false, typename Ar = double>
+__device__ __forceinline__ void tri_group(Ar (&d)[MR], T* S, const double* __restrict__ ax,
+                                          const double* __restrict__ az, const Ar* smb, int nx, int ny,
                                           long plane, double pcr_min, long mode, const double* __restrict__ msum,
-                                          double* __restrict__ pbar, double inv_nxnz, TriRed<P, M>& R) {
+                                          double* __restrict__ pbar, double inv_nxnz, TriRed<P, M, Ar>& R) {
     const int mi = (int)threadIdx.x % M;
     const int p = (int)threadIdx.x / M;
     const int s = p * MR;
@@ -504,30 +518,31 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
     const int i = (int)(um % (unsigned)nx);
     const int j = (int)(um / (unsigned)nx);
     // idle lanes solve a regular system (the plain bands are singular at 0)
-    const double lam = valid ? ax[i] + az[j] : -1.0;
-    const double* A = smb;
-    const double* B = A + MR * P;
-    const double* C = B + MR * P;
+    const double lam64 = valid ? ax[i] + az[j] : -1.0;
+    const Ar lam = (Ar)lam64;
+    const Ar* A = smb;
+    const Ar* B = A + MR * P;
+    const Ar* C = B + MR * P;
     auto band_a = [&](int row) { return A[row]; };
     auto band_c = [&](int row) { return C[row]; };
     auto beta = [&](int row) { return B[row] + lam; };
 
-    double Ah[TP ? 1 : MR], Ch[TP ? 1 : MR];  // TP = false: d[] is overwritten by D^ for interior rows
-    double Ae, Be, De;
-    double Pp = 0.0, Qq = 0.0, Rr = -1.0;  // x_{s+1} = Pp - Qq x_s - Rr x_e
+    Ar Ah[TP ? 1 : MR], Ch[TP ? 1 : MR];  // TP = false: d[] is overwritten by D^ for interior rows
+    Ar Ae, Be, De;
+    Ar Pp = 0, Qq = 0, Rr = -1;  // x_{s+1} = Pp - Qq x_s - Rr x_e
     if constexpr (TP) {
-        double Ak = band_a(s + 1);
-        double Bk = beta(s + 1);
-        double Dk = d[1];
-        double cprev = band_c(s + 1);
-        double pi = 1.0;  // product of -C^ over the rows eliminated so far
+        Ar Ak = band_a(s + 1);
+        Ar Bk = beta(s + 1);
+        Ar Dk = d[1];
+        Ar cprev = band_c(s + 1);
+        Ar pi = 1;  // product of -C^ over the rows eliminated so far
 #pragma unroll
         for (int l = 2; l < MR; ++l) {
-            const double rinv = rcp64(Bk);
+            const Ar rinv = rcp64(Bk);
             Pp += pi * (Dk * rinv);
             Qq += pi * (Ak * rinv);
             pi = -pi * (cprev * rinv);
-            const double f = band_a(s + l) * rinv;
+            const Ar f = band_a(s + l) * rinv;
             Ak = -f * Ak;
             Bk = beta(s + l) - f * cprev;
             Dk = d[l] - f * Dk;
@@ -538,17 +553,17 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
         Be = Bk;
         De = Dk;
     } else {
-        double Ak = band_a(s + 1);
-        double Bk = beta(s + 1);
-        double Dk = d[1];
-        double cprev = band_c(s + 1);
+        Ar Ak = band_a(s + 1);
+        Ar Bk = beta(s + 1);
+        Ar Dk = d[1];
+        Ar cprev = band_c(s + 1);
 #pragma unroll
         for (int l = 2; l < MR; ++l) {
-            const double rinv = rcp64(Bk);
+            const Ar rinv = rcp64(Bk);
             d[l - 1] = Dk * rinv;
             Ah[l - 1] = Ak * rinv;
             Ch[l - 1] = cprev * rinv;
-            const double f = band_a(s + l) * rinv;
+            const Ar f = band_a(s + l) * rinv;
             Ak = -f * Ak;
             Bk = beta(s + l) - f * cprev;
             Dk = d[l] - f * Dk;
@@ -566,46 +581,46 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
         }
     }
     // row s: sa x_{s-1} + sb x_s + sc x_e = sd (sib = 1/sb)
-    const double cs = band_c(s);
-    const double sa = band_a(s), sb = beta(s) - cs * Qq, sc = -cs * Rr, sd = d[0] - cs * Pp;
-    const double sib = rcp64(sb);
+    const Ar cs = band_c(s);
+    const Ar sa = band_a(s), sb = beta(s) - cs * Qq, sc = -cs * Rr, sd = d[0] - cs * Pp;
+    const Ar sib = rcp64(sb);
     R.a[1][p][mi] = sa;
     R.b[1][p][mi] = sib;
     R.c[1][p][mi] = sc;
     R.d[1][p][mi] = sd;
-    const bool serial = __syncthreads_or(valid && fabs(lam) < pcr_min) != 0;
+    const bool serial = __syncthreads_or(valid && fabs(lam64) < pcr_min) != 0;
     // row e with x_s and x_{e+1} eliminated: ya x_{e-} + yb x_e + yc x_{e+} = yd
-    double ya, yb, yc, yd;
+    Ar ya, yb, yc, yd;
     {
-        const double al = -Ae * sib;
-        ya = p > 0 ? al * sa : 0.0;
+        const Ar al = -Ae * sib;
+        ya = p > 0 ? al * sa : Ar(0);
         yb = Be + al * sc;
         yd = De + al * sd;
-        yc = 0.0;
+        yc = Ar(0);
         if (p + 1 < P) {
-            const double g = -band_c(s + MR - 1) * R.b[1][p + 1][mi];
+            const Ar g = -band_c(s + MR - 1) * R.b[1][p + 1][mi];
             yc = g * R.c[1][p + 1][mi];
             yb += g * R.a[1][p + 1][mi];
             yd += g * R.d[1][p + 1][mi];
         }
     }
-    double yib = serial ? yb : rcp64(yb);
+    Ar yib = serial ? yb : rcp64(yb);
     R.a[0][p][mi] = ya;
     R.b[0][p][mi] = yib;
     R.c[0][p][mi] = yc;
     R.d[0][p][mi] = yd;
     __syncthreads();
-    double xe, xem;  // x_e of this chunk and of the one below
+    Ar xe, xem;  // x_e of this chunk and of the one below
     if (serial) {
         // one thread per mode: Thomas on the P rows; x overwrites d
         if (p == 0) {
 #pragma unroll 1
             for (int q = 1; q < P; ++q) {
-                const double m = R.a[0][q][mi] / R.b[0][q - 1][mi];
+                const Ar m = R.a[0][q][mi] / R.b[0][q - 1][mi];
                 R.b[0][q][mi] = R.b[0][q][mi] - m * R.c[0][q - 1][mi];
                 R.d[0][q][mi] = R.d[0][q][mi] - m * R.d[0][q - 1][mi];
             }
-            double xq = R.d[0][P - 1][mi] / R.b[0][P - 1][mi];
+            Ar xq = R.d[0][P - 1][mi] / R.b[0][P - 1][mi];
             R.d[0][P - 1][mi] = xq;
 #pragma unroll 1
             for (int q = P - 2; q >= 0; --q) {
@@ -615,20 +630,20 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
         }
         __syncthreads();
         xe = R.d[0][p][mi];
-        xem = p > 0 ? R.d[0][p - 1][mi] : 0.0;
+        xem = p > 0 ? R.d[0][p - 1][mi] : Ar(0);
     } else {
         // parallel cyclic reduction; step st reads buffer cur, writes cur ^ 1
         int cur = 0;
         for (int st = 1; st < P; st *= 2) {
-            double a2 = 0.0, b2 = yb, c2 = 0.0, d2 = yd;
+            Ar a2 = 0, b2 = yb, c2 = 0, d2 = yd;
             if (p - st >= 0) {
-                const double f = -ya * R.b[cur][p - st][mi];
+                const Ar f = -ya * R.b[cur][p - st][mi];
                 a2 = f * R.a[cur][p - st][mi];
                 b2 += f * R.c[cur][p - st][mi];
                 d2 += f * R.d[cur][p - st][mi];
             }
             if (p + st < P) {
-                const double g = -yc * R.b[cur][p + st][mi];
+                const Ar g = -yc * R.b[cur][p + st][mi];
                 c2 = g * R.c[cur][p + st][mi];
                 b2 += g * R.a[cur][p + st][mi];
                 d2 += g * R.d[cur][p + st][mi];
@@ -646,9 +661,9 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
             __syncthreads();
         }
         xe = yd * yib;
-        xem = p > 0 ? R.d[cur][p - 1][mi] * R.b[cur][p - 1][mi] : 0.0;
+        xem = p > 0 ? R.d[cur][p - 1][mi] * R.b[cur][p - 1][mi] : Ar(0);
     }
-    const double xs = (sd - sa * xem - sc * xe) * sib;
+    const Ar xs = (sd - sa * xem - sc * xe) * sib;
     d[0] = xs;
     if constexpr (TP) {
         // rows s+1..e-1 with x_s and x_e as boundary data; d[] becomes the
@@ -657,23 +672,23 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
         // reduced solve.
         asm volatile("" ::: "memory");
         if constexpr (MR > 2) {
-            double cp[MR];
+            Ar cp[MR];
 #pragma unroll
             for (int l = 1; l < MR - 1; ++l) {
-                const double al = band_a(s + l);
-                const double cl = band_c(s + l);
-                double piv = beta(s + l), r = d[l];
+                const Ar al = band_a(s + l);
+                const Ar cl = band_c(s + l);
+                Ar piv = beta(s + l), r = d[l];
                 if (l == 1) r -= al * xs;
                 else {
                     piv -= al * cp[l - 1];
                     r -= al * d[l - 1];
                 }
                 if (l == MR - 2) r -= cl * xe;
-                const double rinv = rcp64(piv);
+                const Ar rinv = rcp64(piv);
                 cp[l] = cl * rinv;
                 d[l] = r * rinv;
             }
-            double xn = d[MR - 2];
+            Ar xn = d[MR - 2];
 #pragma unroll
             for (int l = MR - 3; l >= 1; --l) {
                 xn = d[l] - cp[l] * xn;
@@ -681,7 +696,7 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
             }
         }
     } else {
-        double xn = xe;
+        Ar xn = xe;
 #pragma unroll
         for (int l = MR - 2; l >= 1; --l) {
             xn = d[l] - Ah[l] * xs - Ch[l] * xn;
@@ -700,7 +715,7 @@ __device__ __forceinline__ void tri_group(double (&d)[MR], T* S, const double* _
 #pragma unroll
         for (int r = 0; r < MR; ++r) {
             if (r < nrow) {
-                if (mean) pbar[s + r] = d[r] * inv_nxnz;
+                if (mean) pbar[s + r] = (double)d[r] * inv_nxnz;
                 q[(size_t)r * plane] = mean ? T(0) : (T)d[r];
             }
         }
@@ -717,30 +732,35 @@ __device__ __forceinline__ long tri_group_mode(long grp, long ngroups, int M, in
     return grp * M + mi;
 }
 
-// Grid: tri_groups(plane, M) workgroups of M * P threads, 3 * MR * P doubles
-// of dynamic LDS.
-template <int MR, int P, int M, typename T = double, int OCC = 1, bool TP = false>
+// Grid: tri_groups(plane, M) workgroups of M * P threads, 3 * MR * P values
+// of type Ar of dynamic LDS.  With glist, workgroup b solves group
+// glist[b] (gridDim.x entries): the mixed form launches its FP32-arithmetic
+// groups and its FP64 ones from two such lists.
+template <int MR, int P, int M, typename T = double, int OCC = 1, bool TP = false, typename Ar = double>
 __global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __restrict__ ax,
                                                           const double* __restrict__ az,
                                                           const double* __restrict__ bands, int nx, int ny,
                                                           long plane, double pcr_min,
                                                           const double* __restrict__ msum = nullptr,
                                                           double* __restrict__ pbar = nullptr,
-                                                          double inv_nxnz = 0.0) {
+                                                          double inv_nxnz = 0.0,
+                                                          const int* __restrict__ glist = nullptr) {
     constexpr int NT = M * P;
-    __shared__ TriRed<P, M> red;
-    extern __shared__ double smb[];  // the group's band set, 3 * MR * P values
+    __shared__ TriRed<P, M, Ar> red;
+    extern __shared__ double smb_raw[];  // the group's band set, 3 * MR * P values
+    Ar* smb = reinterpret_cast<Ar*>(smb_raw);
     const int mi = (int)threadIdx.x % M;
     const int s = ((int)threadIdx.x / M) * MR;
     // adjacent mode groups share cache lines: keep them on one XCD (each has
     // its own L2; workgroups reach XCDs round-robin)
-    const long grp = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int slot = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const long grp = glist ? (long)glist[slot] : (long)slot;
     const long mode = tri_group_mode(grp, (plane + M - 1) / M, M, mi);
-    tri_stage_bands<MR, P>(smb, bands + (grp == 0 ? 3 * ny : 0), ny, NT);
-    double d[MR];
-    tri_load_rows<MR, P, M, T>(d, S, mode, plane, ny, s, msum);
+    tri_stage_bands<MR, P, Ar>(smb, bands + (grp == 0 ? 3 * ny : 0), ny, NT);
+    Ar d[MR];
+    tri_load_rows<MR, P, M, T, Ar>(d, S, mode, plane, ny, s, msum);
     __syncthreads();  // bands staged
-    tri_group<MR, P, M, T, TP>(d, S, ax, az, smb, nx, ny, plane, pcr_min, mode, msum, pbar, inv_nxnz, red);
+    tri_group<MR, P, M, T, TP, Ar>(d, S, ax, az, smb, nx, ny, plane, pcr_min, mode, msum, pbar, inv_nxnz, red);
 }
 
 }  // namespace dnsp

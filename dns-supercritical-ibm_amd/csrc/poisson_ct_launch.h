@@ -27,9 +27,14 @@ constexpr int kPartVariants = 8;
 constexpr int kTuneSlot[3] = {0, 4, 8};
 constexpr int kTuneFormSlots = 16;
 int part_rows_for(int ny);  // rows per chunk at kPartChunks chunks (0: ny too large)
-template <typename T>
+// modes per workgroup (M) of a tridiagonal variant, FP32 or FP64 spectra
+int part_group_modes(int variant, int ny, bool fp32);
+// Ar: arithmetic type (double, or float for FP32 spectra); glist (device,
+// nlist entries): solve only the listed mode groups (NULL: all of them).
+template <typename T, typename Ar = double>
 void part_launch(int variant, T* S, const double* ax, const double* az, const double* bands, int nx, int nz,
-                 int ny, double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st);
+                 int ny, double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st,
+                 const int* glist = nullptr, int nlist = 0);
 
 // Mixed-precision x kernels (poisson_mixed.h).
 enum MxKind { MX_FWD = 0, MX_RES_F32 = 1, MX_RES_F64 = 2, MX_BWD = 3, MX_ACC_F32 = 4, MX_ACC_F64 = 5 };

@@ -46,6 +46,14 @@ struct dnsp_poisson {
     double* d_lsB = nullptr;
     double* d_small = nullptr;  // msum, pbar0, pbar1, q (ny each), residual partials
     double* d_fcopy = nullptr;  // copy of an in-place right-hand side (allocated when used)
+    // mode groups of the mixed tridiagonal for M = 8, 16, 32 modes per group:
+    // list [m][0] (FP64 arithmetic: a mode with |lambda| < f32_min, mode 0
+    // and the group of modes 1..M-1) and [m][1] (FP32 arithmetic), in d_glist
+    int* d_glist = nullptr;
+    int gl_off[3][2] = {}, gl_n[3][2] = {};
+    size_t gl_total = 0;    // ints in d_glist
+    double f32_min = 0;     // |lambda| from which a mode's sweeps run in FP32
+    bool f32_sweeps = true; // FP32 arithmetic for the well-conditioned groups
 };
 
 namespace dnsp {

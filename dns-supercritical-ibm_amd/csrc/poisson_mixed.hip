@@ -5,6 +5,7 @@
 // (poisson_solver_multigpu.f90:132-155 eigenvalues, :395-422 bands).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -149,6 +150,54 @@ static int make_twiddles32(int n, float2** dev) {
 
 static size_t small_len(const dnsp_poisson* h) { return 4 * (size_t)h->ny + 2 * (size_t)kResBlocks; }
 
+// The FP32 sweeps keep their accuracy where a mode's y system is well
+// conditioned: with |lambda| >= max_k(|a_k| + |c_k|) / kF32Cond its condition
+// number is below about 2 kF32Cond, so FP32 rounding costs no more than the
+// FP32 transforms do.  Lower modes (the (0,0) mode and the lowest kx, kz)
+// keep FP64 arithmetic.
+constexpr double kF32Cond = 128.0;
+
+// Group lists of the mixed tridiagonal (see dnsp_poisson::d_glist): a group
+// goes to the FP32 list when every valid mode in it has |lambda| >= f32_min.
+static int make_group_lists(dnsp_poisson* h) {
+    const int nx = h->nx, nz = h->nz, ny = h->ny;
+    std::vector<double> ax((size_t)nx), az((size_t)nz), bands((size_t)3 * ny);
+    DNSP_HIP_TRY(hipMemcpy(ax.data(), h->d_ax, sizeof(double) * nx, hipMemcpyDeviceToHost));
+    DNSP_HIP_TRY(hipMemcpy(az.data(), h->d_az, sizeof(double) * nz, hipMemcpyDeviceToHost));
+    DNSP_HIP_TRY(hipMemcpy(bands.data(), h->d_bands, sizeof(double) * 3 * ny, hipMemcpyDeviceToHost));
+    double cmax = 0.0;
+    for (int k = 0; k < ny; ++k) cmax = std::max(cmax, std::fabs(bands[(size_t)k]) + std::fabs(bands[(size_t)2 * ny + k]));
+    h->f32_min = cmax / kF32Cond;
+    const long plane = (long)nx * nz;
+    std::vector<int> all;
+    const int Ms[3] = {8, 16, 32};
+    for (int mi = 0; mi < 3; ++mi) {
+        const int M = Ms[mi];
+        const long ngroups = (plane + M - 1) / M;
+        std::vector<int> lo, hi;
+        lo.push_back(0);
+        lo.push_back((int)ngroups);  // modes 1..M-1
+        for (long g = 1; g < ngroups; ++g) {
+            bool f32 = true;
+            for (long m = g * M; m < std::min(plane, (g + 1) * M) && f32; ++m) {
+                const double lam = ax[(size_t)(m % nx)] + az[(size_t)(m / nx)];
+                f32 = std::fabs(lam) >= h->f32_min;
+            }
+            (f32 ? hi : lo).push_back((int)g);
+        }
+        h->gl_off[mi][0] = (int)all.size();
+        h->gl_n[mi][0] = (int)lo.size();
+        all.insert(all.end(), lo.begin(), lo.end());
+        h->gl_off[mi][1] = (int)all.size();
+        h->gl_n[mi][1] = (int)hi.size();
+        all.insert(all.end(), hi.begin(), hi.end());
+    }
+    h->gl_total = all.size();
+    DNSP_HIP_TRY(hipMalloc(&h->d_glist, sizeof(int) * all.size()));
+    DNSP_HIP_TRY(hipMemcpy(h->d_glist, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice));
+    return DNSP_OK;
+}
+
 int mixed_setup(dnsp_poisson* h) {
     const size_t nlines = (size_t)h->nz * h->ny;
     if (!h->d_small) {
@@ -172,7 +221,25 @@ int mixed_setup(dnsp_poisson* h) {
     DNSP_HIP_TRY(hipMalloc(&h->d_A, sizeof(float) * N));
     DNSP_HIP_TRY(hipMalloc(&h->d_B, sizeof(float) * N));
     h->mtvar = 0;
-    return DNSP_OK;
+    return make_group_lists(h);
+}
+
+size_t poisson_device_bytes(const dnsp_poisson* h) {
+    if (!h) return 0;
+    const size_t nx = (size_t)h->nx, nz = (size_t)h->nz, ny = (size_t)h->ny, N = nx * nz * ny;
+    size_t b = sizeof(double) * (nx + nz + 6 * ny) + sizeof(int) * (nx + nz);  // eigenvalues, flags, bands
+    if (h->d_twx) b += sizeof(double2) * nx;
+    if (h->d_twz) b += sizeof(double2) * nz;
+    if (h->d_cp) b += sizeof(double) * N;
+    if (h->d_host) b += sizeof(double) * N;
+    if (h->d_small) b += sizeof(double) * small_len(h) + 2 * sizeof(double) * nz * ny;  // + line sums
+    if (h->d_twx32) b += sizeof(float2) * nx;
+    if (h->d_twz32) b += sizeof(float2) * nz;
+    if (h->d_A) b += sizeof(float) * N;
+    if (h->d_B) b += sizeof(float) * N;
+    if (h->d_fcopy) b += sizeof(double) * N;
+    if (h->d_glist) b += sizeof(int) * h->gl_total;
+    return b;
 }
 
 void mixed_release(dnsp_poisson* h) {
@@ -184,6 +251,7 @@ void mixed_release(dnsp_poisson* h) {
     (void)hipFree(h->d_lsB);
     (void)hipFree(h->d_small);
     (void)hipFree(h->d_fcopy);
+    (void)hipFree(h->d_glist);
 }
 
 namespace {
@@ -212,6 +280,24 @@ MxArgs base_args(const dnsp_poisson* h) {
     return a;
 }
 
+// The tridiagonal stage of a mixed sweep on the FP32 spectrum S: the FP64
+// groups, then the FP32 ones (or every group in FP64 when the FP32 sweeps
+// are off).
+void mixed_tri(const dnsp_poisson* h, int variant, float* S, const double* msum, double* pbar, hipStream_t st) {
+    const double inv = inv_nxnz_of(h);
+    if (!h->f32_sweeps) {
+        part_launch<float, double>(variant, S, h->d_ax, h->d_az, h->d_bands, h->nx, h->nz, h->ny, h->pcr_min, msum,
+                                   pbar, inv, st);
+        return;
+    }
+    const int M = part_group_modes(variant, h->ny, true);
+    const int mi = M == 8 ? 0 : (M == 16 ? 1 : 2);
+    part_launch<float, double>(variant, S, h->d_ax, h->d_az, h->d_bands, h->nx, h->nz, h->ny, h->pcr_min, msum, pbar,
+                               inv, st, h->d_glist + h->gl_off[mi][0], h->gl_n[mi][0]);
+    part_launch<float, float>(variant, S, h->d_ax, h->d_az, h->d_bands, h->nx, h->nz, h->ny, h->pcr_min, msum, pbar,
+                              inv, st, h->d_glist + h->gl_off[mi][1], h->gl_n[mi][1]);
+}
+
 // Sweep 0: f -> d_A holds the FP32 field pt (line sums in d_lsA) and pbar0 the
 // FP64 plane-mean profile; with `out` it writes out = pt + pbar0 in FP64 instead.
 void sweep0(dnsp_poisson* h, const double* f, double* out, hipStream_t st) {
@@ -222,8 +308,7 @@ void sweep0(dnsp_poisson* h, const double* f, double* out, hipStream_t st) {
     a.lsum = h->d_lsA;
     mx_x_launch(MX_FWD, h->nx, h->mxvar, a, st);
     ct_z_launch(h->nz, h->mzvar, false, h->d_A, h->nx, h->ny, h->d_twz32, h->d_lsA, sm.msum, st);
-    part_launch<float>(h->mtvar, h->d_A, h->d_ax, h->d_az, h->d_bands, h->nx,
-                       h->nz, h->ny, h->pcr_min, sm.msum, sm.pbar0, inv_nxnz_of(h), st);
+    mixed_tri(h, h->mtvar, h->d_A, sm.msum, sm.pbar0, st);
     ct_z_launch(h->nz, h->mzvar, true, h->d_A, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
     MxArgs b = base_args(h);
     b.src = h->d_A;
@@ -255,8 +340,7 @@ void refine(dnsp_poisson* h, const double* f, double* P, bool first, bool sums, 
     a.lsum = h->d_lsB;
     mx_x_launch(first ? MX_RES_F32 : MX_RES_F64, h->nx, h->mxvar, a, st);
     ct_z_launch(h->nz, h->mzvar, false, h->d_B, h->nx, h->ny, h->d_twz32, h->d_lsB, sm.msum, st);
-    part_launch<float>(h->mtvar, h->d_B, h->d_ax, h->d_az, h->d_bands, h->nx,
-                       h->nz, h->ny, h->pcr_min, sm.msum, sm.pbar1, inv_nxnz_of(h), st);
+    mixed_tri(h, h->mtvar, h->d_B, sm.msum, sm.pbar1, st);
     ct_z_launch(h->nz, h->mzvar, true, h->d_B, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
     MxArgs b = base_args(h);
     b.src = h->d_B;
@@ -269,24 +353,29 @@ void refine(dnsp_poisson* h, const double* f, double* P, bool first, bool sums, 
 }
 
 // rel = ||f - L P|| / ||f|| (||f - L P|| when f = 0), with P's line sums in
-// lsum.  Synchronises the stream.
-int resid_norm(dnsp_poisson* h, const double* f, const double* P, const double* lsum, double* rel, hipStream_t st) {
-    const Small sm = small_of(h);
-    double* q = sm.q;
+// lsum; q (ny) and part (2 kResBlocks) are device scratch, host_part (2
+// kResBlocks) host memory.  Synchronises the stream.
+int resid_norm_ws(const dnsp_poisson* h, const double* f, const double* P, const double* lsum, double* q,
+                  double* part, double* host_part, double* rel, hipStream_t st) {
     k_mixed_q<<<h->ny, 64, 0, st>>>(q, lsum, nullptr, h->d_bands, h->nz, h->ny, inv_nxnz_of(h));
     k_resid_norm<<<kResBlocks, 256, 0, st>>>(P, f, q, h->d_bands, h->nx, h->nz, h->ny, h->inv_dx * h->inv_dx,
-                                             h->inv_dz * h->inv_dz, sm.part);
+                                             h->inv_dz * h->inv_dz, part);
     DNSP_LAUNCH_CHECK();
-    std::vector<double> part(2 * (size_t)kResBlocks);
-    DNSP_HIP_TRY(hipMemcpyAsync(part.data(), sm.part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+    DNSP_HIP_TRY(hipMemcpyAsync(host_part, part, sizeof(double) * 2 * kResBlocks, hipMemcpyDeviceToHost, st));
     DNSP_HIP_TRY(hipStreamSynchronize(st));
     long double sr = 0.0L, sf = 0.0L;
     for (int b = 0; b < kResBlocks; ++b) {
-        sr += part[2 * (size_t)b];
-        sf += part[2 * (size_t)b + 1];
+        sr += host_part[2 * (size_t)b];
+        sf += host_part[2 * (size_t)b + 1];
     }
     *rel = (double)(sf > 0 ? sqrtl(sr / sf) : sqrtl(sr));
     return DNSP_OK;
+}
+
+int resid_norm(dnsp_poisson* h, const double* f, const double* P, const double* lsum, double* rel, hipStream_t st) {
+    const Small sm = small_of(h);
+    std::vector<double> part(2 * (size_t)kResBlocks);
+    return resid_norm_ws(h, f, P, lsum, sm.q, sm.part, part.data(), rel, st);
 }
 
 int check_mixed(const dnsp_poisson* h) {
@@ -342,9 +431,7 @@ int mixed_autotune(dnsp_poisson* h, const double* f, int reps, hipStream_t st, d
                     ct_z_launch(h->nz, v, false, h->d_A, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
                     ct_z_launch(h->nz, v, true, h->d_A, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
                 } else {
-                    for (int r = 0; r < 2; ++r)
-                        part_launch<float>(v, h->d_A, h->d_ax, h->d_az, h->d_bands, h->nx, h->nz, h->ny,
-                                           h->pcr_min, nullptr, nullptr, 0.0, st);
+                    for (int r = 0; r < 2; ++r) mixed_tri(h, v, h->d_A, nullptr, nullptr, st);
                 }
             };
             pair();  // warm-up
@@ -477,6 +564,38 @@ int dnsp_poisson_check(dnsp_poisson* h, const double* P, long long* n_nonfinite,
     if (cnt) {
         set_error("%lld of %ld values of the solution are NaN or Inf (non-finite right-hand side?)", cnt, n);
         return DNSP_ERR_NONFINITE;
+    }
+    return DNSP_OK;
+}
+
+int dnsp_poisson_set_mixed_fp32(dnsp_poisson* h, int on) {
+    clear_error();
+    int rc = check_mixed(h);
+    if (rc) return rc;
+    h->f32_sweeps = on != 0;
+    return DNSP_OK;
+}
+
+int dnsp_poisson_mixed_info(const dnsp_poisson* h, int* fp32_on, double* f32_min, long long* fp32_modes) {
+    clear_error();
+    int rc = check_mixed(h);
+    if (rc) return rc;
+    if (fp32_on) *fp32_on = h->f32_sweeps ? 1 : 0;
+    if (f32_min) *f32_min = h->f32_min;
+    if (fp32_modes) {
+        // modes of the FP32 groups of the variant in use
+        const int M = part_group_modes(h->mtvar, h->ny, true);
+        const int mi = M == 8 ? 0 : (M == 16 ? 1 : 2);
+        const long plane = (long)h->nx * h->nz;
+        long long n = 0;
+        if (h->f32_sweeps) {
+            std::vector<int> g((size_t)h->gl_n[mi][1]);
+            if (!g.empty())
+                DNSP_HIP_TRY(hipMemcpy(g.data(), h->d_glist + h->gl_off[mi][1], sizeof(int) * g.size(),
+                                       hipMemcpyDeviceToHost));
+            for (int q : g) n += std::min<long>(M, plane - (long)q * M);
+        }
+        *fp32_modes = n;
     }
     return DNSP_OK;
 }

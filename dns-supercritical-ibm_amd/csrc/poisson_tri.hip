@@ -17,24 +17,37 @@ static int rows_for(int ny, int chunks) {
 
 int part_rows_for(int ny) { return rows_for(ny, kPartChunks); }
 
-template <int MR, int P, int M, typename T, int OCC, bool TP>
+int part_group_modes(int variant, int ny, bool fp32) {
+    switch (variant) {
+        case 1: case 5: return 8;
+        case 3: case 7: return fp32 ? (ny <= 16 * 32 ? 32 : 16) : 8;
+        default: return 16;
+    }
+}
+
+template <int MR, int P, int M, typename T, int OCC, bool TP, typename Ar>
 static void part_launch_mr(T* S, const double* ax, const double* az, const double* bands, int nx, int nz, int ny,
-                           double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st) {
+                           double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st,
+                           const int* glist, int nlist) {
     const long plane = (long)nx * nz;
-    const int ngroups = (int)((plane + M - 1) / M) + 1;  // plus the group of modes 1..M-1
-    const size_t lds = 3 * sizeof(double) * MR * P;
-    kt_tri_part<MR, P, M, T, OCC, TP><<<ngroups, M * P, lds, st>>>(S, ax, az, bands, nx, ny, plane, pcr_min, msum, pbar,
-                                                                inv_nxnz);
+    // every group, plus the group of modes 1..M-1; or the listed ones
+    const int ngroups = glist ? nlist : (int)((plane + M - 1) / M) + 1;
+    if (ngroups <= 0) return;
+    const size_t lds = 3 * sizeof(Ar) * MR * P;
+    kt_tri_part<MR, P, M, T, OCC, TP, Ar><<<ngroups, M * P, lds, st>>>(S, ax, az, bands, nx, ny, plane, pcr_min, msum,
+                                                                    pbar, inv_nxnz, glist);
 }
 
 // P chunks per mode; rows per chunk from ny (at most 32 rows, ny <= 32 * P)
-template <int P, int M, typename T, int OCC, bool TP>
+template <int P, int M, typename T, int OCC, bool TP, typename Ar>
 static void part_launch_pm(T* S, const double* ax, const double* az, const double* bands, int nx, int nz, int ny,
-                           double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st) {
+                           double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st,
+                           const int* glist, int nlist) {
     switch (rows_for(ny, P)) {
-#define DNSP_CASE(MR)                                                                                         \
-    case MR:                                                                                                  \
-        part_launch_mr<MR, P, M, T, OCC, TP>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st);   \
+#define DNSP_CASE(MR)                                                                                          \
+    case MR:                                                                                                   \
+        part_launch_mr<MR, P, M, T, OCC, TP, Ar>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st, \
+                                                 glist, nlist);                                                \
         break;
         DNSP_PART_ROWS(DNSP_CASE)
 #undef DNSP_CASE
@@ -42,10 +55,13 @@ static void part_launch_pm(T* S, const double* ax, const double* az, const doubl
     }
 }
 
-template <typename T>
+template <typename T, typename Ar>
 void part_launch(int variant, T* S, const double* ax, const double* az, const double* bands, int nx, int nz,
-                 int ny, double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st) {
-#define DNSP_PV(P, M, OCC, TP) part_launch_pm<P, M, T, OCC, TP>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st)
+                 int ny, double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st,
+                 const int* glist, int nlist) {
+#define DNSP_PV(P, M, OCC, TP)                                                                                    \
+    part_launch_pm<P, M, T, OCC, TP, Ar>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st, glist, \
+                                         nlist)
     if constexpr (sizeof(T) == 4) {
         // FP32 spectra: (16, 32) in place of (64, 8), whose 8-mode FP32 rows
         // are 32-byte segments (32 modes make a 128-byte row).  16 chunks of
@@ -83,9 +99,11 @@ void part_launch(int variant, T* S, const double* ax, const double* az, const do
 #undef DNSP_PV
 }
 
-template void part_launch<double>(int, double*, const double*, const double*, const double*, int, int, int, double,
-                                  const double*, double*, double, hipStream_t);
-template void part_launch<float>(int, float*, const double*, const double*, const double*, int, int, int, double,
-                                 const double*, double*, double, hipStream_t);
+template void part_launch<double, double>(int, double*, const double*, const double*, const double*, int, int, int,
+                                          double, const double*, double*, double, hipStream_t, const int*, int);
+template void part_launch<float, double>(int, float*, const double*, const double*, const double*, int, int, int,
+                                         double, const double*, double*, double, hipStream_t, const int*, int);
+template void part_launch<float, float>(int, float*, const double*, const double*, const double*, int, int, int,
+                                        double, const double*, double*, double, hipStream_t, const int*, int);
 
 }  // namespace dnsp

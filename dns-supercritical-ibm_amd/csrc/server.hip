@@ -117,7 +117,12 @@ struct CaseDesc {
     std::vector<double> c[3];
     std::vector<int> bot, top;
     unsigned long long seed;
-    size_t footprint = 0;
+    size_t footprint = 0;  // upper bound before the first build, the measured bytes after it
+    bool measured = false;
+    // form of the solve: -1 FP64, 0/1 mixed refinements (the server's, or
+    // per case the faster one when the server picks)
+    int form = -1;
+    double ms_fp64 = -1.0, ms_mixed = -1.0;  // step times measured when the server picks
     // device state while resident
     dnsp_step* step = nullptr;
     dnsp_ibm* ibm[3] = {nullptr, nullptr, nullptr};
@@ -163,6 +168,39 @@ void drop_case(dnsp_server* sv, CaseDesc& c, bool count = true) {
     if (count) ++sv->evictions;
 }
 
+// Times the case's step in both forms (eagerly on `st`, on whatever its
+// fields hold: the work does not depend on the values) and keeps the faster.
+int pick_form(CaseDesc& c, hipStream_t st) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        set_error("event creation failed");
+        return DNSP_ERR_HIP;
+    }
+    int rc = DNSP_OK;
+    double ms[2] = {0.0, 0.0};
+    const int forms[2] = {-1, 1};
+    for (int f = 0; f < 2 && !rc; ++f) {
+        rc = dnsp_step_enqueue(c.step, forms[f], reinterpret_cast<void*>(st));  // warm-up
+        if (hipEventRecord(e0, st) != hipSuccess) rc = DNSP_ERR_HIP;
+        for (int r = 0; r < 3 && !rc; ++r) rc = dnsp_step_enqueue(c.step, forms[f], reinterpret_cast<void*>(st));
+        if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)) {
+            set_error("timing the case's step failed");
+            rc = DNSP_ERR_HIP;
+        }
+        float t = 0.f;
+        if (!rc) (void)hipEventElapsedTime(&t, e0, e1);
+        ms[f] = t / 3.0;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    c.ms_fp64 = ms[0];
+    c.ms_mixed = ms[1];
+    c.form = ms[1] < ms[0] ? 1 : -1;
+    return DNSP_OK;
+}
+
 // Creates the case's device state; the graph is recorded on `st`.
 int build_case(dnsp_server* sv, CaseDesc& c, hipStream_t st) {
     int rc = dnsp_step_create(c.nx, c.nz, c.ny, c.yu.data(), c.inv_dx, c.inv_dz, c.dtmax, c.incompressible,
@@ -184,7 +222,8 @@ int build_case(dnsp_server* sv, CaseDesc& c, hipStream_t st) {
             rc = DNSP_ERR_NOMEM;
         }
     }
-    if (!rc) rc = dnsp_step_capture(c.step, sv->refinements, reinterpret_cast<void*>(st));
+    if (!rc && sv->refinements == DNSP_SERVER_PICK_FORM && c.ms_fp64 < 0) rc = pick_form(c, st);
+    if (!rc) rc = dnsp_step_capture(c.step, c.form, reinterpret_cast<void*>(st));
     if (rc) {
         // leave no half-built state behind
         if (c.step) dnsp_step_free(c.step);
@@ -197,6 +236,13 @@ int build_case(dnsp_server* sv, CaseDesc& c, hipStream_t st) {
         c.step = nullptr;
         return rc;
     }
+    // the bytes the case really holds replace the admission bound
+    size_t bytes = 0;
+    (void)dnsp_step_footprint(c.step, &bytes);
+    for (dnsp_ibm* h : c.ibm) bytes += ibm_device_bytes(h);
+    bytes += sizeof(double) * c.ypad.size();
+    c.footprint = bytes;
+    c.measured = true;
     sv->resident += c.footprint;
     sv->peak_resident = std::max(sv->peak_resident, sv->resident);
     ++sv->admissions;
@@ -209,6 +255,12 @@ int admit_case(dnsp_server* sv, int ci, hipStream_t st, int* rc_out) {
     CaseDesc& c = sv->cases[(size_t)ci];
     *rc_out = DNSP_OK;
     if (c.step) return 1;
+    // evict only when the idle cases' bytes make room: otherwise dropping
+    // them would only force their rebuild later
+    size_t idle = 0;
+    for (const CaseDesc& o : sv->cases)
+        if (o.step && !o.busy) idle += o.footprint;
+    if (sv->resident - idle + c.footprint > sv->budget) return 0;
     while (sv->resident + c.footprint > sv->budget) {
         int victim = -1;
         for (size_t k = 0; k < sv->cases.size(); ++k) {
@@ -255,8 +307,10 @@ int dnsp_server_create(size_t hbm_budget, int max_slots, int precision, int refi
     DNSP_ARG_CHECK(out, "out is NULL");
     *out = nullptr;
     DNSP_ARG_CHECK(max_slots >= 1 && max_slots <= 16, "max_slots=%d not in [1, 16]", max_slots);
-    DNSP_ARG_CHECK(refinements >= -1 && refinements <= 1, "refinements=%d: -1 (FP64), 0 or 1", refinements);
-    DNSP_ARG_CHECK(refinements < 0 || precision == DNSP_PRECISION_MIXED, "the mixed form needs precision 1");
+    DNSP_ARG_CHECK(refinements >= DNSP_SERVER_PICK_FORM && refinements <= 1,
+                   "refinements=%d: -1 (FP64), 0 or 1 (mixed), or DNSP_SERVER_PICK_FORM", refinements);
+    DNSP_ARG_CHECK(refinements == -1 || precision == DNSP_PRECISION_MIXED,
+                   "the mixed form (and picking a form) needs precision 1");
     dnsp_server* sv = new (std::nothrow) dnsp_server();
     if (!sv) {
         set_error("host allocation failed");
@@ -321,12 +375,21 @@ int dnsp_server_add_case(dnsp_server* sv, const dnsp_case_desc* d, int* case_id)
         c.top.assign(d->top, d->top + n);
     }
     c.seed = d->seed;
-    // footprint: 11 halo fields + P, geometry, solver work arrays, IBM records
-    const size_t hc = (size_t)(c.nx + 2) * (c.nz + 2) * (c.ny + 2), pc = (size_t)c.nx * c.nz * c.ny;
-    c.footprint = sizeof(double) * (11 * hc + pc) + 2 * sizeof(double) * (size_t)c.nz * c.ny;
-    if (sv->precision == DNSP_PRECISION_MIXED) c.footprint += 2 * sizeof(float) * pc;
-    for (int k = 0; k < 3; ++k) c.footprint += (size_t)d->n_ibm[k] * 96;
-    c.footprint += 2 * sizeof(int) * c.bot.size();
+    c.form = sv->refinements == DNSP_SERVER_PICK_FORM ? -1 : sv->refinements;
+    // Admission bound until the first build measures the case: 11 halo fields
+    // and P; the solver's arrays (a full field of Thomas workspace when the
+    // partitioned kernel does not apply, the mixed form's two FP32 fields,
+    // line sums, group lists, tables); IBM records (56 bytes each, at most 7
+    // periodic images of 16 bytes per target); slabs, geometry.
+    const size_t nx = (size_t)c.nx, nz = (size_t)c.nz, ny = (size_t)c.ny;
+    const size_t hc = (nx + 2) * (nz + 2) * (ny + 2), pc = nx * nz * ny;
+    size_t fp = sizeof(double) * (11 * hc + pc);
+    if (ny < 32 || ny > 1024) fp += sizeof(double) * pc;  // serial Thomas workspace
+    fp += sizeof(double) * (2 * nz * ny + 16 * ny + 2048) + 64 * (nx + nz);
+    if (sv->precision == DNSP_PRECISION_MIXED) fp += 2 * sizeof(float) * pc + sizeof(int) * (nx * nz / 4 + 8);
+    for (int k = 0; k < 3; ++k) fp += (size_t)d->n_ibm[k] * (56 + 7 * 16);
+    fp += 2 * sizeof(int) * c.bot.size();
+    c.footprint = fp;
     DNSP_ARG_CHECK(c.footprint <= sv->budget, "case footprint %zu bytes exceeds the HBM budget %zu", c.footprint,
                    sv->budget);
     sv->cases.push_back(std::move(c));
@@ -338,6 +401,17 @@ int dnsp_server_case_footprint(const dnsp_server* sv, int case_id, size_t* bytes
     clear_error();
     DNSP_ARG_CHECK(sv && bytes && case_id >= 0 && case_id < (int)sv->cases.size(), "bad argument");
     *bytes = sv->cases[(size_t)case_id].footprint;
+    return DNSP_OK;
+}
+
+int dnsp_server_case_form(const dnsp_server* sv, int case_id, int* refinements, double* ms_fp64,
+                          double* ms_mixed) {
+    clear_error();
+    DNSP_ARG_CHECK(sv && case_id >= 0 && case_id < (int)sv->cases.size(), "bad argument");
+    const CaseDesc& c = sv->cases[(size_t)case_id];
+    if (refinements) *refinements = c.form;
+    if (ms_fp64) *ms_fp64 = c.ms_fp64;
+    if (ms_mixed) *ms_mixed = c.ms_mixed;
     return DNSP_OK;
 }
 

@@ -192,9 +192,6 @@ int dnsp_step_create(int nx, int nz, int ny, const double* yu, double inv_dx, do
             break;
         }
         s->bytes += sizeof(double) * (g.size() + 7);
-        // the solver's own device memory: FP32 fields of the mixed form, line sums
-        if (precision == DNSP_PRECISION_MIXED) s->bytes += 2 * sizeof(float) * pc;
-        s->bytes += 2 * sizeof(double) * (size_t)nz * ny;
     } while (0);
     if (rc) {
         release(s);
@@ -289,7 +286,8 @@ int dnsp_step_poisson(dnsp_step* s, dnsp_poisson** h) {
 int dnsp_step_footprint(const dnsp_step* s, size_t* bytes) {
     clear_error();
     DNSP_ARG_CHECK(s && bytes, "NULL argument");
-    *bytes = s->bytes;
+    // the step's fields, geometry and slabs, plus every array of its solver
+    *bytes = s->bytes + poisson_device_bytes(s->poisson);
     return DNSP_OK;
 }
 

@@ -79,6 +79,8 @@ _SIGNATURES = [
     ("dnsp_poisson_run_mixed", [_vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     ("dnsp_poisson_solve_refine", [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _ip, _dp, _vp], ctypes.c_int),
     ("dnsp_poisson_residual", [_vp, _vp, _vp, _dp, _vp], ctypes.c_int),
+    ("dnsp_poisson_set_mixed_fp32", [_vp, ctypes.c_int], ctypes.c_int),
+    ("dnsp_poisson_mixed_info", [_vp, _ip, _dp, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     ("dnsp_poisson_check", [_vp, _vp, ctypes.POINTER(ctypes.c_longlong), _vp], ctypes.c_int),
     ("dnsp_batch_create", [ctypes.c_int, _ip, _ip, _ip, _dp, _dp, _dp, ctypes.c_int, ctypes.POINTER(_vp)],
      ctypes.c_int),
@@ -227,6 +229,18 @@ class PoissonSolver:
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(lib().dnsp_poisson_get_mixed_variants(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return (a.value, b.value, c.value)
+
+    def set_mixed_fp32(self, on=True):
+        """FP32 arithmetic in the mixed sweeps' well-conditioned tridiagonal
+        solves (default on); off: FP64 arithmetic for every mode."""
+        _check(lib().dnsp_poisson_set_mixed_fp32(self._h, int(bool(on))))
+
+    @property
+    def mixed_info(self):
+        """(fp32 sweeps on, |lambda| from which a mode runs in FP32, modes that do)."""
+        on, lam, n = ctypes.c_int(), ctypes.c_double(), ctypes.c_longlong()
+        _check(lib().dnsp_poisson_mixed_info(self._h, ctypes.byref(on), ctypes.byref(lam), ctypes.byref(n)))
+        return bool(on.value), lam.value, n.value
 
     def autotune(self, work=None, reps=5, stream=None):
         """Picks the fastest kernel variants; returns the timing table (ms per pair)."""

@@ -26,6 +26,7 @@ _SIGS = [
     ("dnsp_server_create", [ctypes.c_size_t, _i, _i, _i, ctypes.POINTER(_vp)]),
     ("dnsp_server_add_case", [_vp, ctypes.POINTER(CaseDesc), ctypes.POINTER(_i)]),
     ("dnsp_server_case_footprint", [_vp, _i, ctypes.POINTER(ctypes.c_size_t)]),
+    ("dnsp_server_case_form", [_vp, _i, ctypes.POINTER(_i), _dp, _dp]),
     ("dnsp_server_run", [_vp, _i, _ip, _dp, ctypes.POINTER(ctypes.c_ulonglong), _dp, _dp, _dp, _dp]),
     ("dnsp_server_free", [_vp]),
 ]
@@ -48,8 +49,13 @@ def exported_names():
     return [s[0] for s in _SIGS]
 
 
+PICK_FORM = -2  # DNSP_SERVER_PICK_FORM
+
+
 class Server:
-    """hbm_budget in bytes; refinements -1 (FP64) or 0/1 (mixed, precision 1)."""
+    """hbm_budget in bytes; refinements -1 (FP64), 0/1 (mixed, precision 1) or
+    PICK_FORM (precision 1: per case the faster of FP64 and mixed with one
+    refinement, timed at its first admission)."""
 
     def __init__(self, hbm_budget, max_slots=4, precision=1, refinements=1):
         h = _vp()
@@ -97,6 +103,14 @@ class Server:
         v = ctypes.c_size_t()
         _check(_lib().dnsp_server_case_footprint(self._h, int(case_id), ctypes.byref(v)))
         return v.value
+
+    def case_form(self, case_id):
+        """(refinements of the case's form, FP64 step ms, mixed step ms); the
+        times are -1 unless the server picks forms and the case was admitted."""
+        r, a, b = _i(), _d(), _d()
+        _check(_lib().dnsp_server_case_form(self._h, int(case_id), ctypes.byref(r), ctypes.byref(a),
+                                            ctypes.byref(b)))
+        return r.value, a.value, b.value
 
     def run(self, case_ids, arrivals_s, seeds):
         """Replays the trace; returns dict(latency_ms, start_ms, checksum (n, 4), stats)."""

@@ -170,8 +170,10 @@ int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream,
 /*
  * Mixed-precision form of the solve (no counterpart in the reference, whose
  * solve is FP64 throughout; it solves the same discrete system).  Sweep 0:
- * FP32 x and z transforms and FP32 spectra, FP64 arithmetic in the y
- * tridiagonal solves, and the (0,0) mode (the mean profile, pinned by the
+ * FP32 x and z transforms and FP32 spectra, FP32 arithmetic in the y
+ * tridiagonal solves of the well-conditioned modes (|lambda| >=
+ * max_k(|a_k| + |c_k|) / 128, all but the lowest kx, kz), FP64 arithmetic in
+ * those of the others, and the (0,0) mode (the mean profile, pinned by the
  * P_band_*_00 set) solved entirely in FP64 from FP64 plane sums.  Each
  * refinement sweep evaluates the FP64 residual r = rhs - L p inside the
  * forward x transform, solves L d = r the same way and adds d in FP64.
@@ -199,6 +201,13 @@ int dnsp_poisson_solve_mixed(dnsp_poisson* h, const double* rhs, double* P, int 
 int dnsp_poisson_run_mixed(dnsp_poisson* h, double* P, int refinements, void* stream);
 int dnsp_poisson_solve_refine(dnsp_poisson* h, const double* rhs, double* P, double tol, int max_refinements,
                               int* refinements_done, double* rel_resid, void* stream);
+/* FP32 arithmetic in the sweeps' well-conditioned tridiagonal solves: on (1,
+ * the default) or off (0: FP64 arithmetic for every mode). */
+int dnsp_poisson_set_mixed_fp32(dnsp_poisson* h, int on);
+/* The mixed form's split: whether FP32 sweeps are on, the |lambda| from which
+ * a mode runs in FP32, and how many of the nx*nz modes do (for the current
+ * tridiagonal variant).  Any output may be NULL. */
+int dnsp_poisson_mixed_info(const dnsp_poisson* h, int* fp32_on, double* f32_min, long long* fp32_modes);
 
 /*
  * Batched form: many independent solves of uneven sizes (no counterpart in
@@ -242,6 +251,7 @@ int dnsp_batch_free(dnsp_batch* b);
  * any precision of handle.  Synchronises `stream`.  DNSP_ERR_NONFINITE (with
  * *rel_resid written) when the residual is NaN or Inf. */
 int dnsp_poisson_residual(dnsp_poisson* h, const double* rhs, const double* P, double* rel_resid, void* stream);
+
 
 /* min |pivot| over all modes and rows: the init diagnostic min_abs_div_bacp
  * (poisson_solver_multigpu.f90:181-222), computed on the device at init. */
@@ -617,17 +627,30 @@ typedef struct {
 
 /* hbm_budget: device bytes the resident cases may hold; max_slots (1..16):
  * requests in flight at once (one stream each); precision and refinements
- * as for dnsp_step_enqueue. */
+ * as for dnsp_step_enqueue, or refinements = DNSP_SERVER_PICK_FORM
+ * (precision MIXED): on its first admission each case times its step in
+ * the FP64 form and in the mixed form with one refinement (both meet a
+ * relative residual of 1e-10) and keeps the faster. */
+#define DNSP_SERVER_PICK_FORM (-2)
 int dnsp_server_create(size_t hbm_budget, int max_slots, int precision, int refinements, dnsp_server** out);
 /* Registers a case (its device state is created on first admission). */
 int dnsp_server_add_case(dnsp_server* sv, const dnsp_case_desc* d, int* case_id);
+/* Device bytes the case holds while resident: an upper bound computed from
+ * its description until its first admission, the bytes its step, solver,
+ * IBM handles and tables hold after it (the admission test uses this). */
 int dnsp_server_case_footprint(const dnsp_server* sv, int case_id, size_t* bytes);
+/* The case's form (-1 FP64, 0/1 mixed refinements) and, when the server
+ * picks forms, the step times (ms) it measured for FP64 and mixed (-1 before
+ * the case's first admission).  Any output may be NULL. */
+int dnsp_server_case_form(const dnsp_server* sv, int case_id, int* refinements, double* ms_fp64,
+                          double* ms_mixed);
 /* Replays n requests (case_id[i], seed[i]) arriving at arrival_s[i] seconds
  * after the call starts (non-decreasing), against the wall clock: waiting
  * requests are admitted into free slots in arrival order, skipping those
  * whose case is busy or does not fit the budget yet; a case is made
  * resident on first admission (least-recently-used idle cases evicted when
- * needed) and stays resident otherwise.  Outputs per request: latency_ms
+ * needed, and only when evicting every idle case would make room) and stays
+ * resident otherwise.  Outputs per request: latency_ms
  * (completion - arrival), start_ms (admission, may be NULL), checksum[4]
  * (may be NULL: sum and index-weighted sum of P, then of U).  stats (may be
  * NULL, 4 doubles): admissions, evictions, peak resident bytes, wall s.

@@ -107,6 +107,46 @@ def test_mixed_matches_oracle(cuda, nx, nz, ny):
     s.close()
 
 
+def test_fp32_sweeps_split_and_accuracy(cuda):
+    """The sweeps run FP32 arithmetic on the mode groups whose modes all have
+    |lambda| >= max_k(|a_k| + |c_k|) / 128 and FP64 on the rest (the (0,0)
+    and lowest modes); with either arithmetic one refinement meets 1e-10 on
+    a wall-refined 384-row grid (the production y grid at a smaller x-z
+    box), and the two agree with the oracle's FP64 solution to 1e-9."""
+    say("mixed fp32 sweeps split")
+    nx, nz, ny = 192, 96, 384
+    yu = W.wall_refined_faces(ny, 0.05, 1.015)
+    c = dict(nx=nx, nz=nz, ny=ny, inv_dx=nx / W.LX, inv_dz=nz / W.LZ, yu=yu,
+             rhs=W.rhs_band_limited((ny, nz, nx), 1, yu=yu))
+    s, bands = make_solver(c)
+    on, lam_min, n32 = s.mixed_info
+    a, cc = np.asarray(bands[0]), np.asarray(bands[2])
+    assert on and lam_min == np.max(np.abs(a) + np.abs(cc)) / 128.0
+    ax, _ = O.eigen(nx, c["inv_dx"])
+    az, _ = O.eigen(nz, c["inv_dz"])
+    lam = np.abs((az[:, None] + ax[None, :]).ravel())  # mode a + nx*b
+    M = 16  # modes per group of the default variant
+    want = 0
+    for g in range(1, (nx * nz + M - 1) // M):
+        blk = lam[g * M:(g + 1) * M]
+        if np.all(blk >= lam_min):
+            want += blk.size
+    assert n32 == want and 0.5 * nx * nz < n32 < nx * nz
+    ref = O.solve(c["rhs"], c["inv_dx"], c["inv_dz"], bands)
+    f = to_dev(c["rhs"])
+    got = {}
+    for fp32 in (True, False):
+        s.set_mixed_fp32(fp32)
+        P = to_dev(np.zeros_like(c["rhs"]))
+        s.solve_mixed(f, P, refinements=1)
+        got[fp32] = to_host(P)
+        assert oracle_rel_resid(got[fp32], c, bands) <= 1e-10, fp32
+        assert rel_mean_removed(got[fp32], ref) <= 1e-9, fp32
+    assert s.mixed_info[0] is False and s.mixed_info[2] == 0
+    assert not np.array_equal(got[True], got[False])
+    s.close()
+
+
 def test_mixed_in_place_equals_out_of_place(cuda):
     say("mixed in place")
     c = case_grid(192, 96, 24, 11)

@@ -91,6 +91,46 @@ def test_mixed_agrees_with_fp64(cuda):
     s64.close()
 
 
+def test_pick_form_runs_each_case_in_its_faster_form(cuda):
+    """With DNSP_SERVER_PICK_FORM each case times its step in both forms at
+    its first admission and keeps the faster: its results are bit-identical
+    to a server fixed to that form."""
+    say("serving pick form")
+    cases = small_ensemble()
+    t, cid, sd = trace(16, 2000.0, len(cases), seed=11)
+    sp, ids = make_server(cases, 1 << 36, 3, refinements=SV.PICK_FORM)
+    assert all(sp.case_form(i)[1] == -1.0 for i in ids)
+    got = sp.run(cid, t, sd)["checksum"]
+    forms = {i: sp.case_form(i) for i in ids}
+    fixed = {}
+    for ref in (-1, 1):
+        s, _ = make_server(cases, 1 << 36, 3, refinements=ref)
+        fixed[ref] = s.run(cid, t, sd)["checksum"]
+        s.close()
+    for r in range(len(cid)):
+        form, ms64, msmx = forms[int(cid[r])]
+        assert ms64 > 0 and msmx > 0 and form == (1 if msmx < ms64 else -1)
+        assert np.array_equal(got[r], fixed[form][r]), r
+    sp.close()
+
+
+def test_footprint_bound_then_measured(cuda):
+    """Before its first admission a case's footprint is the admission bound;
+    after it, the bytes its step, solver and IBM handles hold, which the
+    bound covers."""
+    say("serving footprint")
+    cases = small_ensemble()
+    s, ids = make_server(cases, 1 << 36, 2)
+    bound = [s.footprint(i) for i in ids]
+    s.run(np.array(ids, dtype=np.int32), np.zeros(len(ids)), np.arange(len(ids), dtype=np.uint64))
+    real = [s.footprint(i) for i in ids]
+    for n, c in enumerate(cases):
+        nx, nz, ny = c["nx"], c["nz"], c["ny"]
+        fields = 8 * (11 * (nx + 2) * (nz + 2) * (ny + 2) + nx * nz * ny) + 8 * nx * nz * ny  # + two FP32 fields
+        assert fields < real[n] <= bound[n], (n, real[n], bound[n])
+    s.close()
+
+
 def test_serving_errors(cuda):
     say("serving errors")
     cases = small_ensemble()

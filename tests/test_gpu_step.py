@@ -199,7 +199,10 @@ def generated_rough_wall(c, seed):
         x, z, y = coords[comp]
         ijk, cc, sb, st = DP.ibm_generate(np.array(mnab), W.LX, W.LZ, W.LY, x, z, y,
                                           bc_bot=c["walls"][n], bc_top=c["walls"][3 + n])
-        ibm[comp] = (ijk, cc)
+        # sources of records near x/z = 0 or n lie one period away: wrapped
+        # mod n as the coefficient reader does (and dnsp_ibm_create)
+        ijk = ijk.reshape(-1, 3, 3) % np.array([nx, nz, ny], dtype=np.int32)
+        ibm[comp] = (ijk.reshape(-1, 9), cc)
         cols[comp] = (sb, st)
     return ibm, cols
 
