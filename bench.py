@@ -487,7 +487,9 @@ def main():
                   "p50_ms": round(float(np.percentile(lat_m, 50)), 4),
                   "p99_ms": round(float(np.percentile(lat_m, 99)), 4),
                   "refinements": n_ref, "rel_residual_l2": relm, "rel_residual_maxabs": rel_res_m,
-                  "err_vs_fp64": err_m, "variants": list(solver.mixed_variants)},
+                  "err_vs_fp64": err_m, "variants": list(solver.mixed_variants),
+                  "fp32_sweep_modes": solver.mixed_info[2], "fp32_sweep_min_abs_lambda": solver.mixed_info[1],
+                  "modes": nx * nz},
     }
     forms["mixed_over_fp64"] = round(forms["mixed"]["solves_per_s"] / forms["fp64"]["solves_per_s"], 4)
     ok = {k: forms[k]["rel_residual_l2"] <= 1e-10 for k in ("fp64", "mixed")}
