@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -378,6 +379,35 @@ int resid_norm(dnsp_poisson* h, const double* f, const double* P, const double* 
     return resid_norm_ws(h, f, P, lsum, sm.q, sm.part, part.data(), rel, st);
 }
 
+// Scratch of dnsp_poisson_residual_noalloc in a file-scope device array: the
+// line sums (up to kNaLines), the plane constants q (up to kNaRows) and the
+// reduction partials.  Each GPU the library runs on holds its own copy
+// (2.1 MB) until the process ends.  Host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device (the array's
+// name is a host-side stand-in, not a device address).
+constexpr long kNaLines = 1L << 18;
+constexpr int kNaRows = 4096;
+__device__ __align__(256) double scratch[kNaLines + kNaRows + 2 * kResBlocks];
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mutex;
+double* g_na_ptr[kNaMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device.
+int noalloc_scratch(int device, double** out) {
+    DNSP_ARG_CHECK(device >= 0 && device < kNaMaxDevices, "device %d: at most %d devices", device,
+                   kNaMaxDevices);
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (!g_na_ptr[device]) {
+        void* ptr = nullptr;
+        DNSP_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_na_ptr[device] = static_cast<double*>(ptr);
+    }
+    *out = g_na_ptr[device];
+    return DNSP_OK;
+}
+
 int check_mixed(const dnsp_poisson* h) {
     DNSP_ARG_CHECK(h, "NULL handle");
     if (h->precision != DNSP_PRECISION_MIXED) {
@@ -539,6 +569,36 @@ int dnsp_poisson_residual(dnsp_poisson* h, const double* rhs, const double* P, d
     const long nlines = (long)h->nz * h->ny;
     k_line_sums<<<(int)((nlines + 3) / 4), 256, 0, st>>>(P, h->d_lsB, h->nx, nlines);
     const int rc = resid_norm(h, rhs, P, h->d_lsB, rel_resid, st);
+    if (rc) return rc;
+    if (!std::isfinite(*rel_resid)) {
+        set_error("residual is %g: the solution or the right-hand side holds a NaN or Inf", *rel_resid);
+        return DNSP_ERR_NONFINITE;
+    }
+    return DNSP_OK;
+}
+
+int dnsp_poisson_residual_noalloc(dnsp_poisson* h, const double* rhs, const double* P, double* rel_resid,
+                                  void* stream) {
+    clear_error();
+    DNSP_ARG_CHECK(h && rhs && P && rel_resid, "NULL handle, array or output");
+    const long nlines = (long)h->nz * h->ny;
+    DNSP_ARG_CHECK(nlines <= kNaLines && h->ny <= kNaRows,
+                   "nz*ny = %ld lines and ny = %d: the static scratch of dnsp_poisson_residual_noalloc holds at "
+                   "most %ld lines and %d rows",
+                   nlines, h->ny, kNaLines, kNaRows);
+    int dev = -1;
+    DNSP_HIP_TRY(hipGetDevice(&dev));
+    DNSP_ARG_CHECK(dev == h->device, "the handle lives on device %d but device %d is current", h->device, dev);
+    double* ws = nullptr;
+    int rc = noalloc_scratch(dev, &ws);
+    if (rc) return rc;
+    double* lsum = ws;
+    double* q = ws + kNaLines;
+    double* part = q + kNaRows;
+    hipStream_t st = as_stream(stream);
+    k_line_sums<<<(int)((nlines + 3) / 4), 256, 0, st>>>(P, lsum, h->nx, nlines);
+    double host_part[2 * kResBlocks];
+    rc = resid_norm_ws(h, rhs, P, lsum, q, part, host_part, rel_resid, st);
     if (rc) return rc;
     if (!std::isfinite(*rel_resid)) {
         set_error("residual is %g: the solution or the right-hand side holds a NaN or Inf", *rel_resid);

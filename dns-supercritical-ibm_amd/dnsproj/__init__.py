@@ -81,6 +81,7 @@ _SIGNATURES = [
     ("dnsp_poisson_residual", [_vp, _vp, _vp, _dp, _vp], ctypes.c_int),
     ("dnsp_poisson_set_mixed_fp32", [_vp, ctypes.c_int], ctypes.c_int),
     ("dnsp_poisson_mixed_info", [_vp, _ip, _dp, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    ("dnsp_poisson_residual_noalloc", [_vp, _vp, _vp, _dp, _vp], ctypes.c_int),
     ("dnsp_poisson_check", [_vp, _vp, ctypes.POINTER(ctypes.c_longlong), _vp], ctypes.c_int),
     ("dnsp_batch_create", [ctypes.c_int, _ip, _ip, _ip, _dp, _dp, _dp, ctypes.c_int, ctypes.POINTER(_vp)],
      ctypes.c_int),
@@ -288,6 +289,16 @@ class PoissonSolver:
         _check(lib().dnsp_poisson_residual(self._h, _vp(_device_ptr(rhs, self.numel)),
                                            _vp(_device_ptr(P, self.numel)), ctypes.byref(rel),
                                            _vp(_stream_ptr(stream))))
+        return rel.value
+
+    def residual_noalloc(self, rhs, P, stream=None):
+        """As residual(), through the entry point that allocates nothing (its
+        scratch is a static device array of the library, shared by every call
+        on the GPU: overlapping calls race)."""
+        rel = ctypes.c_double()
+        _check(lib().dnsp_poisson_residual_noalloc(self._h, _vp(_device_ptr(rhs, self.numel)),
+                                                   _vp(_device_ptr(P, self.numel)), ctypes.byref(rel),
+                                                   _vp(_stream_ptr(stream))))
         return rel.value
 
     def check(self, P, stream=None):

@@ -252,6 +252,19 @@ int dnsp_batch_free(dnsp_batch* b);
  * *rel_resid written) when the residual is NaN or Inf. */
 int dnsp_poisson_residual(dnsp_poisson* h, const double* rhs, const double* P, double* rel_resid, void* stream);
 
+/* dnsp_poisson_residual with no device allocation anywhere on its path: its
+ * scratch (line sums, plane constants, reduction partials) lives in a
+ * file-scope device array of the library, 2.1 MB per GPU, held until the
+ * process ends.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (tight serving loops, code run while a
+ * caller captures a stream into a graph, harnesses that forbid hipMalloc).
+ * Limits: nz*ny <= 262144 lines and ny <= 4096 (DNSP_ERR_ARG beyond them);
+ * the handle's device must be current.  Same result as
+ * dnsp_poisson_residual, bit for bit.  Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU (other streams, other host
+ * threads) race.  Synchronises `stream`. */
+int dnsp_poisson_residual_noalloc(dnsp_poisson* h, const double* rhs, const double* P, double* rel_resid,
+                                  void* stream);
 
 /* min |pivot| over all modes and rows: the init diagnostic min_abs_div_bacp
  * (poisson_solver_multigpu.f90:181-222), computed on the device at init. */

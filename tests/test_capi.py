@@ -95,5 +95,6 @@ def test_null_handle_calls_fail_cleanly():
     out = ctypes.c_double()
     assert L.dnsp_poisson_min_pivot(None, ctypes.byref(out)) == D.DNSP_ERR_ARG
     assert L.dnsp_poisson_free(None) == 0
+    assert L.dnsp_poisson_residual_noalloc(None, None, None, ctypes.byref(out), None) == D.DNSP_ERR_ARG
     assert L.dnsp_poisson_set_mixed_fp32(None, 1) == D.DNSP_ERR_ARG
     assert L.dnsp_poisson_mixed_info(None, None, None, None) == D.DNSP_ERR_ARG
