@@ -217,8 +217,14 @@ def bench_serve(torch, D, W, n_req):
         out = srv.run(cid, t, sd + 1000)
         lat = out["latency_ms"]
         done = t + lat * 1e-3
+        wait = out["start_ms"] - t * 1e3  # arrival -> admission into a slot
+        serv = lat - wait                 # admission -> completion
         st = out["stats"]
         r = {"p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+             "wait_p50_ms": round(float(np.percentile(wait, 50)), 4),
+             "wait_p99_ms": round(float(np.percentile(wait, 99)), 4),
+             "service_p50_ms": round(float(np.percentile(serv, 50)), 4),
+             "service_p99_ms": round(float(np.percentile(serv, 99)), 4),
              "mean_ms": round(float(lat.mean()), 4), "solves_per_s": round(n_req / st["wall_s"], 2),
              "sustained_per_s": round((n_req - 1) / float(done.max() - done.min()), 2),
              "peak_resident_GB": round(st["peak_resident_bytes"] / 1e9, 3)}
