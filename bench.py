@@ -413,10 +413,8 @@ def main():
     for s in range(K):
         e = ev[s]
         e[0].record(stream)
-        solver.stage_oop(f, P, D.STAGE_X_FWD)
-        e[1].record(stream)
-        for st in (D.STAGE_Z_FWD, D.STAGE_TRIDIAG, D.STAGE_Z_BWD, D.STAGE_X_BWD):
-            solver.stage(P, st)
+        for st in range(5):  # the five kernels of solve(f, P), one event after each
+            solver.solve_stage(f, P, st)
             e[st + 1].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -463,20 +461,25 @@ def main():
     dom = max(stage_ms, key=stage_ms.get)
     alg_bytes = 16.0 * N
     achieved = alg_bytes / (stage_ms[dom] * 1e-3) / 1e9
-    # measured HBM bytes of that kernel, per launch, for the variant the tuner picked
-    traffic = None
+    # measured HBM bytes of that kernel, per launch, for the variant the solve runs (rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes committed under profiles/, keyed by pipeline, stage and variant)
+    traffic, traffic_note = None, None
+    pipeline = "blocked" if solver.blocked[1] else "in_place"
+    variants = solver.solve_variants
     if args.traffic and os.path.exists(args.traffic):
         try:
             tj = json.load(open(args.traffic))
-            picked = {"x_fwd": tune["picked"][0], "x_bwd": tune["picked"][0], "z_fwd": tune["picked"][1],
-                      "z_bwd": tune["picked"][1], "tridiag": tune["picked"][2]}
-            per = tj.get("per_launch_bytes", {}).get(dom)
-            traffic = per.get(str(picked[dom])) if isinstance(per, dict) else per
-        except Exception:
-            traffic = None
+            per = tj["per_launch_bytes"][pipeline][dom]
+            traffic = per.get(str(variants[D.STAGE_NAMES.index(dom)]))
+            if traffic is None:
+                traffic_note = f"no counter pass for {pipeline} {dom} variant {variants[D.STAGE_NAMES.index(dom)]}"
+        except (KeyError, ValueError, TypeError) as e:
+            traffic_note = f"traffic profile unreadable: {e!r}"
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                "alg_bytes_per_launch": alg_bytes}
+                "alg_bytes_per_launch": alg_bytes, "pipeline": pipeline, "variants": list(variants)}
+    if traffic_note:
+        roofline["traffic_note"] = traffic_note
     solve_alg = 48.0 * N
     solve_roof = {"alg_bytes": solve_alg, "achieved": round(solve_alg / (solve_ms * 1e-3) / 1e9, 1),
                   "frac": round(solve_alg / (solve_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}

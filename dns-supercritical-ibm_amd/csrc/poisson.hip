@@ -355,6 +355,7 @@ static void poisson_release(dnsp_poisson* h) {
     (void)hipFree(h->d_twx);
     (void)hipFree(h->d_twz);
     (void)hipFree(h->d_cp);
+    (void)hipFree(h->d_W);
     (void)hipFree(h->d_host);
     mixed_release(h);
     if (cur != h->device) (void)hipSetDevice(cur);
@@ -456,6 +457,13 @@ static int poisson_apply_strategy(dnsp_poisson* h) {
         const size_t N = (size_t)h->nx * h->nz * h->ny;
         DNSP_HIP_TRY(hipMalloc(&h->d_cp, sizeof(double) * N));
     }
+    // the blocked solve: compile-time x and z transforms, nx a multiple of 16,
+    // the partitioned tridiagonal; its workspace is one field
+    h->blk_ok = h->fft_strategy == DNSP_FFT_AUTO && h->ct_x && h->ct_z && xb_supported(h->nx) && part;
+    if (h->blk_ok && !h->d_W) {
+        const size_t N = (size_t)h->nx * h->nz * h->ny;
+        DNSP_HIP_TRY(hipMalloc(&h->d_W, sizeof(double) * N));
+    }
     return DNSP_OK;
 }
 
@@ -494,9 +502,13 @@ int dnsp_poisson_set_variants(dnsp_poisson* h, int xvar, int zvar, int tvar) {
     DNSP_ARG_CHECK(xvar >= 0 && xvar < kCtVariants && zvar >= 0 && zvar < kCtVariants,
                    "FFT kernel variants must be in [0, %d)", kCtVariants);
     DNSP_ARG_CHECK(tvar >= 0 && tvar < kPartVariants, "tridiagonal variant must be in [0, %d)", kPartVariants);
-    h->xvar = xvar;
-    h->zvar = zvar;
+    h->xvar = h->xvar_b = xvar;
+    h->zvar = h->zvar_b = zvar;
     h->tvar = tvar;
+    // the blocked solve's knobs alike (its z kernels: 256 or 512 threads)
+    h->bxvar = h->bxvar_b = xvar;
+    h->bzvar = h->bzvar_b = ct_zvar_threads(zvar) == 512 ? 1 : 0;
+    h->btvar = tvar;
     return DNSP_OK;
 }
 
@@ -509,11 +521,12 @@ int dnsp_poisson_get_variants(const dnsp_poisson* h, int* xvar, int* zvar, int* 
     return DNSP_OK;
 }
 
-// Times `reps` forward+inverse pairs of each FFT kernel variant per direction,
-// and pairs of tridiagonal solves per partitioned variant, on `work`
-// (zero-filled first; NULL = a temporary device array) and keeps the fastest;
-// for a mixed-precision handle the same for its kernels.  times_ms (optional,
-// DNSP_TUNE_SLOTS entries): mean ms of the x, z (4 slots each) and
+// Times each FFT kernel variant per direction (`reps` forward transforms,
+// then `reps` inverse ones, keeping the fastest of each) and pairs of
+// tridiagonal solves per partitioned variant, on `work` (zero-filled first;
+// NULL = a temporary device array); for a mixed-precision handle the same
+// for its kernels.  times_ms (optional, DNSP_TUNE_SLOTS entries): mean ms of
+// a forward + inverse pair of the x, z (4 slots each) and of a pair of
 // tridiagonal (8 slots) variants, then the same for the mixed-precision
 // kernels; -1 where not applicable.
 static_assert(kCtVariants == 4 && kPartVariants == 8 && DNSP_TUNE_SLOTS == 2 * kTuneFormSlots, "tuner table layout");
@@ -527,11 +540,29 @@ int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream,
     if (!buf) DNSP_HIP_TRY(hipMalloc(&buf, sizeof(double) * N));
     int rc = DNSP_OK;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    double best[3] = {1e30, 1e30, 1e30};
-    int pick[3] = {h->xvar, h->zvar, h->tvar};
-    const int keep_x = h->xvar, keep_z = h->zvar, keep_t = h->tvar;
+    // slots: x fwd, x bwd, z fwd, z bwd, tridiagonal
+    double best[5] = {1e30, 1e30, 1e30, 1e30, 1e30};
+    int pick[5] = {h->xvar, h->xvar_b, h->zvar, h->zvar_b, h->tvar};
+    const int keep[5] = {h->xvar, h->xvar_b, h->zvar, h->zvar_b, h->tvar};
+    int* knob[5] = {&h->xvar, &h->xvar_b, &h->zvar, &h->zvar_b, &h->tvar};
     if (times_ms)
         for (int i = 0; i < DNSP_TUNE_SLOTS; ++i) times_ms[i] = -1.0;
+    auto timed = [&](int stage, int n, double* per) -> int {
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < n; ++r) {
+            const int c = dnsp_poisson_stage(h, buf, buf, stage, stream);
+            if (c) return c;
+        }
+        (void)hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            set_error("autotune: event synchronize failed");
+            return DNSP_ERR_HIP;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *per = (double)ms / n;
+        return DNSP_OK;
+    };
     do {
         if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
             hipMemsetAsync(buf, 0, sizeof(double) * N, st) != hipSuccess) {
@@ -543,48 +574,81 @@ int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream,
             const bool has = dir == 0 ? h->ct_x : dir == 1 ? h->ct_z : h->part_rows > 0;
             if (!has) continue;
             const int nvar = dir == 2 ? kPartVariants : kCtVariants;
+            const int sf = dir == 0 ? DNSP_STAGE_X_FWD : dir == 1 ? DNSP_STAGE_Z_FWD : DNSP_STAGE_TRIDIAG;
+            const int sb = dir == 0 ? DNSP_STAGE_X_BWD : dir == 1 ? DNSP_STAGE_Z_BWD : DNSP_STAGE_TRIDIAG;
+            const int kf = 2 * dir, kb = dir == 2 ? 4 : 2 * dir + 1;  // knob slots
             for (int v = 0; v < nvar && rc == DNSP_OK; ++v) {
-                if (dir == 0) h->xvar = v; else if (dir == 1) h->zvar = v; else h->tvar = v;
-                const int sf = dir == 0 ? DNSP_STAGE_X_FWD : dir == 1 ? DNSP_STAGE_Z_FWD : DNSP_STAGE_TRIDIAG;
-                const int sb = dir == 0 ? DNSP_STAGE_X_BWD : dir == 1 ? DNSP_STAGE_Z_BWD : DNSP_STAGE_TRIDIAG;
+                *knob[kf] = v;
+                *knob[kb] = v;
                 rc = dnsp_poisson_stage(h, buf, buf, sf, stream);  // warm-up
                 if (!rc) rc = dnsp_poisson_stage(h, buf, buf, sb, stream);
+                double tf = 0.0, tb = 0.0;
+                if (!rc) rc = timed(sf, reps, &tf);
+                if (!rc && dir < 2) rc = timed(sb, reps, &tb);
                 if (rc) break;
-                (void)hipEventRecord(e0, st);
-                for (int r = 0; r < reps && !rc; ++r) {
-                    rc = dnsp_poisson_stage(h, buf, buf, sf, stream);
-                    if (!rc) rc = dnsp_poisson_stage(h, buf, buf, sb, stream);
+                if (dir == 2) tf *= 2.0;  // a pair of tridiagonal solves, as the FFT pairs
+                if (times_ms) times_ms[kTuneSlot[dir] + v] = tf + tb;
+                if (tf < best[kf]) {
+                    best[kf] = tf;
+                    pick[kf] = v;
                 }
-                if (rc) break;
-                (void)hipEventRecord(e1, st);
-                if (hipEventSynchronize(e1) != hipSuccess) {
-                    set_error("autotune: event synchronize failed");
-                    rc = DNSP_ERR_HIP;
-                    break;
-                }
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                const double per = (double)ms / reps;
-                if (times_ms) times_ms[kTuneSlot[dir] + v] = per;
-                if (per < best[dir]) {
-                    best[dir] = per;
-                    pick[dir] = v;
+                if (dir < 2 && tb < best[kb]) {
+                    best[kb] = tb;
+                    pick[kb] = v;
                 }
             }
         }
+        if (rc || !h->blk_ok) break;
+        // the blocked solve's own knobs, timed stage by stage through
+        // dnsp_poisson_solve_stage on buf (rhs and solution) and the workspace
+        if (hipMemsetAsync(h->d_W, 0, sizeof(double) * N, st) != hipSuccess) {
+            set_error("autotune: memset failed");
+            rc = DNSP_ERR_HIP;
+            break;
+        }
+        const bool on = h->blk_on;
+        h->blk_on = true;
+        auto timed_b = [&](int stage, double* per) -> int {
+            int c = dnsp_poisson_solve_stage(h, buf, buf, stage, stream);  // warm-up
+            (void)hipEventRecord(e0, st);
+            for (int r = 0; r < reps && !c; ++r) c = dnsp_poisson_solve_stage(h, buf, buf, stage, stream);
+            if (c) return c;
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                set_error("autotune: event synchronize failed");
+                return DNSP_ERR_HIP;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            *per = (double)ms / reps;
+            return DNSP_OK;
+        };
+        int* bk[5] = {&h->bxvar, &h->bzvar, &h->btvar, &h->bzvar_b, &h->bxvar_b};  // by stage
+        const int nv[5] = {kCtVariants, 2, kPartVariants, 2, kCtVariants};
+        for (int stage = 0; stage < 5 && !rc; ++stage) {
+            double bt = 1e30;
+            int bv = *bk[stage];
+            for (int v = 0; v < nv[stage] && !rc; ++v) {
+                *bk[stage] = v;
+                double t = 0.0;
+                rc = timed_b(stage, &t);
+                if (!rc && t < bt) {
+                    bt = t;
+                    bv = v;
+                }
+            }
+            *bk[stage] = bv;
+        }
+        h->blk_on = on;
     } while (0);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (rc) {
         if (!work && buf) (void)hipFree(buf);
-        h->xvar = keep_x;
-        h->zvar = keep_z;
-        h->tvar = keep_t;
+        for (int i = 0; i < 5; ++i) *knob[i] = keep[i];
         return rc;
     }
-    h->xvar = pick[0];
-    h->zvar = pick[1];
-    h->tvar = pick[2];
+    for (int i = 0; i < 5; ++i) *knob[i] = pick[i];
     if (h->precision == DNSP_PRECISION_MIXED) rc = mixed_autotune(h, buf, reps, st, times_ms ? times_ms + kTuneFormSlots : nullptr);
     if (!work && buf) (void)hipFree(buf);
     return rc;
@@ -643,9 +707,13 @@ int dnsp_poisson_stage(dnsp_poisson* h, const double* src, double* P, int stage,
     const size_t ldsx = sizeof(double2) * h->Gx * h->nx;
     const size_t ldsz = sizeof(double2) * h->TAz * h->nz;
     const double inv_nxnz = 1.0 / ((double)h->nx * (double)h->nz + 0.0);
+    // the compile-time x kernels move (re, im) pairs as 16-byte accesses: an
+    // array that is only 8-byte aligned (a view offset by one double, a
+    // Fortran section) takes the runtime-sized kernels, which access doubles
+    const bool x16 = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(P)) & 15) == 0;
     switch (stage) {
         case DNSP_STAGE_X_FWD:
-            if (h->ct_x) ct_x_launch(h->nx, h->xvar, false, src, P, nlines, h->d_twx, 1.0, st);
+            if (h->ct_x && x16) ct_x_launch(h->nx, h->xvar, false, src, P, nlines, h->d_twx, 1.0, st);
             else k_xfwd<<<gx, kFftThreads, ldsx, st>>>(src, P, h->nx, nlines, h->Gx, h->px);
             break;
         case DNSP_STAGE_Z_FWD:
@@ -665,11 +733,11 @@ int dnsp_poisson_stage(dnsp_poisson* h, const double* src, double* P, int stage,
             break;
         }
         case DNSP_STAGE_Z_BWD:
-            if (h->ct_z) ct_z_launch(h->nz, h->zvar, true, P, h->nx, h->ny, h->d_twz, st);
+            if (h->ct_z) ct_z_launch(h->nz, h->zvar_b, true, P, h->nx, h->ny, h->d_twz, st);
             else k_zbwd<<<gz, kFftThreads, ldsz, st>>>(P, h->nx, h->nz, h->ncol, h->TAz, h->pz);
             break;
         case DNSP_STAGE_X_BWD:
-            if (h->ct_x) ct_x_launch(h->nx, h->xvar, true, src, P, nlines, h->d_twx, inv_nxnz, st);
+            if (h->ct_x && x16) ct_x_launch(h->nx, h->xvar_b, true, src, P, nlines, h->d_twx, inv_nxnz, st);
             else k_xbwd<<<gx, kFftThreads, ldsx, st>>>(src, P, h->nx, nlines, h->Gx, h->px, inv_nxnz);
             break;
         default:
@@ -705,23 +773,74 @@ int dnsp_poisson_set_form(dnsp_poisson* h, int refinements) {
     return DNSP_OK;
 }
 
-int dnsp_poisson_run(dnsp_poisson* h, double* P, void* stream) {
-    if (h && h->form >= 0) return dnsp_poisson_run_mixed(h, P, h->form, stream);
-    int rc = dnsp_poisson_forward(h, P, stream);
-    if (rc) return rc;
-    rc = dnsp_poisson_tridiag(h, P, stream);
-    if (rc) return rc;
-    return dnsp_poisson_backward(h, P, stream);
+int dnsp_poisson_solve_stage(dnsp_poisson* h, const double* rhs, double* P, int stage, void* stream) {
+    clear_error();
+    DNSP_ARG_CHECK(h && rhs && P, "NULL handle or array");
+    DNSP_ARG_CHECK(stage >= 0 && stage < 5, "stage %d not in [0, 5)", stage);
+    const bool blk = h->blk_ok && h->blk_on &&
+                     ((reinterpret_cast<uintptr_t>(rhs) | reinterpret_cast<uintptr_t>(P)) & 15) == 0;
+    if (!blk) {
+        // the in-place stages; the first one reads rhs
+        const int st_of[5] = {DNSP_STAGE_X_FWD, DNSP_STAGE_Z_FWD, DNSP_STAGE_TRIDIAG, DNSP_STAGE_Z_BWD,
+                              DNSP_STAGE_X_BWD};
+        return dnsp_poisson_stage(h, stage == 0 ? rhs : P, P, st_of[stage], stream);
+    }
+    hipStream_t st = as_stream(stream);
+    const long nlines = (long)h->nz * h->ny;
+    const double inv_nxnz = 1.0 / ((double)h->nx * (double)h->nz + 0.0);
+    // P lines -> W x-blocked -> P mode-major -> W x-blocked -> P lines -> P
+    switch (stage) {
+        case 0: xb_x_launch(h->nx, h->bxvar, rhs, h->d_W, nlines, h->nz, h->d_twx, st); break;
+        case 1: mm_z_launch(h->nz, h->bzvar, false, h->d_W, P, h->nx, h->ny, h->d_twz, st); break;
+        case 2:
+            part_launch<double, double, true>(h->btvar, P, h->d_ax, h->d_az, h->d_bands, h->nx, h->nz, h->ny,
+                                              h->pcr_min, nullptr, nullptr, 0.0, st, nullptr, 0, h->d_W);
+            break;
+        case 3: mm_z_launch(h->nz, h->bzvar_b, true, h->d_W, P, h->nx, h->ny, h->d_twz, st); break;
+        default: ct_x_launch(h->nx, h->bxvar_b, true, P, P, nlines, h->d_twx, inv_nxnz, st); break;
+    }
+    DNSP_LAUNCH_CHECK();
+    return DNSP_OK;
 }
 
 int dnsp_poisson_solve(dnsp_poisson* h, const double* rhs, double* P, void* stream) {
-    int rc = dnsp_poisson_stage(h, rhs, P, DNSP_STAGE_X_FWD, stream);
-    if (rc) return rc;
-    rc = dnsp_poisson_stage(h, P, P, DNSP_STAGE_Z_FWD, stream);
-    if (rc) return rc;
-    rc = dnsp_poisson_tridiag(h, P, stream);
-    if (rc) return rc;
-    return dnsp_poisson_backward(h, P, stream);
+    for (int s = 0; s < 5; ++s) {
+        const int rc = dnsp_poisson_solve_stage(h, rhs, P, s, stream);
+        if (rc) return rc;
+    }
+    return DNSP_OK;
+}
+
+int dnsp_poisson_run(dnsp_poisson* h, double* P, void* stream) {
+    if (h && h->form >= 0) return dnsp_poisson_run_mixed(h, P, h->form, stream);
+    return dnsp_poisson_solve(h, P, P, stream);
+}
+
+int dnsp_poisson_set_blocked(dnsp_poisson* h, int on) {
+    clear_error();
+    DNSP_ARG_CHECK(h, "NULL handle");
+    h->blk_on = on != 0;
+    return DNSP_OK;
+}
+
+int dnsp_poisson_get_solve_variants(const dnsp_poisson* h, int* v) {
+    clear_error();
+    DNSP_ARG_CHECK(h && v, "NULL argument");
+    const bool blk = h->blk_ok && h->blk_on;
+    v[0] = blk ? h->bxvar : h->xvar;
+    v[1] = blk ? h->bzvar : h->zvar;
+    v[2] = blk ? h->btvar : h->tvar;
+    v[3] = blk ? h->bzvar_b : h->zvar_b;
+    v[4] = blk ? h->bxvar_b : h->xvar_b;
+    return DNSP_OK;
+}
+
+int dnsp_poisson_get_blocked(const dnsp_poisson* h, int* supported, int* on) {
+    clear_error();
+    DNSP_ARG_CHECK(h, "NULL handle");
+    if (supported) *supported = h->blk_ok ? 1 : 0;
+    if (on) *on = (h->blk_ok && h->blk_on) ? 1 : 0;
+    return DNSP_OK;
 }
 
 int dnsp_poisson_run_host(dnsp_poisson* h, double* P_host) {

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <new>
@@ -73,11 +74,11 @@ __device__ __forceinline__ int find_inst(const long* __restrict__ start, int cnt
 enum BxKind { BX_F64_FWD = 0, BX_F64_BWD, BX_MX_FWD, BX_MX_RES, BX_MX_BWD, BX_MX_ACC, BX_MX_OUT0 };
 
 template <int N, int V, int KIND>
-__global__ __launch_bounds__(ct_var_threads(V)) void kb_x(const BInst* __restrict__ inst,
+__global__ __launch_bounds__(ct_xvar_threads(V)) void kb_x(const BInst* __restrict__ inst,
                                                           const int* __restrict__ ids,
                                                           const long* __restrict__ start, int cnt, int sums) {
-    constexpr int G = ct_lines<N, ct_var_tile(V)>();
-    constexpr int NT = ct_var_threads(V);
+    constexpr int G = ct_lines<N, ct_xvar_tile(V)>();
+    constexpr int NT = ct_xvar_threads(V);
     constexpr bool F64 = KIND == BX_F64_FWD || KIND == BX_F64_BWD;
     using VT = typename std::conditional<F64, double2, float2>::type;
     // the residual kind may stage 2G + 2 lines of its instance (mr_lds_f2)
@@ -120,11 +121,11 @@ __global__ __launch_bounds__(ct_var_threads(V)) void kb_x(const BInst* __restric
 enum BzKind { BZ_F64_FWD = 0, BZ_F64_BWD, BZ_A_FWD, BZ_A_BWD, BZ_B_FWD, BZ_B_BWD };
 
 template <int N, int V, int KIND>
-__global__ __launch_bounds__(ct_var_threads(V)) void kb_z(const BInst* __restrict__ inst,
+__global__ __launch_bounds__(ct_zvar_threads(V)) void kb_z(const BInst* __restrict__ inst,
                                                           const int* __restrict__ ids,
                                                           const long* __restrict__ start, int cnt) {
-    constexpr int TA = ct_cols<N, ct_var_tile(V)>();
-    constexpr int NT = ct_var_threads(V);
+    constexpr int TA = ct_cols<N, ct_zvar_tile(V)>();
+    constexpr int NT = ct_zvar_threads(V);
     constexpr bool F64 = KIND == BZ_F64_FWD || KIND == BZ_F64_BWD;
     using T = typename std::conditional<F64, double, float>::type;
     using VT = typename Cx<T>::V;
@@ -250,7 +251,7 @@ __global__ __launch_bounds__(64) void kb_q(const BInst* __restrict__ inst, const
 template <int N, int V>
 static void bx_launch_nv(int kind, const BInst* inst, const BGroup& g, int sums, hipStream_t st) {
     const int nb = (int)g.total;
-    constexpr int NT = ct_var_threads(V);
+    constexpr int NT = ct_xvar_threads(V);
     switch (kind) {
         case BX_F64_FWD: kb_x<N, V, BX_F64_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
         case BX_F64_BWD: kb_x<N, V, BX_F64_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
@@ -285,7 +286,7 @@ static void bx_launch(int kind, int v, const BInst* inst, const BGroup& g, int s
 template <int N, int V>
 static void bz_launch_nv(int kind, const BInst* inst, const BGroup& g, hipStream_t st) {
     const int nb = (int)g.total;
-    constexpr int NT = ct_var_threads(V);
+    constexpr int NT = ct_zvar_threads(V);
     switch (kind) {
         case BZ_F64_FWD: kb_z<N, V, BZ_F64_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
         case BZ_F64_BWD: kb_z<N, V, BZ_F64_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
@@ -466,7 +467,7 @@ int make_groups(dnsp_batch* b, const std::vector<int>& key, const std::vector<lo
 }
 
 int variant_tasks(int n, int v, bool xdir) {
-    const int tile = ct_var_tile(v);
+    const int tile = xdir ? ct_xvar_tile(v) : ct_zvar_tile(v);
     if (xdir) {
         const int G = std::max(1, tile / n);
         return G;
@@ -715,6 +716,9 @@ int dnsp_batch_bind(dnsp_batch* b, const double* const* rhs, double* const* P) {
     for (int i = 0; i < b->n; ++i) {
         DNSP_ARG_CHECK(rhs[i] && P[i], "instance %d: NULL array", i);
         DNSP_ARG_CHECK(rhs[i] != P[i], "instance %d: rhs and P must be distinct arrays", i);
+        // the x kernels move (re, im) pairs as 16-byte accesses
+        DNSP_ARG_CHECK(((reinterpret_cast<uintptr_t>(rhs[i]) | reinterpret_cast<uintptr_t>(P[i])) & 15) == 0,
+                       "instance %d: rhs and P must be 16-byte aligned", i);
     }
     for (int i = 0; i < b->n; ++i) {
         b->h_inst[(size_t)i].f = rhs[i];

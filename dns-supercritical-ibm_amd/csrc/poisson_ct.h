@@ -11,11 +11,17 @@
 
 namespace dnsp {
 
-// FFT kernel variants: (complex elements per workgroup tile, threads).
-// The tuner picks one per direction and handle.
+// FFT kernel variants: (complex elements per workgroup tile, threads), one
+// table per direction.  The tuner picks one per direction and handle.  x
+// variant 0 is one complex line (two real lines) per workgroup at nx = 768:
+// the smallest tiles keep the most workgroups in flight, which the
+// contiguous x lines turn into bandwidth (x forward 0.374 -> 0.336 ms at
+// 768x384x384 against two complex lines).
 constexpr int kCtVariants = 4;
-__host__ __device__ constexpr int ct_var_tile(int v) { return v == 0 ? 3072 : v == 1 ? 3072 : v == 2 ? 1536 : 6144; }
-__host__ __device__ constexpr int ct_var_threads(int v) { return v == 0 ? 256 : v == 1 ? 512 : v == 2 ? 256 : 512; }
+__host__ __device__ constexpr int ct_xvar_tile(int v) { return v == 0 ? 768 : v == 1 ? 3072 : v == 2 ? 1536 : 6144; }
+__host__ __device__ constexpr int ct_xvar_threads(int v) { return v == 0 ? 256 : v == 1 ? 512 : v == 2 ? 256 : 512; }
+__host__ __device__ constexpr int ct_zvar_tile(int v) { return v == 0 ? 3072 : v == 1 ? 3072 : v == 2 ? 1536 : 6144; }
+__host__ __device__ constexpr int ct_zvar_threads(int v) { return v == 0 ? 256 : v == 1 ? 512 : v == 2 ? 256 : 512; }
 
 template <int N, int TILE>
 constexpr int ct_lines() {  // complex lines (= 2 real lines) per x workgroup
@@ -32,9 +38,12 @@ constexpr int ct_cols() {  // complex columns per z workgroup, at most 16
 // Bodies take the tile index and the LDS tile (G * ct_pitch(N) complex
 // values); the __global__ wrappers pick the tile with xcd_tile.  All threads
 // of the workgroup call a body.
-template <int N, int G, int NT>
+// XB = true: S receives the x-blocked layout of the blocked solve,
+// [k][a/16][j][a%16] (line k*nz + j, halfcomplex index a; nx a multiple of
+// 16), instead of the lines in place.
+template <int N, int G, int NT, bool XB = false>
 __device__ __forceinline__ void xfwd_body(const double* P, double* S, long nlines, const double2* __restrict__ tw,
-                                          long tile, double2* sm) {
+                                          long tile, double2* sm, int nz = 0) {
     constexpr int LP = ct_pitch(N);
     constexpr int NL = (G * N + NT - 1) / NT;  // double2 loads per thread
     double* smd = reinterpret_cast<double*>(sm);
@@ -79,7 +88,15 @@ __device__ __forceinline__ void xfwd_body(const double* P, double* S, long nline
                 if ((r & 1) == 0) out[h] = lo ? 0.5 * (C.x + Cm.x) : 0.5 * (C.y - Cm.y);
                 else              out[h] = lo ? 0.5 * (C.y + Cm.y) : 0.5 * (Cm.x - C.x);
             }
-            dst[e] = make_double2(out[0], out[1]);
+            if constexpr (XB) {
+                const long line = l0 + r;
+                const long k = line / nz;
+                const long j = line - k * nz;
+                *reinterpret_cast<double2*>(S + ((k * (N / 16) + (t0 >> 4)) * nz + j) * 16 + (t0 & 15)) =
+                    make_double2(out[0], out[1]);
+            } else {
+                dst[e] = make_double2(out[0], out[1]);
+            }
         }
     }
 }
@@ -104,21 +121,66 @@ __device__ __forceinline__ void xbwd_body(const double* S, double* P, long nline
     const long l0 = tile * 2 * G;
     const double* src = S + l0 * N;
     double xr[NI], xi[NI], yr[NI], yi[NI];
+    if ((((size_t)src) & 15) == 0) {
+        // 16-byte aligned tile: the 2G lines are staged in LDS (inside sm,
+        // which they fit) with contiguous 16-byte loads from clamped lines,
+        // and the t / N - t gather reads LDS instead of 8-byte global gathers
+        constexpr int NV = G * N;  // double2 of the tile
+        constexpr int NQ = (NV + NT - 1) / NT;
+        double2 v[NQ];
 #pragma unroll
-    for (int q = 0; q < NI; ++q) {
-        const int e = (int)threadIdx.x + q * NT;
-        const int g = e / H;
-        const int t = e - g * H;
-        const bool ok = e < G * H;
-        const bool single = (t == 0) || (2 * t == N);
-        const bool va = ok && (l0 + 2 * g < nlines);
-        const bool vb = ok && (l0 + 2 * g + 1 < nlines);
-        const double* la = src + (2 * g) * N;
-        const double* lb = la + N;
-        xr[q] = va ? la[t] : 0.0;
-        xi[q] = (va && !single) ? la[N - t] : 0.0;
-        yr[q] = vb ? lb[t] : 0.0;
-        yi[q] = (vb && !single) ? lb[N - t] : 0.0;
+        for (int q = 0; q < NQ; ++q) {
+            int e = (int)threadIdx.x + q * NT;
+            e = (NV % NT == 0 || e < NV) ? e : 0;
+            const int line = (2 * e) / N;
+            const int off = 2 * e - line * N;
+            const int ln = l0 + line < nlines ? line : 0;
+            v[q] = *reinterpret_cast<const double2*>(src + (long)ln * N + off);
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            if (NV % NT == 0 || e < NV) sm[e] = v[q];
+        }
+        __syncthreads();
+        const double* sd = reinterpret_cast<const double*>(sm);
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            const bool ok = e < G * H;
+            const int ee = ok ? e : 0;
+            const int g = ee / H;
+            const int t = ee - g * H;
+            const bool single = (t == 0) || (2 * t == N);
+            const bool va = ok && (l0 + 2 * g < nlines);
+            const bool vb = ok && (l0 + 2 * g + 1 < nlines);
+            const int tm = single ? t : N - t;
+            const double* la = sd + (2 * g) * N;
+            const double* lb = la + N;
+            const double a0 = la[t], a1 = la[tm], b0 = lb[t], b1 = lb[tm];
+            xr[q] = va ? a0 : 0.0;
+            xi[q] = (va && !single) ? a1 : 0.0;
+            yr[q] = vb ? b0 : 0.0;
+            yi[q] = (vb && !single) ? b1 : 0.0;
+        }
+        __syncthreads();  // staged lines read; sm becomes the FFT lines
+    } else {
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            const int e = (int)threadIdx.x + q * NT;
+            const int g = e / H;
+            const int t = e - g * H;
+            const bool ok = e < G * H;
+            const bool single = (t == 0) || (2 * t == N);
+            const bool va = ok && (l0 + 2 * g < nlines);
+            const bool vb = ok && (l0 + 2 * g + 1 < nlines);
+            const double* la = src + (2 * g) * N;
+            const double* lb = la + N;
+            xr[q] = va ? la[t] : 0.0;
+            xi[q] = (va && !single) ? la[N - t] : 0.0;
+            yr[q] = vb ? lb[t] : 0.0;
+            yi[q] = (vb && !single) ? lb[N - t] : 0.0;
+        }
     }
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
@@ -386,6 +448,128 @@ __global__ __launch_bounds__(NT) void kt_zbwd(T* S, int nx, int ncol, int ntc,
 }
 
 // ---------------------------------------------------------------------------
+// Blocked solve (nx a multiple of 16), through a workspace W.  Every pass
+// reads contiguous memory and, where it must transpose, scatters its writes
+// (a scattered read waits for its round trip, a scattered write does not):
+//   x forward   P lines -> W  x-blocked [k][a/16][j][a%16]      (kt_xfwd_xb)
+//   z forward   W       -> P  mode-major [a/16][b][k][a%16]     (kt_zfwd_mm)
+//   tridiagonal P       -> W  x-blocked, from whole ny x 16 blocks
+//   z inverse   W       -> P  halfcomplex lines                 (kt_zbwd_xl)
+//   x inverse   P lines, in place                               (kt_xbwd)
+// A z tile (plane k, halfcomplex x rows 16c..16c+15) is one contiguous block
+// of nz * 16 values in W, and a tridiagonal workgroup reads one contiguous
+// block of ny * 16 values.  Same transforms and arithmetic as the in-place
+// kernels above, only the addresses differ.
+// ---------------------------------------------------------------------------
+template <int N, int G, int NT>
+__global__ __launch_bounds__(NT) void kt_xfwd_xb(const double* P, double* W, long nlines, int nz,
+                                                 const double2* __restrict__ tw) {
+    __shared__ double2 sm[G * ct_pitch(N)];
+    xfwd_body<N, G, NT, true>(P, W, nlines, tw, xcd_tile(blockIdx.x, gridDim.x), sm, nz);
+}
+
+// z forward, XB -> MM.  Tile (k, c): the 16 x rows 16c..16c+15 of plane k as
+// 8 complex columns (rows 2t and 2t + 1 paired as in kt_zfwd); N = nz.
+template <int N, int NT>
+__global__ __launch_bounds__(NT) void kt_zfwd_mm(const double* __restrict__ W, double* __restrict__ S, int nx,
+                                                 int ny, const double2* __restrict__ tw) {
+    constexpr int TA = 8;
+    constexpr int LP = ct_pitch(N);
+    constexpr int TOT2 = TA * N;
+    constexpr int NL2 = (TOT2 + NT - 1) / NT;
+    __shared__ double2 sm[TA * LP];
+    const int nab = nx / 16;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = tile / nab;
+    const int c = tile - k * nab;
+    // the tile is one contiguous block: pair e = (j, t) sits at 2e
+    const double2* src = reinterpret_cast<const double2*>(W + ((size_t)k * nab + c) * N * 16);
+    double2 v[NL2];
+#pragma unroll
+    for (int q = 0; q < NL2; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        v[q] = (TOT2 % NT == 0 || e < TOT2) ? src[e] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < NL2; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        if (TOT2 % NT == 0 || e < TOT2) sm[(e % TA) * LP + ct_pad(e / TA)] = v[q];
+    }
+    __syncthreads();
+    fft_ct<N, TA, NT, false>(sm, tw);
+    double* dst = S + (size_t)c * N * ny * 16 + (size_t)k * 16;
+#pragma unroll
+    for (int q = 0; q < NL2; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int tt = e % TA;
+        const int b = e / TA;
+        if (TOT2 % NT == 0 || e < TOT2) {
+            const bool lo = 2 * b <= N;
+            const int qq = lo ? b : N - b;
+            const double2 Z = sm[tt * LP + ct_pad(qq)];
+            const double2 Zm = sm[tt * LP + ct_pad(qq == 0 ? 0 : N - qq)];
+            const double ve = lo ? 0.5 * (Z.x + Zm.x) : 0.5 * (Z.y - Zm.y);
+            const double vo = lo ? 0.5 * (Z.y + Zm.y) : 0.5 * (Zm.x - Z.x);
+            *reinterpret_cast<double2*>(dst + (size_t)b * ny * 16 + 2 * tt) = make_double2(ve, vo);
+        }
+    }
+}
+
+// z inverse, x-blocked -> halfcomplex lines: the gather of rows q and N - q of
+// kt_zbwd from the tile's contiguous block of W; N = nz.
+template <int N, int NT>
+__global__ __launch_bounds__(NT) void kt_zbwd_xl(const double* __restrict__ W, double* __restrict__ P, int nx,
+                                                 int ny, const double2* __restrict__ tw) {
+    constexpr int TA = 8;
+    constexpr int LP = ct_pitch(N);
+    constexpr int H = N / 2 + 1;
+    constexpr int NI = (H * TA + NT - 1) / NT;
+    constexpr int TOT2 = TA * N;
+    constexpr int NL2 = (TOT2 + NT - 1) / NT;
+    __shared__ double2 sm[TA * LP];
+    const int nab = nx / 16;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = tile / nab;
+    const int c = tile - k * nab;
+    const double* src = W + ((size_t)k * nab + c) * N * 16;
+    double2 pu[NI], pi[NI];
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int ee = e < H * TA ? e : 0;
+        const int tt = ee % TA;
+        const int qq = ee / TA;
+        const int qm = (qq == 0 || 2 * qq == N) ? qq : N - qq;
+        pu[q] = *reinterpret_cast<const double2*>(src + (size_t)qq * 16 + 2 * tt);
+        pi[q] = *reinterpret_cast<const double2*>(src + (size_t)qm * 16 + 2 * tt);
+    }
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        if (e < H * TA) {
+            const int tt = e % TA;
+            const int qq = e / TA;
+            const bool single = (qq == 0) || (2 * qq == N);
+            const double ur = pu[q].x, vr = pu[q].y;
+            const double ui = single ? 0.0 : pi[q].x, vi = single ? 0.0 : pi[q].y;
+            sm[tt * LP + ct_pad(qq)] = make_double2(ur - vi, ui + vr);
+            if (!single) sm[tt * LP + ct_pad(N - qq)] = make_double2(ur + vi, vr - ui);
+        }
+    }
+    __syncthreads();
+    fft_ct<N, TA, NT, true>(sm, tw);
+    double* dst = P + (size_t)k * nx * N + (size_t)c * 16;
+#pragma unroll
+    for (int q = 0; q < NL2; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        const int tt = e % TA;
+        const int j = e / TA;
+        if (TOT2 % NT == 0 || e < TOT2)
+            *reinterpret_cast<double2*>(dst + (size_t)j * nx + 2 * tt) = sm[tt * LP + ct_pad(j)];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // y: partitioned tridiagonal solve, one workgroup = M consecutive modes x P
 // chunks of MR rows.  Thread (mode mi, chunk p) keeps its MR right-hand sides
 // in registers; rows past ny are identity rows (a = c = 0, b = 1, d = 0).
@@ -441,9 +625,42 @@ __device__ __forceinline__ float keep_or_zero(float v, bool keep) {
 // modes are all well conditioned.  When msum is not NULL the (0,0) mode is
 // solved from the FP64 plane sums msum[k] instead of S, its solution times
 // inv_nxnz goes to pbar[k] and its S entries are set to zero.
-template <int MR, int P, int M, typename T, typename R = double>
+// Element offset of row 0 of mode md (= a + nx b) and the stride between its
+// rows: the reference layout (md + row * plane), or MM = true, the
+// mode-major blocks [a/16][b][row][a%16] of the blocked solve (nx a
+// multiple of 16), where a workgroup's 16 consecutive modes are one
+// contiguous block of ny * 16 values (read; the rows go out to So in the
+// x-blocked layout, tri_addr_out).
+template <bool MM>
+__device__ __forceinline__ void tri_addr(long md, long plane, int nx, int nz, int ny, size_t& base, size_t& rs) {
+    if constexpr (MM) {
+        const unsigned um = (unsigned)md;
+        const unsigned a = um % (unsigned)nx, b = um / (unsigned)nx;
+        base = ((size_t)(a >> 4) * (size_t)nz + b) * (size_t)ny * 16 + (a & 15);
+        rs = 16;
+    } else {
+        base = (size_t)md;
+        rs = (size_t)plane;
+    }
+}
+
+// Where a lane stores its rows: in place (reference layout), or MM = true,
+// the x-blocked layout [k][a/16][b][a%16] of the blocked solve's z inverse.
+template <bool MM>
+__device__ __forceinline__ void tri_addr_out(long md, long plane, int nx, int nz, int ny, size_t& base, size_t& rs) {
+    if constexpr (MM) {
+        const unsigned um = (unsigned)md;
+        const unsigned a = um % (unsigned)nx, b = um / (unsigned)nx;
+        base = ((size_t)(a >> 4) * (size_t)nz + b) * 16 + (a & 15);
+        rs = (size_t)plane;
+    } else {
+        tri_addr<false>(md, plane, nx, nz, ny, base, rs);
+    }
+}
+
+template <int MR, int P, int M, typename T, typename R = double, bool MM = false>
 __device__ __forceinline__ void tri_load_rows(R (&d)[MR], const T* S, long mode, long plane, int ny, int s,
-                                              const double* __restrict__ msum) {
+                                              const double* __restrict__ msum, int nx = 0, int nz = 0) {
     const bool valid = mode < plane;
     const bool mean = msum != nullptr && mode == 0;
     // Branch-free: every row load goes to a valid address (row and mode
@@ -454,11 +671,13 @@ __device__ __forceinline__ void tri_load_rows(R (&d)[MR], const T* S, long mode,
     // fuse into the elimination's first multiply-add (which would change the
     // rounding); masked rows are +0.
     const long md = valid ? mode : 0;
+    size_t base, rs;
+    tri_addr<MM>(md, plane, nx, nz, ny, base, rs);
     T raw[MR];
 #pragma unroll
     for (int r = 0; r < MR; ++r) {
         const int row = s + r < ny ? s + r : ny - 1;
-        raw[r] = S[(size_t)row * plane + md];
+        raw[r] = S[base + (size_t)row * rs];
     }
 #pragma unroll
     for (int r = 0; r < MR; ++r) d[r] = keep_or_zero((R)raw[r], valid && !mean && s + r < ny);
@@ -503,11 +722,12 @@ __device__ __forceinline__ void tri_stage_bands(Ar* smb, const double* __restric
 // known the interior rows are solved again as a tridiagonal system with those
 // two values as boundary data.  About a third fewer values stay live across
 // the reduced solve, for one more elimination of the chunk.
-template <int MR, int P, int M, typename T, bool TP = false, typename Ar = double>
+template <int MR, int P, int M, typename T, bool TP = false, typename Ar = double, bool MM = false>
 __device__ __forceinline__ void tri_group(Ar (&d)[MR], T* S, const double* __restrict__ ax,
                                           const double* __restrict__ az, const Ar* smb, int nx, int ny,
                                           long plane, double pcr_min, long mode, const double* __restrict__ msum,
-                                          double* __restrict__ pbar, double inv_nxnz, TriRed<P, M, Ar>& R) {
+                                          double* __restrict__ pbar, double inv_nxnz, TriRed<P, M, Ar>& R,
+                                          int nz = 0, T* So = nullptr) {
     const int mi = (int)threadIdx.x % M;
     const int p = (int)threadIdx.x / M;
     const int s = p * MR;
@@ -708,15 +928,17 @@ __device__ __forceinline__ void tri_group(Ar (&d)[MR], T* S, const double* __res
         const bool mean = msum != nullptr && mode == 0;
         // addresses from one offset the compiler cannot relate to the loads'
         // (otherwise it keeps every row index live across the solve)
-        size_t off = (size_t)s * plane + md;
+        size_t base, rs;
+        tri_addr_out<MM>(md, plane, nx, nz, ny, base, rs);
+        size_t off = base + (size_t)s * rs;
         int nrow = ny - s;
         asm volatile("" : "+v"(off), "+v"(nrow));
-        T* q = S + off;
+        T* q = (MM ? So : S) + off;
 #pragma unroll
         for (int r = 0; r < MR; ++r) {
             if (r < nrow) {
                 if (mean) pbar[s + r] = (double)d[r] * inv_nxnz;
-                q[(size_t)r * plane] = mean ? T(0) : (T)d[r];
+                q[(size_t)r * rs] = mean ? T(0) : (T)d[r];
             }
         }
     }
@@ -736,7 +958,8 @@ __device__ __forceinline__ long tri_group_mode(long grp, long ngroups, int M, in
 // of type Ar of dynamic LDS.  With glist, workgroup b solves group
 // glist[b] (gridDim.x entries): the mixed form launches its FP32-arithmetic
 // groups and its FP64 ones from two such lists.
-template <int MR, int P, int M, typename T = double, int OCC = 1, bool TP = false, typename Ar = double>
+template <int MR, int P, int M, typename T = double, int OCC = 1, bool TP = false, typename Ar = double,
+          bool MM = false>
 __global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __restrict__ ax,
                                                           const double* __restrict__ az,
                                                           const double* __restrict__ bands, int nx, int ny,
@@ -744,7 +967,8 @@ __global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __
                                                           const double* __restrict__ msum = nullptr,
                                                           double* __restrict__ pbar = nullptr,
                                                           double inv_nxnz = 0.0,
-                                                          const int* __restrict__ glist = nullptr) {
+                                                          const int* __restrict__ glist = nullptr, int nz = 0,
+                                                          T* So = nullptr) {
     constexpr int NT = M * P;
     __shared__ TriRed<P, M, Ar> red;
     extern __shared__ double smb_raw[];  // the group's band set, 3 * MR * P values
@@ -758,9 +982,10 @@ __global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __
     const long mode = tri_group_mode(grp, (plane + M - 1) / M, M, mi);
     tri_stage_bands<MR, P, Ar>(smb, bands + (grp == 0 ? 3 * ny : 0), ny, NT);
     Ar d[MR];
-    tri_load_rows<MR, P, M, T, Ar>(d, S, mode, plane, ny, s, msum);
+    tri_load_rows<MR, P, M, T, Ar, MM>(d, S, mode, plane, ny, s, msum, nx, nz);
     __syncthreads();  // bands staged
-    tri_group<MR, P, M, T, TP, Ar>(d, S, ax, az, smb, nx, ny, plane, pcr_min, mode, msum, pbar, inv_nxnz, red);
+    tri_group<MR, P, M, T, TP, Ar, MM>(d, S, ax, az, smb, nx, ny, plane, pcr_min, mode, msum, pbar, inv_nxnz, red,
+                                       nz, So);
 }
 
 }  // namespace dnsp

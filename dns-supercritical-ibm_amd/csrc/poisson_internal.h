@@ -25,7 +25,15 @@ struct dnsp_poisson {
     int fft_strategy = DNSP_FFT_AUTO;
     int tri_strategy = DNSP_TRIDIAG_AUTO;
     bool ct_x = false, ct_z = false;  // compile-time FFT kernels in use
-    int xvar = 0, zvar = 0;           // their (tile, threads) variants
+    int xvar = 0, zvar = 0;           // their (tile, threads) variants, forward
+    int xvar_b = 0, zvar_b = 0;       // and inverse transforms
+    // blocked FP64 solve (x-blocked and mode-major layouts through d_W)
+    bool blk_ok = false;              // supported sizes (nx % 16 == 0, compile-time FFTs, partitioned)
+    bool blk_on = true;               // used by dnsp_poisson_run / solve when supported
+    double* d_W = nullptr;            // its workspace, nx*nz*ny doubles
+    int bxvar = 0, bxvar_b = 2;       // its x variants (forward, inverse)
+    int bzvar = 1, bzvar_b = 1;       // its z variants (0: 256, 1: 512 threads)
+    int btvar = 4;                    // its tridiagonal variant
     int tvar = 1;                     // partitioned tridiagonal variant
     int part_rows = 0;                // partitioned tridiagonal rows per chunk (0 = serial)
     double* d_host = nullptr;  // staging for dnsp_poisson_run_host

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <mutex>
 #include <vector>
 
@@ -232,6 +233,7 @@ size_t poisson_device_bytes(const dnsp_poisson* h) {
     if (h->d_twx) b += sizeof(double2) * nx;
     if (h->d_twz) b += sizeof(double2) * nz;
     if (h->d_cp) b += sizeof(double) * N;
+    if (h->d_W) b += sizeof(double) * N;
     if (h->d_host) b += sizeof(double) * N;
     if (h->d_small) b += sizeof(double) * small_len(h) + 2 * sizeof(double) * nz * ny;  // + line sums
     if (h->d_twx32) b += sizeof(float2) * nx;
@@ -408,6 +410,12 @@ int noalloc_scratch(int device, double** out) {
     return DNSP_OK;
 }
 
+// The mixed form's x kernels move (re, im) pairs of rhs and P as 16-byte
+// accesses.
+bool aligned16(const void* a, const void* b) {
+    return ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
+}
+
 int check_mixed(const dnsp_poisson* h) {
     DNSP_ARG_CHECK(h, "NULL handle");
     if (h->precision != DNSP_PRECISION_MIXED) {
@@ -505,6 +513,7 @@ int dnsp_poisson_solve_mixed(dnsp_poisson* h, const double* rhs, double* P, int 
     DNSP_ARG_CHECK(rhs && P, "NULL array");
     DNSP_ARG_CHECK(refinements >= 0 && refinements <= DNSP_MAX_REFINEMENTS, "refinements=%d not in [0, %d]",
                    refinements, DNSP_MAX_REFINEMENTS);
+    DNSP_ARG_CHECK(aligned16(rhs, P), "the mixed form needs 16-byte aligned rhs and P");
     hipStream_t st = as_stream(stream);
     const double* f = rhs;
     rc = rhs_for(h, rhs, P, refinements > 1, st, &f);
@@ -525,6 +534,7 @@ int dnsp_poisson_solve_refine(dnsp_poisson* h, const double* rhs, double* P, dou
     int rc = check_mixed(h);
     if (rc) return rc;
     DNSP_ARG_CHECK(rhs && P, "NULL array");
+    DNSP_ARG_CHECK(aligned16(rhs, P), "the mixed form needs 16-byte aligned rhs and P");
     DNSP_ARG_CHECK(tol >= 0 && std::isfinite(tol), "tol must be finite and >= 0");
     DNSP_ARG_CHECK(max_refinements >= 0 && max_refinements <= DNSP_MAX_REFINEMENTS,
                    "max_refinements=%d not in [0, %d]", max_refinements, DNSP_MAX_REFINEMENTS);

@@ -10,8 +10,8 @@ namespace dnsp {
 
 template <int N, int V>
 static void mx_x(int kind, const MxArgs& a, hipStream_t st) {
-    constexpr int G = ct_lines<N, ct_var_tile(V)>();
-    constexpr int NT = ct_var_threads(V);
+    constexpr int G = ct_lines<N, ct_xvar_tile(V)>();
+    constexpr int NT = ct_xvar_threads(V);
     const int nb = (int)((a.nlines + 2 * G - 1) / (2 * G));
     switch (kind) {
         case MX_FWD:

@@ -25,29 +25,29 @@ int part_group_modes(int variant, int ny, bool fp32) {
     }
 }
 
-template <int MR, int P, int M, typename T, int OCC, bool TP, typename Ar>
+template <int MR, int P, int M, typename T, int OCC, bool TP, typename Ar, bool MM>
 static void part_launch_mr(T* S, const double* ax, const double* az, const double* bands, int nx, int nz, int ny,
                            double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st,
-                           const int* glist, int nlist) {
+                           const int* glist, int nlist, T* So) {
     const long plane = (long)nx * nz;
     // every group, plus the group of modes 1..M-1; or the listed ones
     const int ngroups = glist ? nlist : (int)((plane + M - 1) / M) + 1;
     if (ngroups <= 0) return;
     const size_t lds = 3 * sizeof(Ar) * MR * P;
-    kt_tri_part<MR, P, M, T, OCC, TP, Ar><<<ngroups, M * P, lds, st>>>(S, ax, az, bands, nx, ny, plane, pcr_min, msum,
-                                                                    pbar, inv_nxnz, glist);
+    kt_tri_part<MR, P, M, T, OCC, TP, Ar, MM><<<ngroups, M * P, lds, st>>>(S, ax, az, bands, nx, ny, plane, pcr_min,
+                                                                        msum, pbar, inv_nxnz, glist, nz, So);
 }
 
 // P chunks per mode; rows per chunk from ny (at most 32 rows, ny <= 32 * P)
-template <int P, int M, typename T, int OCC, bool TP, typename Ar>
+template <int P, int M, typename T, int OCC, bool TP, typename Ar, bool MM>
 static void part_launch_pm(T* S, const double* ax, const double* az, const double* bands, int nx, int nz, int ny,
                            double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st,
-                           const int* glist, int nlist) {
+                           const int* glist, int nlist, T* So) {
     switch (rows_for(ny, P)) {
 #define DNSP_CASE(MR)                                                                                          \
     case MR:                                                                                                   \
-        part_launch_mr<MR, P, M, T, OCC, TP, Ar>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st, \
-                                                 glist, nlist);                                                \
+        part_launch_mr<MR, P, M, T, OCC, TP, Ar, MM>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st, \
+                                                 glist, nlist, So);                                            \
         break;
         DNSP_PART_ROWS(DNSP_CASE)
 #undef DNSP_CASE
@@ -55,13 +55,13 @@ static void part_launch_pm(T* S, const double* ax, const double* az, const doubl
     }
 }
 
-template <typename T, typename Ar>
+template <typename T, typename Ar, bool MM>
 void part_launch(int variant, T* S, const double* ax, const double* az, const double* bands, int nx, int nz,
                  int ny, double pcr_min, const double* msum, double* pbar, double inv_nxnz, hipStream_t st,
-                 const int* glist, int nlist) {
+                 const int* glist, int nlist, T* So) {
 #define DNSP_PV(P, M, OCC, TP)                                                                                    \
-    part_launch_pm<P, M, T, OCC, TP, Ar>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st, glist, \
-                                         nlist)
+    part_launch_pm<P, M, T, OCC, TP, Ar, MM>(S, ax, az, bands, nx, nz, ny, pcr_min, msum, pbar, inv_nxnz, st, glist, \
+                                             nlist, So)
     if constexpr (sizeof(T) == 4) {
         // FP32 spectra: (16, 32) in place of (64, 8), whose 8-mode FP32 rows
         // are 32-byte segments (32 modes make a 128-byte row).  16 chunks of
@@ -99,11 +99,13 @@ void part_launch(int variant, T* S, const double* ax, const double* az, const do
 #undef DNSP_PV
 }
 
-template void part_launch<double, double>(int, double*, const double*, const double*, const double*, int, int, int,
-                                          double, const double*, double*, double, hipStream_t, const int*, int);
-template void part_launch<float, double>(int, float*, const double*, const double*, const double*, int, int, int,
-                                         double, const double*, double*, double, hipStream_t, const int*, int);
-template void part_launch<float, float>(int, float*, const double*, const double*, const double*, int, int, int,
-                                        double, const double*, double*, double, hipStream_t, const int*, int);
+#define DNSP_PART_INST(T, AR, MM)                                                                          \
+    template void part_launch<T, AR, MM>(int, T*, const double*, const double*, const double*, int, int, int, \
+                                         double, const double*, double*, double, hipStream_t, const int*, int, T*);
+DNSP_PART_INST(double, double, false)
+DNSP_PART_INST(double, double, true)
+DNSP_PART_INST(float, double, false)
+DNSP_PART_INST(float, float, false)
+#undef DNSP_PART_INST
 
 }  // namespace dnsp

@@ -377,14 +377,16 @@ int dnsp_server_add_case(dnsp_server* sv, const dnsp_case_desc* d, int* case_id)
     c.seed = d->seed;
     c.form = sv->refinements == DNSP_SERVER_PICK_FORM ? -1 : sv->refinements;
     // Admission bound until the first build measures the case: 11 halo fields
-    // and P; the solver's arrays (a full field of Thomas workspace when the
-    // partitioned kernel does not apply, the mixed form's two FP32 fields,
+    // and P; the solver's arrays (a full field of workspace: serial Thomas
+    // where the partitioned kernel does not apply, the blocked solve where nx
+    // is a multiple of 16; the mixed form's two FP32 fields,
     // line sums, group lists, tables); IBM records (56 bytes each, at most 7
     // periodic images of 16 bytes per target); slabs, geometry.
     const size_t nx = (size_t)c.nx, nz = (size_t)c.nz, ny = (size_t)c.ny;
     const size_t hc = (nx + 2) * (nz + 2) * (ny + 2), pc = nx * nz * ny;
     size_t fp = sizeof(double) * (11 * hc + pc);
     if (ny < 32 || ny > 1024) fp += sizeof(double) * pc;  // serial Thomas workspace
+    else if (nx % 16 == 0) fp += sizeof(double) * pc;       // blocked-solve workspace
     fp += sizeof(double) * (2 * nz * ny + 16 * ny + 2048) + 64 * (nx + nz);
     if (sv->precision == DNSP_PRECISION_MIXED) fp += 2 * sizeof(float) * pc + sizeof(int) * (nx * nz / 4 + 8);
     for (int k = 0; k < 3; ++k) fp += (size_t)d->n_ibm[k] * (56 + 7 * 16);

@@ -79,6 +79,10 @@ _SIGNATURES = [
     ("dnsp_poisson_run_mixed", [_vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     ("dnsp_poisson_solve_refine", [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _ip, _dp, _vp], ctypes.c_int),
     ("dnsp_poisson_residual", [_vp, _vp, _vp, _dp, _vp], ctypes.c_int),
+    ("dnsp_poisson_solve_stage", [_vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+    ("dnsp_poisson_set_blocked", [_vp, ctypes.c_int], ctypes.c_int),
+    ("dnsp_poisson_get_blocked", [_vp, _ip, _ip], ctypes.c_int),
+    ("dnsp_poisson_get_solve_variants", [_vp, _ip], ctypes.c_int),
     ("dnsp_poisson_set_mixed_fp32", [_vp, ctypes.c_int], ctypes.c_int),
     ("dnsp_poisson_mixed_info", [_vp, _ip, _dp, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     ("dnsp_poisson_residual_noalloc", [_vp, _vp, _vp, _dp, _vp], ctypes.c_int),
@@ -230,6 +234,32 @@ class PoissonSolver:
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(lib().dnsp_poisson_get_mixed_variants(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return (a.value, b.value, c.value)
+
+    def solve_stage(self, rhs, P, stage, stream=None):
+        """Stage 0..4 of solve(rhs, P) (x fwd, z fwd, tridiagonal, z inv, x inv)."""
+        _check(lib().dnsp_poisson_solve_stage(self._h, _vp(_device_ptr(rhs, self.numel)),
+                                              _vp(_device_ptr(P, self.numel)), int(stage),
+                                              _vp(_stream_ptr(stream))))
+        return P
+
+    def set_blocked(self, on=True):
+        """The blocked solve (x-blocked / mode-major layouts through a workspace)
+        where supported (default on), or the in-place stages."""
+        _check(lib().dnsp_poisson_set_blocked(self._h, int(bool(on))))
+
+    @property
+    def blocked(self):
+        """(supported, in use)."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(lib().dnsp_poisson_get_blocked(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return bool(a.value), bool(b.value)
+
+    @property
+    def solve_variants(self):
+        """Variants of the five kernels solve() runs (stage order)."""
+        v = (ctypes.c_int * 5)()
+        _check(lib().dnsp_poisson_get_solve_variants(self._h, v))
+        return tuple(v)
 
     def set_mixed_fp32(self, on=True):
         """FP32 arithmetic in the mixed sweeps' well-conditioned tridiagonal

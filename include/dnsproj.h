@@ -72,7 +72,8 @@ int dnsp_poisson_init(int nx_glob, int nz_glob, int ny, double inv_dx, double in
  * single y-rank branch.  P is a device array of nx*nz*ny doubles holding the
  * right-hand side on entry and the solution on exit (same pinned (0,0) mode as
  * the reference; the mean is not removed).  Enqueued on `stream`; returns
- * without synchronising.
+ * without synchronising.  P needs the alignment of a double only (an array
+ * that is not 16-byte aligned runs runtime-sized x kernels).
  */
 int dnsp_poisson_run(dnsp_poisson* h, double* P, void* stream);
 
@@ -104,6 +105,35 @@ int dnsp_poisson_check(dnsp_poisson* h, const double* P, long long* n_nonfinite,
 int dnsp_poisson_forward(dnsp_poisson* h, double* P, void* stream);
 int dnsp_poisson_tridiag(dnsp_poisson* h, double* S, void* stream);
 int dnsp_poisson_backward(dnsp_poisson* h, double* S, void* stream);
+
+/* Stage `stage` (0..4: x forward, z forward, tridiagonal, z inverse, x
+ * inverse) of dnsp_poisson_solve(h, rhs, P): running the five in order on
+ * one stream equals the solve.  Where the handle runs the blocked solve
+ * (below) the intermediate layouts are internal and stages 0, 3 write the
+ * handle's workspace, stages 1, 2 P; elsewhere these are the in-place
+ * stages of dnsp_poisson_stage.  For timing the kernels of the solve one by
+ * one.  No counterpart in the reference. */
+int dnsp_poisson_solve_stage(dnsp_poisson* h, const double* rhs, double* P, int stage, void* stream);
+
+/* Blocked solve (no counterpart in the reference; same arithmetic, results
+ * bit-identical to the in-place stages with the same kernel variants): for
+ * nx a multiple of 16 with the compile-time FFT kernels and the partitioned
+ * tridiagonal, dnsp_poisson_run / dnsp_poisson_solve go through a workspace
+ * of one field held by the handle: x-transformed lines go to an x-blocked
+ * layout [k][a/16][j][a%16], the z transform turns that into mode-major
+ * blocks [a/16][b][k][a%16], the tridiagonal solve writes the x-blocked
+ * layout back, the inverse z transform halfcomplex lines, so that every z
+ * tile and tridiagonal mode group reads a contiguous block of memory.  On (1) by default where supported;
+ * dnsp_poisson_set_blocked(h, 0) returns to the in-place stages.  Arrays
+ * that are not 16-byte aligned take the in-place stages.  The workspace
+ * makes concurrent solves on one handle race (as the reference's shared work
+ * arrays do). */
+int dnsp_poisson_set_blocked(dnsp_poisson* h, int on);
+int dnsp_poisson_get_blocked(const dnsp_poisson* h, int* supported, int* on);
+/* The variants of the five kernels dnsp_poisson_solve runs now (stage order:
+ * x forward, z forward, tridiagonal, z inverse, x inverse), in the blocked
+ * solve (z: 0 = 256, 1 = 512 threads) or the in-place stages. */
+int dnsp_poisson_get_solve_variants(const dnsp_poisson* h, int* v);
 
 /* One kernel of the solve, for timing and tuning:
  *   DNSP_STAGE_X_FWD (:360-375), DNSP_STAGE_Z_FWD (:377-393), DNSP_STAGE_TRIDIAG (:395-422),
@@ -140,8 +170,11 @@ int dnsp_poisson_set_strategy(dnsp_poisson* h, int fft, int tridiag);
 int dnsp_poisson_get_strategy(const dnsp_poisson* h, int* fft_ct_x, int* fft_ct_z, int* part_rows);
 
 /* Kernel variants.  xvar, zvar (0..3): the compile-time-sized FFT kernels'
- * (complex elements per workgroup tile, threads) = (3072,256), (3072,512),
- * (1536,256), (6144,512); ignored where a length has no such kernels.
+ * (complex elements per workgroup tile, threads): x (768,256), (3072,512),
+ * (1536,256), (6144,512); z (3072,256), (3072,512), (1536,256), (6144,512);
+ * ignored where a length has no such kernels.  set_variants sets both the
+ * forward and the inverse transforms (the tuner picks them separately; get
+ * returns the forward ones) and the blocked solve's knobs alike.
  * tvar (0..7): partitioned tridiagonal workgroups of (chunks per mode, modes)
  * = (32, 16) with registers capped for 4 resident waves per SIMD (0),
  * (32, 8) (1), (48, 16) (2), (64, 8) (3); rows per chunk follow from ny.
@@ -181,7 +214,8 @@ int dnsp_poisson_autotune(dnsp_poisson* h, double* work, int reps, void* stream,
  * 1e-11 on channel grids; the handle must be DNSP_PRECISION_MIXED.
  *   dnsp_poisson_solve_mixed: `refinements` sweeps (0..DNSP_MAX_REFINEMENTS)
  *     after sweep 0; rhs may equal P (a copy of rhs is then kept when
- *     refinements > 1).  Enqueued on `stream`, no synchronisation.
+ *     refinements > 1).  rhs and P 16-byte aligned (DNSP_ERR_ARG
+ *     otherwise).  Enqueued on `stream`, no synchronisation.
  *   dnsp_poisson_run_mixed: in place, P = rhs on entry.
  *   dnsp_poisson_solve_refine: refines until the relative residual is <= tol
  *     or max_refinements sweeps are done; returns the sweeps done and the
@@ -220,7 +254,8 @@ int dnsp_poisson_mixed_info(const dnsp_poisson* h, int* fp32_on, double* f32_min
  *     precision as for dnsp_poisson_init (MIXED adds two FP32 fields per
  *     instance).
  *   dnsp_batch_bind: the device arrays of every instance (host arrays of n
- *     device pointers; rhs[i] != P[i]); synchronous, done once.
+ *     device pointers; rhs[i] != P[i], both 16-byte aligned); synchronous,
+ *     done once.
  *   dnsp_batch_run: FP64 solves P[i] = L_i^{-1} rhs[i] of all instances.
  *   dnsp_batch_run_mixed: mixed-precision solves, `refinements` 0 or 1.
  *   Both enqueue on `stream` only (no allocation, no synchronisation), so a

@@ -248,6 +248,30 @@ def test_repeated_runs_are_deterministic(cuda):
     s.close()
 
 
+def test_eight_byte_aligned_arrays(cuda):
+    """Arrays offset by one double from a 16-byte boundary (a tensor view, a
+    Fortran section): the FP64 solve takes the runtime-sized x kernels and
+    still matches the oracle; the mixed form rejects them (DNSP_ERR_ARG)."""
+    import torch
+    say("8-byte aligned arrays")
+    c = case_grid(768, 32, 12, 4)
+    s, bands = make_solver(c)
+    n = c["rhs"].size
+    buf = torch.zeros(n + 1, dtype=torch.float64, device=cuda)
+    P = buf[1:].view(c["rhs"].shape)
+    assert P.data_ptr() % 16 == 8
+    P.copy_(to_dev(c["rhs"]))
+    s.run(P)
+    ref = O.solve(c["rhs"], c["inv_dx"], c["inv_dz"], bands)
+    assert rel_mean_removed(to_host(P), ref) <= 1e-12
+    s.close()
+    m = D.PoissonSolver(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], bands, precision=D.PRECISION_MIXED)
+    with pytest.raises(D.DnspError) as e:
+        m.solve_mixed(P, P, 1)
+    assert e.value.code == D.DNSP_ERR_ARG
+    m.close()
+
+
 @pytest.mark.slow
 def test_config2_full_size(cuda):
     """768 x 384 x 384: residual, round trip and oracle agreement."""
