@@ -45,6 +45,7 @@ struct dnsp_poisson {
     int form = -1;
     // mixed-precision form (precision == DNSP_PRECISION_MIXED)
     int mxvar = 0, mzvar = 0, mtvar = 0;  // its FFT and tridiagonal variants
+    int mxv[6] = {0, 0, 0, 0, 0, 0};      // x variant per kernel kind (MxKind), picked by the tuner
     int mrows = 0;                         // its partitioned rows per chunk
     float2* d_twx32 = nullptr;
     float2* d_twz32 = nullptr;
@@ -73,6 +74,6 @@ constexpr int kResBlocks = 1024;
 int mixed_setup(dnsp_poisson* h);
 void mixed_release(dnsp_poisson* h);
 // Times the mixed-precision kernel variants and keeps the fastest.
-int mixed_autotune(dnsp_poisson* h, const double* f, int reps, hipStream_t st, double* times_ms);
+int mixed_autotune(dnsp_poisson* h, double* f, int reps, hipStream_t st, double* times_ms);
 
 }  // namespace dnsp

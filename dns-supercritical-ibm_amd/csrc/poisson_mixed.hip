@@ -309,7 +309,7 @@ void sweep0(dnsp_poisson* h, const double* f, double* out, hipStream_t st) {
     a.src = f;
     a.dst = h->d_A;
     a.lsum = h->d_lsA;
-    mx_x_launch(MX_FWD, h->nx, h->mxvar, a, st);
+    mx_x_launch(MX_FWD, h->nx, h->mxv[MX_FWD], a, st);
     ct_z_launch(h->nz, h->mzvar, false, h->d_A, h->nx, h->ny, h->d_twz32, h->d_lsA, sm.msum, st);
     mixed_tri(h, h->mtvar, h->d_A, sm.msum, sm.pbar0, st);
     ct_z_launch(h->nz, h->mzvar, true, h->d_A, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
@@ -319,11 +319,11 @@ void sweep0(dnsp_poisson* h, const double* f, double* out, hipStream_t st) {
         b.base = nullptr;
         b.pbar0 = sm.pbar0;
         b.dst = out;
-        mx_x_launch(MX_ACC_F32, h->nx, h->mxvar, b, st);
+        mx_x_launch(MX_ACC_F32, h->nx, h->mxv[MX_ACC_F32], b, st);
     } else {
         b.dst = h->d_A;
         b.lsum = h->d_lsA;
-        mx_x_launch(MX_BWD, h->nx, h->mxvar, b, st);
+        mx_x_launch(MX_BWD, h->nx, h->mxv[MX_BWD], b, st);
     }
 }
 
@@ -341,7 +341,7 @@ void refine(dnsp_poisson* h, const double* f, double* P, bool first, bool sums, 
     a.bands = h->d_bands;
     a.dst = h->d_B;
     a.lsum = h->d_lsB;
-    mx_x_launch(first ? MX_RES_F32 : MX_RES_F64, h->nx, h->mxvar, a, st);
+    mx_x_launch(first ? MX_RES_F32 : MX_RES_F64, h->nx, h->mxv[first ? MX_RES_F32 : MX_RES_F64], a, st);
     ct_z_launch(h->nz, h->mzvar, false, h->d_B, h->nx, h->ny, h->d_twz32, h->d_lsB, sm.msum, st);
     mixed_tri(h, h->mtvar, h->d_B, sm.msum, sm.pbar1, st);
     ct_z_launch(h->nz, h->mzvar, true, h->d_B, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
@@ -352,7 +352,7 @@ void refine(dnsp_poisson* h, const double* f, double* P, bool first, bool sums, 
     b.pbar1 = sm.pbar1;
     b.dst = P;
     b.lsum = sums ? h->d_lsA : nullptr;
-    mx_x_launch(first ? MX_ACC_F32 : MX_ACC_F64, h->nx, h->mxvar, b, st);
+    mx_x_launch(first ? MX_ACC_F32 : MX_ACC_F64, h->nx, h->mxv[first ? MX_ACC_F32 : MX_ACC_F64], b, st);
 }
 
 // rel = ||f - L P|| / ||f|| (||f - L P|| when f = 0), with P's line sums in
@@ -440,50 +440,95 @@ int rhs_for(dnsp_poisson* h, const double* rhs, const double* P, bool reread, hi
 
 }  // namespace
 
-int mixed_autotune(dnsp_poisson* h, const double* f, int reps, hipStream_t st, double* times_ms) {
+int mixed_autotune(dnsp_poisson* h, double* f, int reps, hipStream_t st, double* times_ms) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
         if (e0) (void)hipEventDestroy(e0);
         set_error("autotune: event creation failed");
         return DNSP_ERR_HIP;
     }
+    const Small sm = small_of(h);
+    // one x kernel of the sweeps, on the handle's arrays (f: the work field)
+    auto x_kind = [&](int kind, int v) {
+        MxArgs a = base_args(h);
+        if (kind == MX_FWD) {
+            a.src = f;
+            a.dst = h->d_A;
+            a.lsum = h->d_lsA;
+        } else if (kind == MX_BWD) {
+            a.src = h->d_A;
+            a.dst = h->d_A;
+            a.lsum = h->d_lsA;
+        } else if (kind == MX_RES_F32) {
+            a.src = h->d_A;
+            a.f = f;
+            a.q = sm.q;
+            a.bands = h->d_bands;
+            a.dst = h->d_B;
+            a.lsum = h->d_lsB;
+        } else {
+            a.src = h->d_B;
+            a.base = h->d_A;
+            a.pbar0 = sm.pbar0;
+            a.pbar1 = sm.pbar1;
+            a.dst = f;
+        }
+        mx_x_launch(kind, h->nx, v, a, st);
+    };
+    auto time_of = [&](auto&& fn, double* per) -> int {
+        fn();  // warm-up
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps; ++r) fn();
+        (void)hipEventRecord(e1, st);
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            set_error("mixed autotune: launch or synchronize failed");
+            return DNSP_ERR_HIP;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *per = (double)ms / reps;
+        return DNSP_OK;
+    };
     int rc = DNSP_OK;
+    // x: each kernel kind of the first two sweeps on its own
+    const int kinds[4] = {MX_FWD, MX_BWD, MX_RES_F32, MX_ACC_F32};
+    int pickx[4];
+    double bestx[4];
+    for (int i = 0; i < 4; ++i) {
+        pickx[i] = h->mxv[kinds[i]];
+        bestx[i] = 1e30;
+    }
+    for (int v = 0; v < kCtVariants && rc == DNSP_OK; ++v) {
+        double sum = 0.0;
+        for (int i = 0; i < 4 && rc == DNSP_OK; ++i) {
+            double t = 0.0;
+            rc = time_of([&]() { x_kind(kinds[i], v); }, &t);
+            sum += t;
+            if (!rc && t < bestx[i]) {
+                bestx[i] = t;
+                pickx[i] = v;
+            }
+        }
+        if (times_ms && !rc) times_ms[kTuneSlot[0] + v] = sum;
+    }
+    // z (forward + inverse pairs) and the tridiagonal (pairs of solves)
     int pick[3] = {h->mxvar, h->mzvar, h->mtvar};
     double best[3] = {1e30, 1e30, 1e30};
-    for (int dir = 0; dir < 3 && rc == DNSP_OK; ++dir) {
+    for (int dir = 1; dir < 3 && rc == DNSP_OK; ++dir) {
         const int nvar = dir == 2 ? kPartVariants : kCtVariants;
         for (int v = 0; v < nvar && rc == DNSP_OK; ++v) {
-            auto pair = [&]() {
-                if (dir == 0) {
-                    MxArgs a = base_args(h);
-                    a.src = f;
-                    a.dst = h->d_A;
-                    a.lsum = h->d_lsA;
-                    mx_x_launch(MX_FWD, h->nx, v, a, st);
-                    MxArgs b = base_args(h);
-                    b.src = h->d_A;
-                    b.dst = h->d_A;
-                    b.lsum = h->d_lsA;
-                    mx_x_launch(MX_BWD, h->nx, v, b, st);
-                } else if (dir == 1) {
-                    ct_z_launch(h->nz, v, false, h->d_A, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
-                    ct_z_launch(h->nz, v, true, h->d_A, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
-                } else {
-                    for (int r = 0; r < 2; ++r) mixed_tri(h, v, h->d_A, nullptr, nullptr, st);
-                }
-            };
-            pair();  // warm-up
-            (void)hipEventRecord(e0, st);
-            for (int r = 0; r < reps; ++r) pair();
-            (void)hipEventRecord(e1, st);
-            if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-                set_error("mixed autotune: launch or synchronize failed");
-                rc = DNSP_ERR_HIP;
-                break;
-            }
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            const double per = (double)ms / reps;
+            double per = 0.0;
+            rc = time_of(
+                [&]() {
+                    if (dir == 1) {
+                        ct_z_launch(h->nz, v, false, h->d_A, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
+                        ct_z_launch(h->nz, v, true, h->d_A, h->nx, h->ny, h->d_twz32, nullptr, nullptr, st);
+                    } else {
+                        for (int r = 0; r < 2; ++r) mixed_tri(h, v, h->d_A, nullptr, nullptr, st);
+                    }
+                },
+                &per);
+            if (rc) break;
             if (times_ms) times_ms[kTuneSlot[dir] + v] = per;
             if (per < best[dir]) {
                 best[dir] = per;
@@ -494,7 +539,8 @@ int mixed_autotune(dnsp_poisson* h, const double* f, int reps, hipStream_t st, d
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    h->mxvar = pick[0];
+    for (int i = 0; i < 4; ++i) h->mxv[kinds[i]] = pickx[i];
+    h->mxvar = pickx[0];
     h->mzvar = pick[1];
     h->mtvar = pick[2];
     return DNSP_OK;
@@ -678,6 +724,7 @@ int dnsp_poisson_set_mixed_variants(dnsp_poisson* h, int xvar, int zvar, int tva
                    "FFT kernel variants must be in [0, %d)", kCtVariants);
     DNSP_ARG_CHECK(tvar >= 0 && tvar < kPartVariants, "tridiagonal variant must be in [0, %d)", kPartVariants);
     h->mxvar = xvar;
+    for (int k = 0; k < 6; ++k) h->mxv[k] = (k == MX_RES_F64 || k == MX_ACC_F64) ? 0 : xvar;
     h->mzvar = zvar;
     h->mtvar = tvar;
     return DNSP_OK;

@@ -450,13 +450,48 @@ __device__ __forceinline__ double law_eval(const Law& L, const double* seg, cons
     return cubic_eval(L, seg, tab, T);
 }
 
+// All six laws piecewise cubic: the union of their segment bounds splits T
+// into merged intervals on each of which every law keeps one segment, so one
+// interval search (instead of one per law) selects all six segments.
+// knots[0..m) increase; interval i holds knots[i-1] < T <= knots[i] (0: T <=
+// knots[0]; m: T above every knot); mseg[6 i + q] is law q's segment there (an
+// index into the segment table), mtab[kMergedBuckets] the first interval of
+// each of the equal buckets over [blo, blo + kMergedBuckets / binv).  The
+// segment is the per-law search's (the first with T <= its bound, the last
+// above every bound): the law's bounds are among the knots, so T <= hi exactly
+// when the interval's upper knot is <= hi.
+constexpr int kMergedBuckets = 1024;
+struct MergedLaws {
+    const double* knots;
+    const int* mseg;
+    const int* mtab;
+    int m;
+    double blo, binv;
+};
+
+__device__ __forceinline__ int merged_interval(const double* knots, const int* mtab, int m, double blo,
+                                               double binv, double T) {
+    const double x = (T - blo) * binv;
+    const int b = !(x >= 0.0) ? 0 : (x >= (double)kMergedBuckets ? kMergedBuckets - 1 : (int)x);
+    int i = mtab[b];
+    while (i > 0 && T <= knots[i - 1]) --i;
+    while (i < m && T > knots[i]) ++i;
+    return i;
+}
+
+__device__ __forceinline__ double seg_eval(const double* seg, int sgi, double T) {
+    const double* s = seg + 6 * sgi;
+    const double d = T - s[1];
+    return ((s[2] + s[3] * d) + s[4] * (d * d)) + s[5] * ((d * d) * d);
+}
+
 // T_ext and the seven properties at every padded point (update_props): in
 // rough-wall columns (bot/top given) T_ext takes the wall values for
 // k <= bot(i, j) and k >= top(i, j), halo points those of their periodic
 // image.  Each workgroup stages the cubic laws' segments (nseg of them) and
 // bucket tables in LDS once and strides over the points, so the interval
 // searches read LDS.
-constexpr int kMaxSegments = 1024;  // 48 KB of LDS, plus 6 KB of bucket tables
+constexpr int kMaxSegments = 1024;  // 48 KB of LDS, plus at most 37 KB of merged tables (6 KB of bucket tables)
 
 // IDX32: the padded field has fewer than 2^31 points (32-bit index split).
 template <bool ALL_CUBIC, bool IDX32>
@@ -467,11 +502,23 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
                                                     double* __restrict__ buoy, const int* __restrict__ bot,
                                                     const int* __restrict__ top, double t_bot, double t_top,
                                                     const double* __restrict__ seg_g, int nseg,
-                                                    const int* __restrict__ tab_g, PropSet ps, Grid g) {
+                                                    const int* __restrict__ tab_g, PropSet ps, Grid g,
+                                                    MergedLaws ml) {
+    // LDS: the segments; then per-law bucket tables (general laws) or the
+    // merged knots, segment map and buckets (all laws cubic)
     extern __shared__ double seg[];
-    int* tab = reinterpret_cast<int*>(seg + 6 * nseg);
+    double* knots = seg + 6 * nseg;
+    int* tab = reinterpret_cast<int*>(ALL_CUBIC ? knots + ml.m : knots);
+    int* mseg = tab;
+    int* mtab = mseg + 6 * (ml.m + 1);
     for (int e = (int)threadIdx.x; e < 6 * nseg; e += kThreads) seg[e] = seg_g[e];
-    for (int e = (int)threadIdx.x; e < Q_N * kBuckets; e += kThreads) tab[e] = tab_g[e];
+    if constexpr (ALL_CUBIC) {
+        for (int e = (int)threadIdx.x; e < ml.m; e += kThreads) knots[e] = ml.knots[e];
+        for (int e = (int)threadIdx.x; e < 6 * (ml.m + 1); e += kThreads) mseg[e] = ml.mseg[e];
+        for (int e = (int)threadIdx.x; e < kMergedBuckets; e += kThreads) mtab[e] = ml.mtab[e];
+    } else {
+        for (int e = (int)threadIdx.x; e < Q_N * kBuckets; e += kThreads) tab[e] = tab_g[e];
+    }
     __syncthreads();
     const long n = g.total();
     for (long idx = (long)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (long)gridDim.x * kThreads) {
@@ -499,11 +546,22 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
                 if (k >= top[col]) te = t_top;
             }
         }
-        const double h = law_eval<ALL_CUBIC>(ps.law[Q_ENTH], seg, tab + Q_ENTH * kBuckets, te);
-        const double r = law_eval<ALL_CUBIC>(ps.law[Q_RHO], seg, tab + Q_RHO * kBuckets, te);
-        const double m = law_eval<ALL_CUBIC>(ps.law[Q_MU], seg, tab + Q_MU * kBuckets, te);
-        const double k = law_eval<ALL_CUBIC>(ps.law[Q_COND], seg, tab + Q_COND * kBuckets, te);
-        const double c = law_eval<ALL_CUBIC>(ps.law[Q_CP], seg, tab + Q_CP * kBuckets, te);
+        double h, r, m, k, c;
+        const int* ms = nullptr;
+        if constexpr (ALL_CUBIC) {
+            ms = mseg + 6 * merged_interval(knots, mtab, ml.m, ml.blo, ml.binv, te);
+            h = seg_eval(seg, ms[Q_ENTH], te);
+            r = seg_eval(seg, ms[Q_RHO], te);
+            m = seg_eval(seg, ms[Q_MU], te);
+            k = seg_eval(seg, ms[Q_COND], te);
+            c = seg_eval(seg, ms[Q_CP], te);
+        } else {
+            h = law_eval<false>(ps.law[Q_ENTH], seg, tab + Q_ENTH * kBuckets, te);
+            r = law_eval<false>(ps.law[Q_RHO], seg, tab + Q_RHO * kBuckets, te);
+            m = law_eval<false>(ps.law[Q_MU], seg, tab + Q_MU * kBuckets, te);
+            k = law_eval<false>(ps.law[Q_COND], seg, tab + Q_COND * kBuckets, te);
+            c = law_eval<false>(ps.law[Q_CP], seg, tab + Q_CP * kBuckets, te);
+        }
         T_ext[idx] = te;
         enth[idx] = h;
         rho[idx] = r;
@@ -511,7 +569,9 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
         cond[idx] = k;
         cp[idx] = c;
         if (ps.divjac_formula) {
-            const double dr = law_eval<ALL_CUBIC>(ps.law[Q_DRHO], seg, tab + Q_DRHO * kBuckets, te);
+            double dr;
+            if constexpr (ALL_CUBIC) dr = seg_eval(seg, ms[Q_DRHO], te);
+            else dr = law_eval<false>(ps.law[Q_DRHO], seg, tab + Q_DRHO * kBuckets, te);
             divjac[idx] = (-1.0 / ((r * r) * c)) * dr;
         } else {
             divjac[idx] = ps.divjac_c;
@@ -577,6 +637,12 @@ struct dnsp_predictor {
     size_t d_seg_n = 0;
     std::vector<int> h_tab = std::vector<int>(Q_N * kBuckets, 0);  // bucket tables, kBuckets per quantity
     int* d_tab = nullptr;
+    // merged tables when every law is cubic (ml.m >= 0): one device array of
+    // the m knots (doubles), the 6 (m + 1) segment map and the kMergedBuckets
+    // buckets (ints)
+    MergedLaws ml{nullptr, nullptr, nullptr, -1, 0.0, 0.0};
+    void* d_merged = nullptr;
+    size_t d_merged_bytes = 0;
     int* d_bot[4] = {};  // rough-wall column slabs of U, V, W, T (nx*nz each), or NULL
     int* d_top[4] = {};
     dnsp_ibm* ibm[5] = {};  // U, V, W, T, div_now forcing (borrowed)
@@ -600,6 +666,7 @@ void release(dnsp_predictor* pr) {
     (void)hipFree(pr->d_geom);
     (void)hipFree(pr->d_seg);
     (void)hipFree(pr->d_tab);
+    (void)hipFree(pr->d_merged);
     for (int c = 0; c < 4; ++c) {
         (void)hipFree(pr->d_bot[c]);
         (void)hipFree(pr->d_top[c]);
@@ -783,6 +850,63 @@ int dnsp_predictor_read(dnsp_predictor* pr, int which, double* dst, void* stream
     return DNSP_OK;
 }
 
+namespace {
+
+// The merged-interval tables of the property kernel (MergedLaws) from the
+// current laws; ml.m = -1 unless every law is cubic.
+int build_merged(dnsp_predictor* pr) {
+    pr->ml.m = -1;
+    for (int q = 0; q < Q_N; ++q)
+        if (pr->ps.law[q].kind != DNSP_LAW_CUBIC) return DNSP_OK;
+    std::vector<double> K;
+    for (int q = 0; q < Q_N; ++q) {
+        const Law& L = pr->ps.law[q];
+        for (int t = 0; t + 1 < L.nseg; ++t) K.push_back(pr->h_seg[6 * (size_t)(L.off + t)]);
+    }
+    std::sort(K.begin(), K.end());
+    K.erase(std::unique(K.begin(), K.end()), K.end());
+    const int m = (int)K.size();
+    std::vector<int> ms((size_t)6 * (m + 1)), mt(kMergedBuckets);
+    for (int q = 0; q < Q_N; ++q) {
+        const Law& L = pr->ps.law[q];
+        int t = 0;  // first segment whose bound is >= the interval's upper knot
+        for (int i = 0; i <= m; ++i) {
+            if (i == m) t = L.nseg - 1;
+            else
+                while (t < L.nseg - 1 && pr->h_seg[6 * (size_t)(L.off + t)] < K[(size_t)i]) ++t;
+            ms[(size_t)6 * i + q] = L.off + t;
+        }
+    }
+    double blo = 0.0, binv = 0.0;
+    if (m >= 2) {
+        blo = K[0];
+        binv = kMergedBuckets / (K[(size_t)m - 1] - K[0]);
+    }
+    for (int b = 0, i = 0; b < kMergedBuckets; ++b) {
+        const double start = binv > 0.0 ? blo + b / binv : -1e300;
+        while (i < m && K[(size_t)i] < start) ++i;
+        mt[(size_t)b] = i;
+    }
+    const size_t bytes = sizeof(double) * m + sizeof(int) * (ms.size() + mt.size());
+    if (bytes > pr->d_merged_bytes) {
+        (void)hipFree(pr->d_merged);
+        pr->d_merged = nullptr;
+        pr->d_merged_bytes = 0;
+        DNSP_HIP_TRY(hipMalloc(&pr->d_merged, bytes));
+        pr->d_merged_bytes = bytes;
+    }
+    double* dk = static_cast<double*>(pr->d_merged);
+    int* dms = reinterpret_cast<int*>(dk + m);
+    int* dmt = dms + ms.size();
+    if (m > 0) DNSP_HIP_TRY(hipMemcpy(dk, K.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+    DNSP_HIP_TRY(hipMemcpy(dms, ms.data(), sizeof(int) * ms.size(), hipMemcpyHostToDevice));
+    DNSP_HIP_TRY(hipMemcpy(dmt, mt.data(), sizeof(int) * mt.size(), hipMemcpyHostToDevice));
+    pr->ml = MergedLaws{dk, dms, dmt, m, blo, binv};
+    return DNSP_OK;
+}
+
+}  // namespace
+
 int dnsp_predictor_set_law(dnsp_predictor* pr, int quantity, int kind, const double* data, int n) {
     clear_error();
     DNSP_ARG_CHECK(pr && data, "NULL argument");
@@ -845,7 +969,7 @@ int dnsp_predictor_set_law(dnsp_predictor* pr, int quantity, int kind, const dou
         DNSP_ARG_CHECK(false, "unknown law kind %d", kind);
     }
     pr->ps.law[quantity] = L;
-    return DNSP_OK;
+    return build_merged(pr);
 }
 
 int dnsp_predictor_set_derived(dnsp_predictor* pr, int divjac_formula, double divjac_const, int buoy_linear,
@@ -1006,15 +1130,17 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
     if (vp) {
         const int nseg = (int)(pr->h_seg.size() / 6);
         const int nbp = std::min(blocks_for(g.total()), 2048);
-        bool all_cubic = true;
-        for (int q = 0; q < Q_N; ++q) all_cubic = all_cubic && pr->ps.law[q].kind == DNSP_LAW_CUBIC;
+        const bool all_cubic = pr->ml.m >= 0;
         const bool idx32 = g.total() < (1L << 31);
         auto kp = all_cubic ? (idx32 ? k_props<true, true> : k_props<true, false>)
                             : (idx32 ? k_props<false, true> : k_props<false, false>);
-        kp<<<nbp, kThreads, sizeof(double) * 6 * nseg + sizeof(int) * Q_N * kBuckets, st>>>(
-            T, pr->f[DNSP_PRED_T_EXT], pr->f[DNSP_PRED_ENTH], rho, mu, pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_CP],
-            pr->f[DNSP_PRED_DIVJAC], buoy, pr->d_bot[3], pr->d_top[3], wl[3], wl[7], pr->d_seg, nseg, pr->d_tab, pr->ps,
-            g);
+        const size_t lds = sizeof(double) * 6 * nseg +
+                           (all_cubic ? sizeof(double) * pr->ml.m + sizeof(int) * (6 * (pr->ml.m + 1) + kMergedBuckets)
+                                      : sizeof(int) * Q_N * kBuckets);
+        kp<<<nbp, kThreads, lds, st>>>(T, pr->f[DNSP_PRED_T_EXT], pr->f[DNSP_PRED_ENTH], rho, mu,
+                                       pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_CP], pr->f[DNSP_PRED_DIVJAC], buoy,
+                                       pr->d_bot[3], pr->d_top[3], wl[3], wl[7], pr->d_seg, nseg, pr->d_tab, pr->ps,
+                                       g, pr->ml);
         if (pr->ibm[3]) {
             // build_rhs_energy (:640-645): T forcing, then T's wall planes
             rc = dnsp_ibm_exec(pr->ibm[3], T, vs);
