@@ -154,7 +154,6 @@ __global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri(const BInst
                                                                       const long* __restrict__ start, int cnt) {
     constexpr int M = kBatchM;
     constexpr int P = kPartChunks;
-    constexpr int NT = M * P;
     using T = typename std::conditional<KIND == 0, double, float>::type;
     __shared__ TriRed<P, M> red;
     extern __shared__ double smb[];
@@ -163,17 +162,15 @@ __global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri(const BInst
     const BInst& b = inst[ids[u]];
     const long grp = task - start[u];
     const int mi = (int)threadIdx.x % M;
-    const int s = ((int)threadIdx.x / M) * MR;
     const long mode = tri_group_mode(grp, (b.plane + M - 1) / M, M, mi);
-    tri_stage_bands<MR, P>(smb, b.bands + (grp == 0 ? 3 * b.ny : 0), b.ny, NT);
     T* S = KIND == 0 ? reinterpret_cast<T*>(b.P) : reinterpret_cast<T*>(KIND == 1 ? b.A : b.B);
     const double* msum = KIND == 0 ? nullptr : b.small;
     double* pbar = KIND == 0 ? nullptr : b.small + (KIND == 1 ? 1 : 2) * b.ny;
     double d[MR];
-    tri_load_rows<MR, P, M, T>(d, S, mode, b.plane, b.ny, s, msum);
+    const double lam64 = tri_prologue<MR, P, M, T>(d, smb, b.bands + (grp == 0 ? 3 * b.ny : 0), S, b.ax, b.az, mode,
+                                                   b.plane, b.nx, b.ny, msum);
     __syncthreads();
-    tri_group<MR, P, M, T, TP>(d, S, b.ax, b.az, smb, b.nx, b.ny, b.plane, b.pcr_min, mode, msum, pbar, b.inv_nxnz,
-                               red);
+    tri_group<MR, P, M, T, TP>(d, S, lam64, smb, b.nx, b.ny, b.plane, b.pcr_min, mode, msum, pbar, b.inv_nxnz, red);
 }
 
 // The mixed form's tridiagonal from a task list: workgroup b solves mode
@@ -184,7 +181,6 @@ __global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri_list(const 
                                                                            const int2* __restrict__ tl) {
     constexpr int M = kBatchM;
     constexpr int P = kPartChunks;
-    constexpr int NT = M * P;
     __shared__ TriRed<P, M, Ar> red;
     extern __shared__ double smb_raw[];
     Ar* smb = reinterpret_cast<Ar*>(smb_raw);
@@ -192,16 +188,15 @@ __global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri_list(const 
     const BInst& b = inst[ids[t.x]];
     const long grp = t.y;
     const int mi = (int)threadIdx.x % M;
-    const int s = ((int)threadIdx.x / M) * MR;
     const long mode = tri_group_mode(grp, (b.plane + M - 1) / M, M, mi);
-    tri_stage_bands<MR, P, Ar>(smb, b.bands + (grp == 0 ? 3 * b.ny : 0), b.ny, NT);
     float* S = KIND == 1 ? b.A : b.B;
     const double* msum = b.small;
     double* pbar = b.small + (KIND == 1 ? 1 : 2) * b.ny;
     Ar d[MR];
-    tri_load_rows<MR, P, M, float, Ar>(d, S, mode, b.plane, b.ny, s, msum);
+    const double lam64 = tri_prologue<MR, P, M, float, Ar>(d, smb, b.bands + (grp == 0 ? 3 * b.ny : 0), S, b.ax, b.az,
+                                                           mode, b.plane, b.nx, b.ny, msum);
     __syncthreads();
-    tri_group<MR, P, M, float, TP, Ar>(d, S, b.ax, b.az, smb, b.nx, b.ny, b.plane, b.pcr_min, mode, msum, pbar,
+    tri_group<MR, P, M, float, TP, Ar>(d, S, lam64, smb, b.nx, b.ny, b.plane, b.pcr_min, mode, msum, pbar,
                                        b.inv_nxnz, red);
 }
 

@@ -695,23 +695,49 @@ struct TriRed {
     R a[2][P][M], b[2][P][M], c[2][P][M], d[2][P][M];
 };
 
-// Stages one band set (a, b, c of ny rows each) into LDS as three arrays of
-// MR * P rows.  Rows past ny get a = c = 0, b = -1: decoupled from the rows
-// above and below, with the diagonal -1 + lambda nonzero for every mode
-// (lambda <= 0), so that they solve to x = 0 from d = 0 and the band reads
-// need no bounds test.
-template <int MR, int P, typename Ar = double>
-__device__ __forceinline__ void tri_stage_bands(Ar* smb, const double* __restrict__ set, int ny, int nt) {
-    constexpr int NP = MR * P;
-    for (int e = (int)threadIdx.x; e < 3 * NP; e += nt) {
+// The loads a workgroup needs before its solve, issued together so that
+// their round trips overlap (one after the other they cost three): the
+// mode's eigenvalue lambda = ax[i] + az[j] (returned; -1 for idle lanes,
+// which then solve a regular system), the group's band set `set` and the
+// lane's MR rows (into d[], masked as tri_load_rows does).  The band set
+// (a, b, c of ny rows each) goes to smb as three arrays of MR * P rows;
+// rows past ny get a = c = 0, b = -1: decoupled from the rows above and
+// below, with the diagonal -1 + lambda nonzero for every mode (lambda <= 0),
+// so that they solve to x = 0 from d = 0 and the band reads need no bounds
+// test.  Workgroups of M * P threads; the caller places a workgroup barrier
+// before smb is read.
+template <int MR, int P, int M, typename T, typename Ar = double, bool MM = false>
+__device__ __forceinline__ double tri_prologue(Ar (&d)[MR], Ar* smb, const double* __restrict__ set, const T* S,
+                                               const double* __restrict__ ax, const double* __restrict__ az,
+                                               long mode, long plane, int nx, int ny,
+                                               const double* __restrict__ msum, int nz = 0) {
+    constexpr int NP = MR * P, NT = M * P, KB = (3 * NP + NT - 1) / NT;
+    const bool valid = mode < plane;
+    // plane is far below 2^32 (the FFT lengths are bounded by LDS)
+    const unsigned um = valid ? (unsigned)mode : 0u;
+    const double axv = ax[um % (unsigned)nx], azv = az[um / (unsigned)nx];
+    double bv[KB];
+#pragma unroll
+    for (int q = 0; q < KB; ++q) {
+        const int e = min((int)threadIdx.x + q * NT, 3 * NP - 1);
         const int which = e / NP, row = e - which * NP;
-        smb[e] = row < ny ? (Ar)set[which * ny + row] : (which == 1 ? Ar(-1) : Ar(0));
+        bv[q] = set[which * ny + min(row, ny - 1)];
     }
+    tri_load_rows<MR, P, M, T, Ar, MM>(d, S, mode, plane, ny, ((int)threadIdx.x / M) * MR, msum, nx, nz);
+#pragma unroll
+    for (int q = 0; q < KB; ++q) {
+        const int e = (int)threadIdx.x + q * NT;
+        if (e < 3 * NP) {
+            const int which = e / NP, row = e - which * NP;
+            smb[e] = row < ny ? (Ar)bv[q] : (which == 1 ? Ar(-1) : Ar(0));
+        }
+    }
+    return valid ? axv + azv : -1.0;
 }
 
 // One mode group: thread (mi, p) holds rows s..s+MR-1 of mode `mode` in d[]
 // and writes its solution rows (mode >= plane: an idle lane).  smb: the band
-// set of every mode of the group, staged in LDS by tri_stage_bands; R: the
+// set of every mode of the group, staged in LDS by tri_prologue; R: the
 // LDS reduced system.  Contains workgroup barriers; all threads call it with
 // the same group.  Ar: the arithmetic type (bands in smb, d[], R).
 //
@@ -723,8 +749,7 @@ __device__ __forceinline__ void tri_stage_bands(Ar* smb, const double* __restric
 // two values as boundary data.  About a third fewer values stay live across
 // the reduced solve, for one more elimination of the chunk.
 template <int MR, int P, int M, typename T, bool TP = false, typename Ar = double, bool MM = false>
-__device__ __forceinline__ void tri_group(Ar (&d)[MR], T* S, const double* __restrict__ ax,
-                                          const double* __restrict__ az, const Ar* smb, int nx, int ny,
+__device__ __forceinline__ void tri_group(Ar (&d)[MR], T* S, double lam64, const Ar* smb, int nx, int ny,
                                           long plane, double pcr_min, long mode, const double* __restrict__ msum,
                                           double* __restrict__ pbar, double inv_nxnz, TriRed<P, M, Ar>& R,
                                           int nz = 0, T* So = nullptr) {
@@ -733,12 +758,6 @@ __device__ __forceinline__ void tri_group(Ar (&d)[MR], T* S, const double* __res
     const int s = p * MR;
     const bool valid = mode < plane;
     const long md = valid ? mode : 0;
-    // plane is far below 2^32 (the FFT lengths are bounded by LDS)
-    const unsigned um = (unsigned)md;
-    const int i = (int)(um % (unsigned)nx);
-    const int j = (int)(um / (unsigned)nx);
-    // idle lanes solve a regular system (the plain bands are singular at 0)
-    const double lam64 = valid ? ax[i] + az[j] : -1.0;
     const Ar lam = (Ar)lam64;
     const Ar* A = smb;
     const Ar* B = A + MR * P;
@@ -969,22 +988,20 @@ __global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __
                                                           double inv_nxnz = 0.0,
                                                           const int* __restrict__ glist = nullptr, int nz = 0,
                                                           T* So = nullptr) {
-    constexpr int NT = M * P;
     __shared__ TriRed<P, M, Ar> red;
     extern __shared__ double smb_raw[];  // the group's band set, 3 * MR * P values
     Ar* smb = reinterpret_cast<Ar*>(smb_raw);
     const int mi = (int)threadIdx.x % M;
-    const int s = ((int)threadIdx.x / M) * MR;
     // adjacent mode groups share cache lines: keep them on one XCD (each has
     // its own L2; workgroups reach XCDs round-robin)
     const int slot = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const long grp = glist ? (long)glist[slot] : (long)slot;
     const long mode = tri_group_mode(grp, (plane + M - 1) / M, M, mi);
-    tri_stage_bands<MR, P, Ar>(smb, bands + (grp == 0 ? 3 * ny : 0), ny, NT);
     Ar d[MR];
-    tri_load_rows<MR, P, M, T, Ar, MM>(d, S, mode, plane, ny, s, msum, nx, nz);
+    const double lam64 = tri_prologue<MR, P, M, T, Ar, MM>(d, smb, bands + (grp == 0 ? 3 * ny : 0), S, ax, az, mode,
+                                                           plane, nx, ny, msum, nz);
     __syncthreads();  // bands staged
-    tri_group<MR, P, M, T, TP, Ar, MM>(d, S, ax, az, smb, nx, ny, plane, pcr_min, mode, msum, pbar, inv_nxnz, red,
+    tri_group<MR, P, M, T, TP, Ar, MM>(d, S, lam64, smb, nx, ny, plane, pcr_min, mode, msum, pbar, inv_nxnz, red,
                                        nz, So);
 }
 

@@ -520,32 +520,34 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
         for (int e = (int)threadIdx.x; e < Q_N * kBuckets; e += kThreads) tab[e] = tab_g[e];
     }
     __syncthreads();
-    const long n = g.total();
-    for (long idx = (long)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (long)gridDim.x * kThreads) {
-        double te = T[idx];
-        if (bot != nullptr) {
-            int i, j, k;
-            if constexpr (IDX32) {
-                const unsigned pe = (unsigned)g.sy(), px = (unsigned)(g.nx + 2), x = (unsigned)idx;
-                const unsigned kp = x / pe, r = x - kp * pe, jp = r / px;
-                k = (int)kp - 1;
-                j = (int)jp - 1;
-                i = (int)(r - jp * px) - 1;
-            } else {
-                const long pe = g.sy();
-                k = (int)(idx / pe) - 1;
-                const long r = idx - (long)(k + 1) * pe;
-                j = (int)(r / (g.nx + 2)) - 1;
-                i = (int)(r - (long)(j + 1) * (g.nx + 2)) - 1;
-            }
-            const int wi = i < 0 ? g.nx - 1 : (i >= g.nx ? 0 : i);
-            const int wj = j < 0 ? g.nz - 1 : (j >= g.nz ? 0 : j);
-            if (k >= 0 && k < g.ny) {
-                const long col = (long)wi + (long)g.nx * wj;
-                if (k <= bot[col]) te = t_bot;
-                if (k >= top[col]) te = t_top;
-            }
+    // T_ext: in rough-wall columns the wall values
+    auto wall_te = [&](long idx, double te) -> double {
+        if (bot == nullptr) return te;
+        int i, j, k;
+        if constexpr (IDX32) {
+            const unsigned pe = (unsigned)g.sy(), px = (unsigned)(g.nx + 2), x = (unsigned)idx;
+            const unsigned kp = x / pe, r = x - kp * pe, jp = r / px;
+            k = (int)kp - 1;
+            j = (int)jp - 1;
+            i = (int)(r - jp * px) - 1;
+        } else {
+            const long pe = g.sy();
+            k = (int)(idx / pe) - 1;
+            const long r = idx - (long)(k + 1) * pe;
+            j = (int)(r / (g.nx + 2)) - 1;
+            i = (int)(r - (long)(j + 1) * (g.nx + 2)) - 1;
         }
+        const int wi = i < 0 ? g.nx - 1 : (i >= g.nx ? 0 : i);
+        const int wj = j < 0 ? g.nz - 1 : (j >= g.nz ? 0 : j);
+        if (k >= 0 && k < g.ny) {
+            const long col = (long)wi + (long)g.nx * wj;
+            if (k <= bot[col]) te = t_bot;
+            if (k >= top[col]) te = t_top;
+        }
+        return te;
+    };
+    // the seven properties at one temperature: o = enth, rho, mu, cond, cp, divjac, buoy
+    auto eval = [&](double te, double* o) {
         double h, r, m, k, c;
         const int* ms = nullptr;
         if constexpr (ALL_CUBIC) {
@@ -562,21 +564,42 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
             k = law_eval<false>(ps.law[Q_COND], seg, tab + Q_COND * kBuckets, te);
             c = law_eval<false>(ps.law[Q_CP], seg, tab + Q_CP * kBuckets, te);
         }
-        T_ext[idx] = te;
-        enth[idx] = h;
-        rho[idx] = r;
-        mu[idx] = m;
-        cond[idx] = k;
-        cp[idx] = c;
+        o[0] = h;
+        o[1] = r;
+        o[2] = m;
+        o[3] = k;
+        o[4] = c;
         if (ps.divjac_formula) {
             double dr;
             if constexpr (ALL_CUBIC) dr = seg_eval(seg, ms[Q_DRHO], te);
             else dr = law_eval<false>(ps.law[Q_DRHO], seg, tab + Q_DRHO * kBuckets, te);
-            divjac[idx] = (-1.0 / ((r * r) * c)) * dr;
+            o[5] = (-1.0 / ((r * r) * c)) * dr;
         } else {
-            divjac[idx] = ps.divjac_c;
+            o[5] = ps.divjac_c;
         }
-        buoy[idx] = ps.buoy_linear ? ps.b1 * (r - ps.b0) : 0.0;
+        o[6] = ps.buoy_linear ? ps.b1 * (r - ps.b0) : 0.0;
+    };
+    double* const out[8] = {T_ext, enth, rho, mu, cond, cp, divjac, buoy};
+    // two consecutive points per thread: 16-byte loads and stores (every
+    // field is its own allocation, so pair p sits at a 16-byte boundary)
+    const long n = g.total();
+    const long np = n >> 1;
+    for (long p = (long)blockIdx.x * kThreads + threadIdx.x; p < np; p += (long)gridDim.x * kThreads) {
+        const double2 t = reinterpret_cast<const double2*>(T)[p];
+        const double te0 = wall_te(2 * p, t.x), te1 = wall_te(2 * p + 1, t.y);
+        double o0[7], o1[7];
+        eval(te0, o0);
+        eval(te1, o1);
+        reinterpret_cast<double2*>(out[0])[p] = make_double2(te0, te1);
+#pragma unroll
+        for (int q = 0; q < 7; ++q) reinterpret_cast<double2*>(out[q + 1])[p] = make_double2(o0[q], o1[q]);
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const double te = wall_te(n - 1, T[n - 1]);
+        double o[7];
+        eval(te, o);
+        out[0][n - 1] = te;
+        for (int q = 0; q < 7; ++q) out[q + 1][n - 1] = o[q];
     }
 }
 
