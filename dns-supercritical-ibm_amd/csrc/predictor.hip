@@ -182,19 +182,16 @@ __global__ void k_div_finish(double* D, Grid g) {
 // points of (i, j, k), the y one at the V point when k >= kmin_Vhat;
 // X_hat = X + dt (1.5 R - 0.5 R_old), R_old = R
 // (mod_cfd_subroutines.f90:147-354).  u, v, w, m(di, dj, dk) return U, V,
-// W, mu at (i + di, j + dj, k + dk).
-template <bool VARPROP, class AU, class AV, class AW, class AM>
+// W, mu at (i + di, j + dj, k + dk); r, dv, b likewise rho, div_now and the
+// buoyancy field (read at the point and at (-1, 0, 0), (0, -1, 0), (0, 0, -1)).
+template <bool VARPROP, class AU, class AV, class AW, class AM, class AR, class AD, class AB>
 __device__ __forceinline__ void momentum_point(
-    const AU& u, const AV& v, const AW& w, const AM& m, int i, int j, int k, const double* __restrict__ rho,
-    const double* __restrict__ dvn, const double* __restrict__ buoy, double* __restrict__ Ru_old,
-    double* __restrict__ Rv_old, double* __restrict__ Rw_old, double* __restrict__ U_hat, double* __restrict__ V_hat,
-    double* __restrict__ W_hat, const Grid& g, const Geom& G, const Phys& p, int kmin_Vhat) {
+    const AU& u, const AV& v, const AW& w, const AM& m, const AR& r, const AD& dv, const AB& b, int i, int j, int k,
+    double* __restrict__ Ru_old, double* __restrict__ Rv_old, double* __restrict__ Rw_old,
+    double* __restrict__ U_hat, double* __restrict__ V_hat, double* __restrict__ W_hat, const Grid& g,
+    const Geom& G, const Phys& p, int kmin_Vhat) {
     const long c = g.at(i, j, k);
-    const long sz = g.sz(), sy = g.sy();
     const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * k);
-    auto r = [&](int di, int dj, int dk) { return rho[c + di + dj * sz + dk * sy]; };
-    auto dv = [&](int di, int dj, int dk) { return dvn[c + di + dj * sz + dk * sy]; };
-    auto b = [&](int di, int dj, int dk) { return buoy[c + di + dj * sz + dk * sy]; };
     const double idx = p.idx, idz = p.idz, idx2 = p.idx2, idz2 = p.idz2, idx__2 = p.idx__2, idz__2 = p.idz__2;
     {
         const double Ra = G.ruw[3 * k], Rb = G.ruw[3 * k + 1], Rc = G.ruw[3 * k + 2];
@@ -324,6 +321,18 @@ __global__ __launch_bounds__(kThreads) void k_momentum(
     ring.put(k0);
     ring.fetch(k0 + 1);
     const int e0 = ring_point();
+    const long sz = g.sz(), sy = g.sy();
+    // rho, div_now and buoy of this thread's column at plane k - 1: read at
+    // the march's first plane, then carried from the step before (the plane
+    // below is otherwise read again from beyond L2 once the step's other
+    // streams have pushed it out)
+    double rm = 0.0, dm = 0.0, bm = 0.0;
+    if (VARPROP && active) {
+        const long cm = g.at(i, j, k0 - 1);
+        rm = rho[cm];
+        dm = dvn[cm];
+        bm = buoy[cm];
+    }
     for (int k = k0; k < k1; ++k) {
         ring.put(k + 1);  // the slot of plane k - 2, free since the last barrier
         __syncthreads();
@@ -335,8 +344,25 @@ __global__ __launch_bounds__(kThreads) void k_momentum(
             auto lv = [&](int di, int dj, int dk) { return sF[1][slot(dk)][e0 + di + dj * kRingX]; };
             auto lw = [&](int di, int dj, int dk) { return sF[2][slot(dk)][e0 + di + dj * kRingX]; };
             auto lm = [&](int di, int dj, int dk) { return sF[NF - 1][slot(dk)][e0 + di + dj * kRingX]; };
-            momentum_point<VARPROP>(lu, lv, lw, lm, i, j, k, rho, dvn, buoy, Ru_old, Rv_old, Rw_old, U_hat, V_hat,
-                                    W_hat, g, G, p, kmin_Vhat);
+            const long c = g.at(i, j, k);
+            double r0 = 0.0, d0 = 0.0, b0 = 0.0;
+            if constexpr (VARPROP) {
+                r0 = rho[c];
+                d0 = dvn[c];
+                b0 = buoy[c];
+            }
+            // (0, 0, -1) from registers, (0, 0, 0) as loaded, x and z neighbours from memory
+            auto at = [&](const double* f, double cen, double below, int di, int dj, int dk) {
+                return dk < 0 ? below : ((di | dj) == 0 ? cen : f[c + di + dj * sz + dk * sy]);
+            };
+            auto lr = [&](int di, int dj, int dk) { return at(rho, r0, rm, di, dj, dk); };
+            auto ld = [&](int di, int dj, int dk) { return at(dvn, d0, dm, di, dj, dk); };
+            auto lb = [&](int di, int dj, int dk) { return at(buoy, b0, bm, di, dj, dk); };
+            momentum_point<VARPROP>(lu, lv, lw, lm, lr, ld, lb, i, j, k, Ru_old, Rv_old, Rw_old, U_hat, V_hat, W_hat,
+                                    g, G, p, kmin_Vhat);
+            rm = r0;
+            dm = d0;
+            bm = b0;
         }
         __syncthreads();  // slot (k + 2) % 3 is restaged next
     }
