@@ -41,6 +41,23 @@ constexpr int ct_cols() {  // complex columns per z workgroup, at most 16
 // XB = true: S receives the x-blocked layout of the blocked solve,
 // [k][a/16][j][a%16] (line k*nz + j, halfcomplex index a; nx a multiple of
 // 16), instead of the lines in place.
+// Plane k and index j of line l0 + r of a tile whose first line l0 is
+// k0 * nz + j0 (0 <= j0 < nz), with mz = line_div_magic(nz): r + j0 divided
+// by nz as a multiply-high by ceil(2^32 / nz), exact while (j0 + r) * nz <
+// 2^32 (nz is an FFT length, far below 2^16).  Branch-free: control flow
+// here would split an unrolled element loop into basic blocks, which keeps
+// the scheduler from issuing the next elements' loads early; and a division
+// per element is a dozen instructions.
+__device__ __forceinline__ unsigned long long line_div_magic(int nz) {
+    return 0xFFFFFFFFull / (unsigned)nz + 1ull;
+}
+__device__ __forceinline__ void tile_line_kj(int k0, int j0, int r, int nz, unsigned long long mz, int& k, int& j) {
+    const unsigned jr = (unsigned)(j0 + r);
+    const unsigned dk = (unsigned)(((unsigned long long)jr * mz) >> 32);
+    k = k0 + (int)dk;
+    j = (int)(jr - dk * (unsigned)nz);
+}
+
 template <int N, int G, int NT, bool XB = false>
 __device__ __forceinline__ void xfwd_body(const double* P, double* S, long nlines, const double2* __restrict__ tw,
                                           long tile, double2* sm, int nz = 0) {
@@ -49,6 +66,13 @@ __device__ __forceinline__ void xfwd_body(const double* P, double* S, long nline
     double* smd = reinterpret_cast<double*>(sm);
     const long l0 = tile * 2 * G;
     const double2* src = reinterpret_cast<const double2*>(P + l0 * N);
+    int kb = 0, jb = 0;
+    unsigned long long mz = 0;
+    if constexpr (XB) {
+        kb = (int)(l0 / nz);
+        jb = (int)(l0 - (long)kb * nz);
+        mz = line_div_magic(nz);
+    }
     double2 v[NL];
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
@@ -89,10 +113,9 @@ __device__ __forceinline__ void xfwd_body(const double* P, double* S, long nline
                 else              out[h] = lo ? 0.5 * (C.y + Cm.y) : 0.5 * (Cm.x - C.x);
             }
             if constexpr (XB) {
-                const long line = l0 + r;
-                const long k = line / nz;
-                const long j = line - k * nz;
-                *reinterpret_cast<double2*>(S + ((k * (N / 16) + (t0 >> 4)) * nz + j) * 16 + (t0 & 15)) =
+                int k, j;
+                tile_line_kj(kb, jb, r, nz, mz, k, j);
+                *reinterpret_cast<double2*>(S + (((long)k * (N / 16) + (t0 >> 4)) * nz + j) * 16 + (t0 & 15)) =
                     make_double2(out[0], out[1]);
             } else {
                 dst[e] = make_double2(out[0], out[1]);

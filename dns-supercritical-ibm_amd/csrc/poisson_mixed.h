@@ -174,17 +174,6 @@ __device__ __forceinline__ long strip_tile(long t, int nz, int ny) {
     return k * tpp + jg;
 }
 
-// Plane k and index j of line l0 + r, for a tile whose first line l0 is
-// k0 * nz + j0.  A 32-bit division (a short branch-free sequence): control
-// flow here would split the unrolled element loop into basic blocks, which
-// keeps the scheduler from issuing the next elements' loads early.
-__device__ __forceinline__ void tile_line_kj(int k0, int j0, int r, int nz, int& k, int& j) {
-    const unsigned jr = (unsigned)(j0 + r);
-    const unsigned dk = jr / (unsigned)nz;
-    k = k0 + (int)dk;
-    j = (int)(jr - dk * (unsigned)nz);
-}
-
 template <int N, int G, int NT, typename TP>
 __device__ __forceinline__ void xfwd_mr_body(const TP* __restrict__ p, const double* __restrict__ f,
                                              const double* __restrict__ qk, const double* __restrict__ bands,
@@ -198,6 +187,7 @@ __device__ __forceinline__ void xfwd_mr_body(const TP* __restrict__ p, const dou
     const long l0 = strip_tile<G>(tile, nz, ny) * 2 * G;
     const int k0 = (int)(l0 / nz);
     const int j0 = (int)(l0 - (long)k0 * nz);
+    const unsigned long long mz = line_div_magic(nz);
     // Branch-free: every address is clamped to a valid line and the wall
     // terms carry a zero coefficient, so all loads of an element issue
     // together (branches around loads serialise their round trips).
@@ -212,7 +202,7 @@ __device__ __forceinline__ void xfwd_mr_body(const TP* __restrict__ p, const dou
         const int rr = ok ? r : 0;
         const long line = l0 + rr;
         int k, j;
-        tile_line_kj(k0, j0, rr, nz, k, j);
+        tile_line_kj(k0, j0, rr, nz, mz, k, j);
         const TP* pl = p + line * N;
         const P2 c = *reinterpret_cast<const P2*>(pl + t0);
         const TP xl = pl[t0 == 0 ? N - 1 : t0 - 1];
@@ -544,6 +534,7 @@ __device__ __forceinline__ void xbwd_ma_body(const float* __restrict__ S, const 
     const long l0 = tile * 2 * G;
     const int k0 = (int)(l0 / nz);
     const int j0 = (int)(l0 - (long)k0 * nz);
+    const unsigned long long mz = line_div_magic(nz);
     xbwd_load_f32<N, G, NT>(sm, S, l0, nlines);
     fft_ct<N, G, NT, true>(sm, tw);
     const float* smf = reinterpret_cast<const float*>(sm);
@@ -558,7 +549,7 @@ __device__ __forceinline__ void xbwd_ma_body(const float* __restrict__ S, const 
             const int g = r >> 1, c = r & 1;
             const long line = l0 + r;
             int k, j;
-            tile_line_kj(k0, j0, r, nz, k, j);
+            tile_line_kj(k0, j0, r, nz, mz, k, j);
             const double add = (pbar0 ? pbar0[k] : 0.0) + (pbar1 ? pbar1[k] : 0.0);
             P2 b;
             b.x = TP(0);
