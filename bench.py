@@ -75,6 +75,11 @@ def bench_batch(torch, D, W):
         return e0.elapsed_time(e1) / K
 
     ms64 = timed(b.run)
+    # the same FP64 solve in one persistent cooperative launch (fixed variants)
+    try:
+        msp, perr = timed(b.run_persistent), None
+    except D.DnspError as e:
+        msp, perr = None, str(e)
     msm = timed(lambda: b.run_mixed(1))
     worst = 0.0  # of the mixed results just timed
     for c, fi, Pi in zip(cases, f, P):
@@ -87,7 +92,11 @@ def bench_batch(torch, D, W):
            "points": int(sum(t.numel() for t in f)),
            "fp64_ms": round(ms64, 4), "fp64_solves_per_s": round(n / ms64 * 1e3, 1),
            "mixed_ms": round(msm, 4), "mixed_solves_per_s": round(n / msm * 1e3, 1),
-           "mixed_worst_rel_residual": worst, "variants": list(b.variants)}
+           "mixed_worst_rel_residual": worst, "variants": list(b.variants),
+           "fp64_persistent_ms": None if msp is None else round(msp, 4),
+           "fp64_persistent_solves_per_s": None if msp is None else round(n / msp * 1e3, 1)}
+    if perr:
+        out["fp64_persistent_error"] = perr
     b.close()
     del f, P
     torch.cuda.empty_cache()

@@ -20,6 +20,8 @@
 #include <type_traits>
 #include <vector>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "common.h"
 #include "poisson_ct.h"
 #include "poisson_ct_launch.h"
@@ -354,6 +356,146 @@ static void bt_launch(int kind, int v, const BInst* inst, const BGroup& g, hipSt
     }
 }
 
+// ---------------------------------------------------------------------------
+// The FP64 batched solve in one persistent launch.  A cooperative grid of
+// kPersistThreads-thread workgroups (all resident at once) runs the five
+// stages; within a stage workgroup w takes tasks w, w + grid, w + 2 grid, ...
+// (w numbered by XCD as xcd_tile numbers tiles, so that each XCD holds
+// consecutive tasks at every step, and tasks of every size spread over all
+// workgroups), finds
+// each task's launch group (transform length or rows per chunk) and
+// instance, and runs the same tile body as the per-length kernels of that
+// group with x and z variant kPersistXV / kPersistZV and the two-pass
+// tridiagonal; a grid barrier separates the stages.  One dynamic LDS buffer,
+// sized for the largest body of the batch, serves every task.
+// ---------------------------------------------------------------------------
+constexpr int kPersistThreads = 256;
+constexpr int kPersistXV = 2, kPersistZV = 2;  // (1536, 256) tiles
+constexpr int kPersistMaxRows = 12;           // rows per chunk (ny <= 12 * kPartChunks)
+static_assert(ct_xvar_threads(kPersistXV) == kPersistThreads && ct_zvar_threads(kPersistZV) == kPersistThreads &&
+                  kBatchM * kPartChunks == kPersistThreads,
+              "one workgroup size for every stage");
+
+// One launch group of one stage: its tasks are [first, first + the group's
+// total) of the stage.
+struct PSeg {
+    long first;
+    int key, cnt;
+    const int* ids;
+    const long* start;
+};
+struct PPlan {
+    int s0[6];      // stage s's groups: seg[s0[s] .. s0[s + 1])
+    long total[5];  // tasks per stage
+};
+
+// One task of each kind and size, each a function of its own (not
+// inlined), so that every body keeps the register allocation it has in its
+// own kernel; one inlined switch over all of them would be allocated for
+// all of them at once.
+template <int N, bool FWD>
+__device__ __attribute__((noinline)) void px_task_n(const BInst& b, long tile, double2* sm) {
+    constexpr int G = ct_lines<N, ct_xvar_tile(kPersistXV)>();
+    if constexpr (FWD) xfwd_body<N, G, kPersistThreads>(b.f, b.P, b.nlines, b.twx, tile, sm);
+    else xbwd_body<N, G, kPersistThreads>(b.P, b.P, b.nlines, b.twx, b.inv_nxnz, tile, sm);
+}
+
+template <int N, bool FWD>
+__device__ __attribute__((noinline)) void pz_task_n(const BInst& b, int tile, double2* sm) {
+    constexpr int TA = ct_cols<N, ct_zvar_tile(kPersistZV)>();
+    const int ntc = ((b.nx + 1) / 2 + TA - 1) / TA;
+    if constexpr (FWD) zfwd_body<N, TA, kPersistThreads, double>(b.P, b.nx, ntc, b.twz, nullptr, nullptr, tile, sm);
+    else zbwd_body<N, TA, kPersistThreads, double>(b.P, b.nx, ntc, b.twz, tile, sm);
+}
+
+template <int MR>
+__device__ __attribute__((noinline)) void pt_task_m(const BInst& b, long grp, char* lds) {
+    constexpr int M = kBatchM, P = kPartChunks;
+    TriRed<P, M>& red = *reinterpret_cast<TriRed<P, M>*>(lds);
+    double* smb = reinterpret_cast<double*>(lds + sizeof(TriRed<P, M>));
+    const int mi = (int)threadIdx.x % M;
+    const long mode = tri_group_mode(grp, (b.plane + M - 1) / M, M, mi);
+    double d[MR];
+    const double lam64 = tri_prologue<MR, P, M, double>(d, smb, b.bands + (grp == 0 ? 3 * b.ny : 0), b.P, b.ax, b.az,
+                                                        mode, b.plane, b.nx, b.ny, nullptr);
+    __syncthreads();
+    tri_group<MR, P, M, double, true>(d, b.P, lam64, smb, b.nx, b.ny, b.plane, b.pcr_min, mode, nullptr, nullptr,
+                                      b.inv_nxnz, red);
+}
+
+template <bool FWD>
+__device__ __forceinline__ void px_task(int n, const BInst& b, long tile, double2* sm) {
+    switch (n) {
+#define DNSP_CASE(N) case N: px_task_n<N, FWD>(b, tile, sm); break;
+        DNSP_CT_SIZES(DNSP_CASE)
+#undef DNSP_CASE
+        default: break;
+    }
+}
+
+template <bool FWD>
+__device__ __forceinline__ void pz_task(int n, const BInst& b, int tile, double2* sm) {
+    switch (n) {
+#define DNSP_CASE(N) case N: pz_task_n<N, FWD>(b, tile, sm); break;
+        DNSP_CT_SIZES(DNSP_CASE)
+#undef DNSP_CASE
+        default: break;
+    }
+}
+
+__device__ __forceinline__ void pt_task(int mr, const BInst& b, long grp, char* lds) {
+    switch (mr) {
+        case 2: pt_task_m<2>(b, grp, lds); break;
+        case 3: pt_task_m<3>(b, grp, lds); break;
+        case 4: pt_task_m<4>(b, grp, lds); break;
+        case 6: pt_task_m<6>(b, grp, lds); break;
+        case 8: pt_task_m<8>(b, grp, lds); break;
+        case 12: pt_task_m<12>(b, grp, lds); break;
+        default: break;
+    }
+}
+
+// LDS bytes a task of stage kind s (0, 4: x; 1, 3: z; 2: tridiagonal) with
+// key n takes (host side).
+inline size_t persist_lds(int s, int n) {
+    if (s == 2) return sizeof(TriRed<kPartChunks, kBatchM>) + 3 * sizeof(double) * (size_t)n * kPartChunks;
+    const int tile = s == 0 || s == 4 ? ct_xvar_tile(kPersistXV) : ct_zvar_tile(kPersistZV);
+    const int lines = s == 0 || s == 4 ? std::max(1, tile / n) : std::min(16, std::max(1, tile / n));
+    return sizeof(double2) * (size_t)lines * (size_t)ct_pitch(n);
+}
+
+// Every workgroup runs every stage and reaches every grid barrier (no early
+// return); the launch is cooperative, so all workgroups are resident.
+__global__ __launch_bounds__(kPersistThreads) void kb_persist(const BInst* __restrict__ inst,
+                                                              const PSeg* __restrict__ seg, PPlan plan) {
+    extern __shared__ __align__(16) char plds[];
+    double2* sm = reinterpret_cast<double2*>(plds);
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const long lid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const long nwg = gridDim.x;
+    for (int s = 0; s < 5; ++s) {
+        const long tot = plan.total[s];
+        int q = plan.s0[s];
+        for (long t = lid; t < tot; t += nwg) {
+            while (q + 1 < plan.s0[s + 1] && seg[q + 1].first <= t) ++q;
+            const PSeg& g = seg[q];
+            const long task = t - g.first;
+            const int u = find_inst(g.start, g.cnt, task);
+            const BInst& b = inst[g.ids[u]];
+            const long tile = task - g.start[u];
+            switch (s) {
+                case 0: px_task<true>(g.key, b, tile, sm); break;
+                case 1: pz_task<true>(g.key, b, (int)tile, sm); break;
+                case 2: pt_task(g.key, b, tile, plds); break;
+                case 3: pz_task<false>(g.key, b, (int)tile, sm); break;
+                default: px_task<false>(g.key, b, tile, sm); break;
+            }
+            __syncthreads();  // LDS is reused by the next task
+        }
+        if (s < 4) grid.sync();
+    }
+}
+
 }  // namespace dnsp
 
 using namespace dnsp;
@@ -377,6 +519,12 @@ struct dnsp_batch {
     int xvar = 0, zvar = 0, tvar = 1;     // FP64 form
     int mxvar = 0, mzvar = 0, mtvar = 1;  // mixed form
     bool bound = false;
+    // persistent launch (built at its first use): group table, plan, LDS
+    // bytes, grid
+    PSeg* d_pseg = nullptr;
+    PPlan pplan{};
+    size_t p_lds = 0;
+    int p_grid = 0;
 };
 
 namespace {
@@ -652,6 +800,52 @@ void run_mixed(dnsp_batch* b, int refinements, int xv, int zv, int tv, hipStream
     for (const BGroup& g : b->gx[xv]) bx_launch(BX_MX_ACC, xv, b->d_inst, g, 0, st);
 }
 
+// The persistent launch's group table and grid, at its first use.
+int persist_setup(dnsp_batch* b) {
+    if (b->d_pseg) return DNSP_OK;
+    const std::vector<BGroup>* st[5] = {&b->gx[kPersistXV], &b->gz[kPersistZV], &b->gt, &b->gz[kPersistZV],
+                                        &b->gx[kPersistXV]};
+    std::vector<PSeg> segs;
+    PPlan plan{};
+    size_t lds = 0;
+    for (int s = 0; s < 5; ++s) {
+        plan.s0[s] = (int)segs.size();
+        long first = 0;
+        for (const BGroup& g : *st[s]) {
+            if (g.total <= 0) continue;
+            if (s == 2 && g.key > kPersistMaxRows) {
+                set_error("persistent batched solve: %d rows per chunk (ny > %d) not supported", g.key,
+                          kPersistMaxRows * kPartChunks);
+                return DNSP_ERR_UNSUPPORTED;
+            }
+            segs.push_back(PSeg{first, g.key, g.cnt, g.ids, g.start});
+            first += g.total;
+            lds = std::max(lds, persist_lds(s, g.key));
+        }
+        plan.total[s] = first;
+    }
+    plan.s0[5] = (int)segs.size();
+    int coop = 0, ncu = 0, per_cu = 0;
+    DNSP_HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, b->device));
+    if (!coop) {
+        set_error("device %d does not support cooperative launches", b->device);
+        return DNSP_ERR_UNSUPPORTED;
+    }
+    DNSP_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
+    DNSP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kb_persist),
+                                                              kPersistThreads, lds));
+    if (per_cu < 1) {
+        set_error("persistent batched solve: no workgroup fits a CU (%zu bytes of LDS)", lds);
+        return DNSP_ERR_UNSUPPORTED;
+    }
+    int rc = dev_copy(b, &b->d_pseg, segs.data(), segs.size());
+    if (rc) return rc;
+    b->pplan = plan;
+    b->p_lds = lds;
+    b->p_grid = per_cu * ncu;
+    return DNSP_OK;
+}
+
 int check_bound(const dnsp_batch* b) {
     DNSP_ARG_CHECK(b, "NULL batch");
     DNSP_ARG_CHECK(b->bound, "no arrays bound to the batch (dnsp_batch_bind)");
@@ -731,6 +925,23 @@ int dnsp_batch_run(dnsp_batch* b, void* stream) {
     if (rc) return rc;
     run_fp64(b, b->xvar, b->zvar, b->tvar, as_stream(stream));
     DNSP_LAUNCH_CHECK();
+    return DNSP_OK;
+}
+
+int dnsp_batch_run_persistent(dnsp_batch* b, void* stream) {
+    clear_error();
+    int rc = check_bound(b);
+    if (!rc) {
+        DNSP_HIP_TRY(hipSetDevice(b->device));
+        rc = persist_setup(b);
+    }
+    if (rc) return rc;
+    const BInst* inst = b->d_inst;
+    const PSeg* seg = b->d_pseg;
+    PPlan plan = b->pplan;
+    void* args[] = {&inst, &seg, &plan};
+    DNSP_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kb_persist), dim3(b->p_grid),
+                                            dim3(kPersistThreads), args, (unsigned)b->p_lds, as_stream(stream)));
     return DNSP_OK;
 }
 

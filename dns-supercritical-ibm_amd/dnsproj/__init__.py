@@ -91,6 +91,7 @@ _SIGNATURES = [
      ctypes.c_int),
     ("dnsp_batch_bind", [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], ctypes.c_int),
     ("dnsp_batch_run", [_vp, _vp], ctypes.c_int),
+    ("dnsp_batch_run_persistent", [_vp, _vp], ctypes.c_int),
     ("dnsp_batch_run_mixed", [_vp, ctypes.c_int, _vp], ctypes.c_int),
     ("dnsp_batch_autotune", [_vp, ctypes.c_int, _vp, _dp], ctypes.c_int),
     ("dnsp_batch_set_variants", [_vp] + [ctypes.c_int] * 6, ctypes.c_int),
@@ -448,6 +449,12 @@ class PoissonBatch:
 
     def run(self, stream=None):
         _check(lib().dnsp_batch_run(self._h, _vp(_stream_ptr(stream))))
+
+    def run_persistent(self, stream=None):
+        """The FP64 solve of every instance in one cooperative launch (x and z
+        variant 2, two-pass tridiagonal), results as run() with those
+        variants."""
+        _check(lib().dnsp_batch_run_persistent(self._h, _vp(_stream_ptr(stream))))
 
     def run_mixed(self, refinements=1, stream=None):
         _check(lib().dnsp_batch_run_mixed(self._h, int(refinements), _vp(_stream_ptr(stream))))

@@ -273,6 +273,16 @@ int dnsp_batch_create(int n, const int* nx, const int* nz, const int* ny, const 
                       const double* inv_dz, const double* bands, int precision, dnsp_batch** out);
 int dnsp_batch_bind(dnsp_batch* b, const double* const* rhs, double* const* P);
 int dnsp_batch_run(dnsp_batch* b, void* stream);
+/* The FP64 solve of every instance in ONE persistent cooperative launch: all
+ * workgroups resident, the five stages separated by grid barriers, the
+ * workgroups taking every stage's tiles of all instances in turn.  x and z use variant 2 ((1536, 256) tiles), the tridiagonal
+ * two-pass 8-mode groups: results equal dnsp_batch_run with
+ * dnsp_batch_set_variants(b, 2, 2, 2, ...).  ny <= 384 (DNSP_ERR_UNSUPPORTED
+ * otherwise, or where the device has no cooperative launch).  One kernel
+ * over all sizes holds the registers of the largest body, so it is slower
+ * than the per-size launches of dnsp_batch_run (measured in bench.py).
+ * Not capturable in a graph.  No counterpart in the reference. */
+int dnsp_batch_run_persistent(dnsp_batch* b, void* stream);
 int dnsp_batch_run_mixed(dnsp_batch* b, int refinements, void* stream);
 int dnsp_batch_autotune(dnsp_batch* b, int reps, void* stream, double* times_ms);
 int dnsp_batch_set_variants(dnsp_batch* b, int xvar, int zvar, int tvar, int mxvar, int mzvar, int mtvar);

@@ -93,6 +93,28 @@ def test_batch_fp64_equals_single_and_oracle(cuda, tvar):
     b.close()
 
 
+def test_batch_persistent_equals_per_size_launches(cuda):
+    """One cooperative launch over all instances and stages gives the results
+    of the per-size launches with the same variants (x 2, z 2, two-pass
+    tridiagonal), bit for bit, and the oracle's to 1e-12."""
+    say("batch persistent")
+    cases = small_cases(1)
+    b = D.PoissonBatch(cases)
+    f, P = bind(b, cases)
+    b.set_variants(2, 2, 2)
+    b.run()
+    ref = [to_host(Pi) for Pi in P]
+    for Pi in P:
+        Pi.zero_()
+    for _ in range(2):  # the second launch reuses the group table
+        b.run_persistent()
+        for c, Pi, r in zip(cases, P, ref):
+            assert np.array_equal(to_host(Pi), r), (c["nx"], c["nz"], c["ny"])
+    for c, Pi in zip(cases, P):
+        assert rel_mean_removed(to_host(Pi), O.solve(c["rhs"], c["inv_dx"], c["inv_dz"], c["bands"])) <= 1e-12
+    b.close()
+
+
 def test_batch_mixed_equals_single_and_reaches_target(cuda):
     say("batch mixed")
     cases = small_cases(2)
