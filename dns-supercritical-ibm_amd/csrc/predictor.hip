@@ -610,8 +610,16 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
     // field is its own allocation, so pair p sits at a 16-byte boundary)
     const long n = g.total();
     const long np = n >> 1;
-    for (long p = (long)blockIdx.x * kThreads + threadIdx.x; p < np; p += (long)gridDim.x * kThreads) {
-        const double2 t = reinterpret_cast<const double2*>(T)[p];
+    const long stride = (long)gridDim.x * kThreads;
+    const double2* T2 = reinterpret_cast<const double2*>(T);
+    long p = (long)blockIdx.x * kThreads + threadIdx.x;
+    double2 tn = p < np ? T2[p] : make_double2(0.0, 0.0);
+    double2 tnn = p + stride < np ? T2[p + stride] : make_double2(0.0, 0.0);
+    for (; p < np; p += stride) {
+        // the next two pairs' loads are in flight during this pair's evaluation
+        const double2 t = tn;
+        tn = tnn;
+        if (p + 2 * stride < np) tnn = T2[p + 2 * stride];
         const double te0 = wall_te(2 * p, t.x), te1 = wall_te(2 * p + 1, t.y);
         double o0[7], o1[7];
         eval(te0, o0);
