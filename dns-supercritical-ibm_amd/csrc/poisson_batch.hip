@@ -931,10 +931,7 @@ int dnsp_batch_run(dnsp_batch* b, void* stream) {
 int dnsp_batch_run_persistent(dnsp_batch* b, void* stream) {
     clear_error();
     int rc = check_bound(b);
-    if (!rc) {
-        DNSP_HIP_TRY(hipSetDevice(b->device));
-        rc = persist_setup(b);
-    }
+    if (!rc) rc = persist_setup(b);  // on the caller's current device, as dnsp_batch_run
     if (rc) return rc;
     const BInst* inst = b->d_inst;
     const PSeg* seg = b->d_pseg;
