@@ -183,6 +183,18 @@ def test_batch_errors(cuda):
         b.bind(f, f)  # rhs and P must differ
     with pytest.raises(D.DnspError):
         b.run_mixed(1)  # FP64 batch
+    with pytest.raises(D.DnspError):
+        b.run_persistent()  # nothing bound
+    b.close()
+    # the persistent launch holds at most 12 rows per chunk (ny <= 384)
+    yu = W.tanh_faces(400, 1.5)
+    tall = dict(nx=32, nz=32, ny=400, inv_dx=32 / W.LX, inv_dz=32 / W.LZ,
+                bands=D.mk_P_abc(yu, False) + D.mk_P_abc(yu, True), rhs=W.rhs_uniform((400, 32, 32), 9, yu))
+    b = D.PoissonBatch([tall])
+    bind(b, [tall])
+    with pytest.raises(D.DnspError, match="rows per chunk"):
+        b.run_persistent()
+    b.run()  # the per-size launches take it
     b.close()
 
 
