@@ -8,7 +8,8 @@
 // launches per FP64 solve instead of 240.  The FP64 form and the
 // mixed-precision form (sweep 0 and one FP64 residual refinement) are
 // provided; the launch sequence allocates nothing and can be captured in a
-// graph.
+// graph.  The FP64 solve also runs as one persistent cooperative launch
+// (kb_persist) over all stages and instances.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
