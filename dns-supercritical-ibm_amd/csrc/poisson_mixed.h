@@ -282,7 +282,11 @@ __device__ __forceinline__ void xfwd_mr_staged(const TP* __restrict__ p, const d
     const int k0 = (int)(l0 / nz);
     const int j0 = (int)(l0 - (long)k0 * nz);
     const long pl = (long)k0 * nz;
-    // loads first (clamped, unconditional), LDS writes after them
+    // Loads first (clamped, unconditional), LDS writes after them.  Every
+    // address is a workgroup-uniform base plus a 32-bit offset (a plane of
+    // nz lines of N values is far below 2^32 elements), so the loads take a
+    // scalar base and no 64-bit address arithmetic per element.
+    const TP* pk = p + pl * N;  // plane k0
     P2 sv[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
@@ -292,23 +296,25 @@ __device__ __forceinline__ void xfwd_mr_staged(const TP* __restrict__ p, const d
         const int t = 2 * e - u * N;
         int jj = j0 - 1 + u;
         jj = jj < 0 ? jj + nz : (jj >= nz ? jj - nz : jj);
-        sv[q] = *reinterpret_cast<const P2*>(p + (pl + jj) * N + t);
+        sv[q] = *reinterpret_cast<const P2*>(reinterpret_cast<const char*>(pk) +
+                                             (unsigned)(jj * N + t) * (unsigned)sizeof(TP));
     }
     const long dn = k0 > 0 ? -(long)nz : 0;
     const long up = k0 < ny - 1 ? (long)nz : 0;
+    // line l0 + r, value t0 is element 2 ee of the tile's lines (r N + t0 = 2 ee)
+    const TP* pd = p + (l0 + dn) * N;
+    const TP* pu = p + (l0 + up) * N;
+    const double* pf = f + l0 * N;
     P2 yd[NL], yu[NL];
     double2 fv[NL];
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
         const int e = (int)threadIdx.x + q * NT;
         const bool in_tile = (G * N % NT == 0 || e < G * N);
-        const int ee = in_tile ? e : 0;
-        const int r = (2 * ee) / N;
-        const int t0 = 2 * ee - r * N;
-        const long line = l0 + r;
-        yd[q] = *reinterpret_cast<const P2*>(p + (line + dn) * N + t0);
-        yu[q] = *reinterpret_cast<const P2*>(p + (line + up) * N + t0);
-        fv[q] = *reinterpret_cast<const double2*>(f + line * N + t0);
+        const unsigned o2 = 2u * (unsigned)(in_tile ? e : 0);  // byte offsets below
+        yd[q] = *reinterpret_cast<const P2*>(reinterpret_cast<const char*>(pd) + o2 * (unsigned)sizeof(TP));
+        yu[q] = *reinterpret_cast<const P2*>(reinterpret_cast<const char*>(pu) + o2 * (unsigned)sizeof(TP));
+        fv[q] = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(pf) + o2 * 8u);
     }
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
