@@ -225,6 +225,16 @@ def test_config3_batch(cuda):
         assert torch.equal(Q, Pi)
         assert s.residual(fi, Pi) <= 1e-10
         s.close()
+    say("config3 persistent")
+    # one persistent launch equals the per-size launches with its variants
+    b.set_variants(2, 2, 2, *v[3:])
+    b.run()
+    ref = [Pi.clone() for Pi in P]
+    for Pi in P:
+        Pi.zero_()
+    b.run_persistent()
+    for c, Pi, r in zip(cases, P, ref):
+        assert torch.equal(Pi, r), (c["nx"], c["nz"], c["ny"])
     say("config3 mixed")
     b.run_mixed(1)
     for c, fi, Pi in zip(cases, f, P):
