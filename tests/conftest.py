@@ -19,5 +19,5 @@ def pytest_configure(config):
 def cuda():
     import torch
     if not torch.cuda.is_available():
-        pytest.fail("no HIP device visible: GPU tests must run on the GPU box")
+        pytest.fail("no HIP device visible: GPU tests need an MI355X")
     return torch.device("cuda:0")

@@ -6,7 +6,7 @@ Tolerances:
     contraction); the partitioned kernel within 1e-11 of each mode's max-abs;
   * transform stages: 1e-13 relative to the max-abs of the stage output;
   * whole solve: mean-removed max-abs error <= 1e-12 relative to the oracle's
-    mean-removed max-abs and <= 1e-10 absolute on the
+    mean-removed max-abs (the FP64 form's parity bar) and <= 1e-10 absolute on the
     reference-style random trials (main_poi_test.f90:259-262);
   * full size (768 x 384 x 384): discrete-operator residual <= 1e-10
     relative, forward/backward round trip, and agreement with the oracle.
