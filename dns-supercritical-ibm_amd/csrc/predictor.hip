@@ -517,7 +517,7 @@ __device__ __forceinline__ double seg_eval(const double* seg, int sgi, double T)
 // image.  Each workgroup stages the cubic laws' segments (nseg of them) and
 // bucket tables in LDS once and strides over the points, so the interval
 // searches read LDS.
-constexpr int kMaxSegments = 1024;  // 48 KB of LDS, plus at most 37 KB of merged tables (6 KB of bucket tables)
+constexpr int kMaxSegments = 1024;  // 48 KB of LDS, plus 6 KB of bucket tables (merged tables: up to 64 KB in all)
 
 // IDX32: the padded field has fewer than 2^31 points (32-bit index split).
 template <bool ALL_CUBIC, bool IDX32>
@@ -923,6 +923,11 @@ int build_merged(dnsp_predictor* pr) {
     std::sort(K.begin(), K.end());
     K.erase(std::unique(K.begin(), K.end()), K.end());
     const int m = (int)K.size();
+    // the kernel stages the segments and these tables in LDS: above 64 KB it
+    // keeps the per-law search (at most 54 KB)
+    const size_t lds = sizeof(double) * pr->h_seg.size() + sizeof(double) * m +
+                       sizeof(int) * ((size_t)6 * (m + 1) + kMergedBuckets);
+    if (lds > 64 * 1024) return DNSP_OK;
     std::vector<int> ms((size_t)6 * (m + 1)), mt(kMergedBuckets);
     for (int q = 0; q < Q_N; ++q) {
         const Law& L = pr->ps.law[q];

@@ -42,6 +42,23 @@ def wan():
     return laws, {"divjac": ("formula",), "buoy": ("linear", *d["buoyancy"])}
 
 
+def many_segments(per_law=170):
+    """Six cubic laws of per_law segments each over T in 290..330, bounds
+    offset per law so that the merged knots number about 6 per_law: the
+    property kernel's merged tables then exceed its 64 KB and it keeps the
+    per-law search."""
+    base = {"enth": 1.0, "rho": 1.0, "mu": 0.01, "cond": 0.01, "cp": 1.2, "drho_dT": -0.01}
+    step = 40.0 / per_law
+    laws = {}
+    for q, (name, c0) in enumerate(base.items()):
+        segs = []
+        for t in range(per_law):
+            hi = 290.0 + (t + 1 + 0.137 * q) * step
+            segs.append((hi, hi - 0.5 * step, c0 * (1.0 + 0.05 * np.sin(t + q)), 0.001 * c0, 1e-4 * c0, 1e-6 * c0))
+        laws[name] = ("cubic", segs)
+    return laws, {"divjac": ("formula",), "buoy": ("linear", -1.5, 1.0)}
+
+
 def nicoud():
     laws = {"enth": ("power", 1.0, 1.0), "rho": ("power", 1.0, -1.0), "mu": ("power", 1.0 / 180.0, -0.5),
             "cond": ("power", 1.0 / (180.0 * 0.76), -0.5), "cp": ("const", 1.0), "drho_dT": ("power", -1.0, -2.0)}
@@ -54,7 +71,7 @@ def make_case(nx, nz, ny, seed, kind):
     yu = np.array(W.tanh_faces(ny, 1.7))
     shp = (ny + 2, nz + 2, nx + 2)
     ish = (ny, nz, nx)
-    if kind == "wan":
+    if kind in ("wan", "many"):
         T0, dT = 307.0, 8.0
     else:
         T0, dT = 1.5, 0.4
@@ -71,6 +88,8 @@ def make_case(nx, nz, ny, seed, kind):
     laws, derived = (None, None)
     if kind == "wan":
         laws, derived = wan()
+    elif kind == "many":
+        laws, derived = many_segments()
     elif kind == "nicoud":
         laws, derived = nicoud()
     return dict(nx=nx, nz=nz, ny=ny, yu=yu, f=f, pf=pf, p=p, laws=laws, derived=derived,
@@ -134,7 +153,7 @@ def compare(s, pr, ref, c):
         assert np.array_equal(got, ref[k]), (k, np.abs(got - ref[k]).max())
 
 
-@pytest.mark.parametrize("kind", ["incomp", "nicoud", "wan"])
+@pytest.mark.parametrize("kind", ["incomp", "nicoud", "wan", "many"])
 @pytest.mark.parametrize("nx,nz,ny", [(24, 16, 20), (33, 7, 5)])
 def test_predictor_matches_oracle(cuda, kind, nx, nz, ny):
     say("predictor", kind, nx, nz, ny)
