@@ -58,28 +58,19 @@ struct BGroup {
     long total = 0;
     int* ids = nullptr;     // device, cnt
     long* start = nullptr;  // device, cnt
+    int* slot = nullptr;    // device, total: the instance slot u of every task (start[u] <= task < start[u + 1])
     // tridiagonal groups of the mixed form: (instance slot u, mode group)
     // pairs solved in FP64 ([0]) and in FP32 ([1]) arithmetic
     int2* tl[2] = {nullptr, nullptr};
     long ntl[2] = {0, 0};
 };
 
-__device__ __forceinline__ int find_inst(const long* __restrict__ start, int cnt, long task) {
-    int lo = 0, hi = cnt - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= task) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 enum BxKind { BX_F64_FWD = 0, BX_F64_BWD, BX_MX_FWD, BX_MX_RES, BX_MX_BWD, BX_MX_ACC, BX_MX_OUT0 };
 
 template <int N, int V, int KIND>
 __global__ __launch_bounds__(ct_xvar_threads(V)) void kb_x(const BInst* __restrict__ inst,
                                                           const int* __restrict__ ids,
-                                                          const long* __restrict__ start, int cnt, int sums) {
+                                                          const long* __restrict__ start, const int* __restrict__ tslot, int sums) {
     constexpr int G = ct_lines<N, ct_xvar_tile(V)>();
     constexpr int NT = ct_xvar_threads(V);
     constexpr bool F64 = KIND == BX_F64_FWD || KIND == BX_F64_BWD;
@@ -88,7 +79,7 @@ __global__ __launch_bounds__(ct_xvar_threads(V)) void kb_x(const BInst* __restri
     __shared__ __align__(16) VT sm[KIND == BX_MX_RES ? mr_lds_f2<N, G, float>() : G * ct_pitch(N)];
     __shared__ double slot[F64 ? 1 : line_slots<N, G>()];
     const long task = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    const int u = find_inst(start, cnt, task);
+    const int u = tslot[task];
     const BInst& b = inst[ids[u]];
     const long tile = task - start[u];
     if constexpr (KIND == BX_F64_FWD) {
@@ -126,7 +117,7 @@ enum BzKind { BZ_F64_FWD = 0, BZ_F64_BWD, BZ_A_FWD, BZ_A_BWD, BZ_B_FWD, BZ_B_BWD
 template <int N, int V, int KIND>
 __global__ __launch_bounds__(ct_zvar_threads(V)) void kb_z(const BInst* __restrict__ inst,
                                                           const int* __restrict__ ids,
-                                                          const long* __restrict__ start, int cnt) {
+                                                          const long* __restrict__ start, const int* __restrict__ tslot) {
     constexpr int TA = ct_cols<N, ct_zvar_tile(V)>();
     constexpr int NT = ct_zvar_threads(V);
     constexpr bool F64 = KIND == BZ_F64_FWD || KIND == BZ_F64_BWD;
@@ -134,7 +125,7 @@ __global__ __launch_bounds__(ct_zvar_threads(V)) void kb_z(const BInst* __restri
     using VT = typename Cx<T>::V;
     __shared__ VT sm[TA * ct_pitch(N)];
     const long task = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    const int u = find_inst(start, cnt, task);
+    const int u = tslot[task];
     const BInst& b = inst[ids[u]];
     const int tile = (int)(task - start[u]);
     const int ntc = ((b.nx + 1) / 2 + TA - 1) / TA;
@@ -154,14 +145,14 @@ constexpr int kBatchTriVariants = 4;  // {registers uncapped, capped for 4 waves
 template <int MR, int KIND, int OCC, bool TP>
 __global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri(const BInst* __restrict__ inst,
                                                                       const int* __restrict__ ids,
-                                                                      const long* __restrict__ start, int cnt) {
+                                                                      const long* __restrict__ start, const int* __restrict__ tslot) {
     constexpr int M = kBatchM;
     constexpr int P = kPartChunks;
     using T = typename std::conditional<KIND == 0, double, float>::type;
     __shared__ TriRed<P, M> red;
     extern __shared__ double smb[];
     const long task = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    const int u = find_inst(start, cnt, task);
+    const int u = tslot[task];
     const BInst& b = inst[ids[u]];
     const long grp = task - start[u];
     const int mi = (int)threadIdx.x % M;
@@ -206,9 +197,9 @@ __global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri_list(const 
 // q(k) of the first refinement for every plane of every instance (one
 // wavefront per plane; see k_mixed_q).
 __global__ __launch_bounds__(64) void kb_q(const BInst* __restrict__ inst, const int* __restrict__ ids,
-                                           const long* __restrict__ start, int cnt) {
+                                           const long* __restrict__ start, const int* __restrict__ tslot) {
     const long task = blockIdx.x;
-    const int u = find_inst(start, cnt, task);
+    const int u = tslot[task];
     const BInst& b = inst[ids[u]];
     const int k = (int)(task - start[u]);
     const int ny = b.ny, nz = b.nz;
@@ -251,13 +242,13 @@ static void bx_launch_nv(int kind, const BInst* inst, const BGroup& g, int sums,
     const int nb = (int)g.total;
     constexpr int NT = ct_xvar_threads(V);
     switch (kind) {
-        case BX_F64_FWD: kb_x<N, V, BX_F64_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
-        case BX_F64_BWD: kb_x<N, V, BX_F64_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
-        case BX_MX_FWD: kb_x<N, V, BX_MX_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
-        case BX_MX_RES: kb_x<N, V, BX_MX_RES><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
-        case BX_MX_BWD: kb_x<N, V, BX_MX_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
-        case BX_MX_ACC: kb_x<N, V, BX_MX_ACC><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
-        default: kb_x<N, V, BX_MX_OUT0><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt, sums); break;
+        case BX_F64_FWD: kb_x<N, V, BX_F64_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.slot, sums); break;
+        case BX_F64_BWD: kb_x<N, V, BX_F64_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.slot, sums); break;
+        case BX_MX_FWD: kb_x<N, V, BX_MX_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.slot, sums); break;
+        case BX_MX_RES: kb_x<N, V, BX_MX_RES><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.slot, sums); break;
+        case BX_MX_BWD: kb_x<N, V, BX_MX_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.slot, sums); break;
+        case BX_MX_ACC: kb_x<N, V, BX_MX_ACC><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.slot, sums); break;
+        default: kb_x<N, V, BX_MX_OUT0><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.slot, sums); break;
     }
 }
 
@@ -286,12 +277,12 @@ static void bz_launch_nv(int kind, const BInst* inst, const BGroup& g, hipStream
     const int nb = (int)g.total;
     constexpr int NT = ct_zvar_threads(V);
     switch (kind) {
-        case BZ_F64_FWD: kb_z<N, V, BZ_F64_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
-        case BZ_F64_BWD: kb_z<N, V, BZ_F64_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
-        case BZ_A_FWD: kb_z<N, V, BZ_A_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
-        case BZ_A_BWD: kb_z<N, V, BZ_A_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
-        case BZ_B_FWD: kb_z<N, V, BZ_B_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
-        default: kb_z<N, V, BZ_B_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.cnt); break;
+        case BZ_F64_FWD: kb_z<N, V, BZ_F64_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.slot); break;
+        case BZ_F64_BWD: kb_z<N, V, BZ_F64_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.slot); break;
+        case BZ_A_FWD: kb_z<N, V, BZ_A_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.slot); break;
+        case BZ_A_BWD: kb_z<N, V, BZ_A_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.slot); break;
+        case BZ_B_FWD: kb_z<N, V, BZ_B_FWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.slot); break;
+        default: kb_z<N, V, BZ_B_BWD><<<nb, NT, 0, st>>>(inst, g.ids, g.start, g.slot); break;
     }
 }
 
@@ -321,7 +312,7 @@ static void bt_launch_m(int kind, const BInst* inst, const BGroup& g, hipStream_
     constexpr int NT = kBatchM * kPartChunks;
     constexpr size_t lds = 3 * sizeof(double) * MR * kPartChunks;
     if (kind == 0) {
-        kb_tri<MR, 0, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
+        kb_tri<MR, 0, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.slot);
         return;
     }
     if (g.tl[0]) {
@@ -337,8 +328,8 @@ static void bt_launch_m(int kind, const BInst* inst, const BGroup& g, hipStream_
         }
         return;
     }
-    if (kind == 1) kb_tri<MR, 1, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
-    else kb_tri<MR, 2, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.cnt);
+    if (kind == 1) kb_tri<MR, 1, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.slot);
+    else kb_tri<MR, 2, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.slot);
 }
 
 static void bt_launch(int kind, int v, const BInst* inst, const BGroup& g, hipStream_t st) {
@@ -384,6 +375,7 @@ struct PSeg {
     int key, cnt;
     const int* ids;
     const long* start;
+    const int* slot;
 };
 struct PPlan {
     int s0[6];      // stage s's groups: seg[s0[s] .. s0[s + 1])
@@ -481,7 +473,7 @@ __global__ __launch_bounds__(kPersistThreads) void kb_persist(const BInst* __res
             while (q + 1 < plan.s0[s + 1] && seg[q + 1].first <= t) ++q;
             const PSeg& g = seg[q];
             const long task = t - g.first;
-            const int u = find_inst(g.start, g.cnt, task);
+            const int u = g.slot[task];
             const BInst& b = inst[g.ids[u]];
             const long tile = task - g.start[u];
             switch (s) {
@@ -602,8 +594,16 @@ int make_groups(dnsp_batch* b, const std::vector<int>& key, const std::vector<lo
             set_error("batch launch group too large (%ld tasks)", tot);
             return DNSP_ERR_ARG;
         }
+        // the slot of every task, one load in the kernels (a search over
+        // start[] costs a chain of dependent loads per workgroup)
+        std::vector<int> sl((size_t)tot);
+        for (int u = 0; u < g.cnt; ++u) {
+            const long end = u + 1 < g.cnt ? st[(size_t)u + 1] : tot;
+            for (long t = st[(size_t)u]; t < end; ++t) sl[(size_t)t] = u;
+        }
         int rc = dev_copy(b, &g.ids, kv.second.data(), kv.second.size());
         if (!rc) rc = dev_copy(b, &g.start, st.data(), st.size());
+        if (!rc && tot > 0) rc = dev_copy(b, &g.slot, sl.data(), sl.size());
         if (rc) return rc;
         out.push_back(g);
     }
@@ -793,7 +793,7 @@ void run_mixed(dnsp_batch* b, int refinements, int xv, int zv, int tv, hipStream
         return;
     }
     for (const BGroup& g : b->gx[xv]) bx_launch(BX_MX_BWD, xv, b->d_inst, g, 0, st);
-    if (b->gq.total > 0) kb_q<<<(int)b->gq.total, 64, 0, st>>>(b->d_inst, b->gq.ids, b->gq.start, b->gq.cnt);
+    if (b->gq.total > 0) kb_q<<<(int)b->gq.total, 64, 0, st>>>(b->d_inst, b->gq.ids, b->gq.start, b->gq.slot);
     for (const BGroup& g : b->gx[xv]) bx_launch(BX_MX_RES, xv, b->d_inst, g, 0, st);
     for (const BGroup& g : b->gz[zv]) bz_launch(BZ_B_FWD, zv, b->d_inst, g, st);
     for (const BGroup& g : b->gt) bt_launch(2, tv, b->d_inst, g, st);
@@ -819,7 +819,7 @@ int persist_setup(dnsp_batch* b) {
                           kPersistMaxRows * kPartChunks);
                 return DNSP_ERR_UNSUPPORTED;
             }
-            segs.push_back(PSeg{first, g.key, g.cnt, g.ids, g.start});
+            segs.push_back(PSeg{first, g.key, g.cnt, g.ids, g.start, g.slot});
             first += g.total;
             lds = std::max(lds, persist_lds(s, g.key));
         }
