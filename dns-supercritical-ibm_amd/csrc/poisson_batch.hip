@@ -2,7 +2,7 @@
 //
 // Every stage of the solve is one launch per distinct transform length (x, z)
 // or rows-per-chunk (tridiagonal) covering the tiles of all instances of that
-// size: a workgroup decodes its task through a prefix table (instance, tile)
+// size: a workgroup decodes its task through a slot table (instance, tile)
 // and runs the same tile bodies as the single-instance kernels.  A batch of
 // 48 instances over 5 x lengths, 5 z lengths and 4 row counts thus takes 24
 // launches per FP64 solve instead of 240.  The FP64 form and the
