@@ -996,6 +996,17 @@ __device__ __forceinline__ long tri_group_mode(long grp, long ngroups, int M, in
     return grp * M + mi;
 }
 
+// The same groups for the mode-major blocks of the blocked solve (M = 16, nx
+// a multiple of 16), numbered z index first: group g in between holds x
+// block g / nz of z index g % nz, so that consecutive groups (which run side
+// by side on one XCD) read adjacent blocks and store adjacent 128-byte pieces.
+__device__ __forceinline__ long tri_group_mode_mm(long grp, long ngroups, int mi, int nx, int nz) {
+    if (grp == 0 || grp == ngroups) return tri_group_mode(grp, ngroups, 16, mi);
+    const unsigned g = (unsigned)grp;
+    const unsigned ab = g / (unsigned)nz, b = g - ab * (unsigned)nz;
+    return (long)ab * 16 + mi + (long)nx * b;
+}
+
 // Grid: tri_groups(plane, M) workgroups of M * P threads, 3 * MR * P values
 // of type Ar of dynamic LDS.  With glist, workgroup b solves group
 // glist[b] (gridDim.x entries): the mixed form launches its FP32-arithmetic
@@ -1019,7 +1030,8 @@ __global__ __launch_bounds__(M * P, OCC) void kt_tri_part(T* S, const double* __
     // its own L2; workgroups reach XCDs round-robin)
     const int slot = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     const long grp = glist ? (long)glist[slot] : (long)slot;
-    const long mode = tri_group_mode(grp, (plane + M - 1) / M, M, mi);
+    const long mode = MM && M == 16 ? tri_group_mode_mm(grp, plane / 16, mi, nx, nz)
+                                    : tri_group_mode(grp, (plane + M - 1) / M, M, mi);
     Ar d[MR];
     const double lam64 = tri_prologue<MR, P, M, T, Ar, MM>(d, smb, bands + (grp == 0 ? 3 * ny : 0), S, ax, az, mode,
                                                            plane, nx, ny, msum, nz);
