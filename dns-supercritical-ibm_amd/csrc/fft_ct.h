@@ -99,6 +99,21 @@ __device__ __forceinline__ void ct_stages(V* buf, const V* __restrict__ tw) {
     }
 }
 
+// Copies the N twiddles exp(-2 pi i t/N) to LDS (stw, N entries); the
+// caller's next __syncthreads publishes them.  A stage gathers tw[r k tstep]
+// at up to 64 different k per wavefront: from global memory that touches
+// several times as many cache lines as the tile's own data, from LDS it costs
+// bank cycles only.  Same values, so the transform's results are unchanged.
+template <int N, int NT, typename V>
+__device__ __forceinline__ void ct_stage_twiddles(V* stw, const V* __restrict__ tw) {
+    constexpr int NQ = (N + NT - 1) / NT;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int i = (int)threadIdx.x + q * NT;
+        if (N % NT == 0 || i < N) stw[i] = tw[i];
+    }
+}
+
 // Transforms G padded lines (pitch ct_pitch(N)) in LDS.  All NT threads call it.
 // V is double2 or float2; tw holds exp(-2 pi i t/N) rounded to that type.
 template <int N, int G, int NT, bool INV, typename V>

@@ -124,11 +124,16 @@ __device__ __forceinline__ void xfwd_body(const double* P, double* S, long nline
     }
 }
 
+// The forward x wrappers stage the twiddles in LDS (ct_stage_twiddles);
+// xfwd_body's barrier before the transform publishes them (at 768x384x384:
+// 0.370 -> 0.322 ms for the x-blocked kernel, G = 2).
 template <int N, int G, int NT>
 __global__ __launch_bounds__(NT) void kt_xfwd(const double* P, double* S, long nlines,
                                               const double2* __restrict__ tw) {
     __shared__ double2 sm[G * ct_pitch(N)];
-    xfwd_body<N, G, NT>(P, S, nlines, tw, xcd_tile(blockIdx.x, gridDim.x), sm);
+    __shared__ double2 stw[N];
+    ct_stage_twiddles<N, NT>(stw, tw);
+    xfwd_body<N, G, NT>(P, S, nlines, stw, xcd_tile(blockIdx.x, gridDim.x), sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -237,7 +242,9 @@ template <int N, int G, int NT>
 __global__ __launch_bounds__(NT) void kt_xbwd(const double* S, double* P, long nlines,
                                               const double2* __restrict__ tw, double scale) {
     __shared__ double2 sm[G * ct_pitch(N)];
-    xbwd_body<N, G, NT>(S, P, nlines, tw, scale, xcd_tile(blockIdx.x, gridDim.x), sm);
+    __shared__ double2 stw[N];
+    ct_stage_twiddles<N, NT>(stw, tw);
+    xbwd_body<N, G, NT>(S, P, nlines, stw, scale, xcd_tile(blockIdx.x, gridDim.x), sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -358,7 +365,9 @@ __global__ __launch_bounds__(NT) void kt_zfwd(T* S, int nx, int ncol, int ntc,
                                               const double* __restrict__ lsum = nullptr,
                                               double* __restrict__ msum = nullptr) {
     __shared__ typename Cx<T>::V sm[TA * ct_pitch(N)];
-    zfwd_body<N, TA, NT, T>(S, nx, ntc, tw, lsum, msum, xcd_tile(blockIdx.x, gridDim.x), sm);
+    __shared__ typename Cx<T>::V stw[N];
+    ct_stage_twiddles<N, NT>(stw, tw);
+    zfwd_body<N, TA, NT, T>(S, nx, ntc, stw, lsum, msum, xcd_tile(blockIdx.x, gridDim.x), sm);
 }
 
 template <int N, int TA, int NT, typename T>
@@ -467,7 +476,9 @@ template <int N, int TA, int NT, typename T = double>
 __global__ __launch_bounds__(NT) void kt_zbwd(T* S, int nx, int ncol, int ntc,
                                               const typename Cx<T>::V* __restrict__ tw) {
     __shared__ typename Cx<T>::V sm[TA * ct_pitch(N)];
-    zbwd_body<N, TA, NT, T>(S, nx, ntc, tw, xcd_tile(blockIdx.x, gridDim.x), sm);
+    __shared__ typename Cx<T>::V stw[N];
+    ct_stage_twiddles<N, NT>(stw, tw);
+    zbwd_body<N, TA, NT, T>(S, nx, ntc, stw, xcd_tile(blockIdx.x, gridDim.x), sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -484,11 +495,20 @@ __global__ __launch_bounds__(NT) void kt_zbwd(T* S, int nx, int ncol, int ntc,
 // block of ny * 16 values.  Same transforms and arithmetic as the in-place
 // kernels above, only the addresses differ.
 // ---------------------------------------------------------------------------
+// Waves per SIMD that workgroups of nt threads holding lds bytes of LDS each
+// leave room for (160 KB of LDS per CU, four SIMDs), between 1 and cap: the
+// occupancy a launch bound asks the register allocator for.
+__host__ __device__ constexpr int lds_waves(int lds, int nt, int cap = 8) {
+    return (163840 / lds) * (nt / 64) / 4 < 1 ? 1
+         : ((163840 / lds) * (nt / 64) / 4 > cap ? cap : (163840 / lds) * (nt / 64) / 4);
+}
 template <int N, int G, int NT>
 __global__ __launch_bounds__(NT) void kt_xfwd_xb(const double* P, double* W, long nlines, int nz,
                                                  const double2* __restrict__ tw) {
     __shared__ double2 sm[G * ct_pitch(N)];
-    xfwd_body<N, G, NT, true>(P, W, nlines, tw, xcd_tile(blockIdx.x, gridDim.x), sm, nz);
+    __shared__ double2 stw[N];
+    ct_stage_twiddles<N, NT>(stw, tw);
+    xfwd_body<N, G, NT, true>(P, W, nlines, stw, xcd_tile(blockIdx.x, gridDim.x), sm, nz);
 }
 
 // z forward, XB -> MM.  Tile (k, c): the 16 x rows 16c..16c+15 of plane k as
@@ -501,6 +521,8 @@ __global__ __launch_bounds__(NT) void kt_zfwd_mm(const double* __restrict__ W, d
     constexpr int TOT2 = TA * N;
     constexpr int NL2 = (TOT2 + NT - 1) / NT;
     __shared__ double2 sm[TA * LP];
+    __shared__ double2 stw[N];
+    ct_stage_twiddles<N, NT>(stw, tw);
     const int nab = nx / 16;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int k = tile / nab;
@@ -519,7 +541,7 @@ __global__ __launch_bounds__(NT) void kt_zfwd_mm(const double* __restrict__ W, d
         if (TOT2 % NT == 0 || e < TOT2) sm[(e % TA) * LP + ct_pad(e / TA)] = v[q];
     }
     __syncthreads();
-    fft_ct<N, TA, NT, false>(sm, tw);
+    fft_ct<N, TA, NT, false>(sm, stw);
     double* dst = S + (size_t)c * N * ny * 16 + (size_t)k * 16;
 #pragma unroll
     for (int q = 0; q < NL2; ++q) {
@@ -550,6 +572,8 @@ __global__ __launch_bounds__(NT) void kt_zbwd_xl(const double* __restrict__ W, d
     constexpr int TOT2 = TA * N;
     constexpr int NL2 = (TOT2 + NT - 1) / NT;
     __shared__ double2 sm[TA * LP];
+    __shared__ double2 stw[N];
+    ct_stage_twiddles<N, NT>(stw, tw);
     const int nab = nx / 16;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int k = tile / nab;
@@ -580,7 +604,7 @@ __global__ __launch_bounds__(NT) void kt_zbwd_xl(const double* __restrict__ W, d
         }
     }
     __syncthreads();
-    fft_ct<N, TA, NT, true>(sm, tw);
+    fft_ct<N, TA, NT, true>(sm, stw);
     double* dst = P + (size_t)k * nx * N + (size_t)c * 16;
 #pragma unroll
     for (int q = 0; q < NL2; ++q) {

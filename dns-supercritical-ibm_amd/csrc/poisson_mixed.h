@@ -133,7 +133,9 @@ __global__ __launch_bounds__(NT) void kt_xfwd_mf(const double* __restrict__ f, f
                                                  const float2* __restrict__ tw) {
     __shared__ float2 sm[G * ct_pitch(N)];
     __shared__ double slot[line_slots<N, G>()];
-    xfwd_mf_body<N, G, NT>(f, S, lsum, nlines, tw, xcd_tile(blockIdx.x, gridDim.x), sm, slot);
+    __shared__ float2 stw[N];
+    ct_stage_twiddles<N, NT>(stw, tw);
+    xfwd_mf_body<N, G, NT>(f, S, lsum, nlines, stw, xcd_tile(blockIdx.x, gridDim.x), sm, slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -369,21 +371,24 @@ __device__ __forceinline__ void xfwd_mr_staged(const TP* __restrict__ p, const d
 }
 
 template <int N, int G, int NT, typename TP>
-__global__ __launch_bounds__(NT) void kt_xfwd_mr(const TP* __restrict__ p, const double* __restrict__ f,
+__global__ __launch_bounds__(NT, (lds_waves(mr_lds_f2<N, G, TP>() * 8 + line_slots<N, G>() * 8 + N * 8, NT, 5)))
+void kt_xfwd_mr(const TP* __restrict__ p, const double* __restrict__ f,
                                                  const double* __restrict__ qk, const double* __restrict__ bands,
                                                  float* __restrict__ S, double* __restrict__ lsum, int nz,
                                                  int ny, double ix2, double iz2, long nlines,
                                                  const float2* __restrict__ tw) {
     __shared__ __align__(16) float2 sm[mr_lds_f2<N, G, TP>()];
     __shared__ double slot[line_slots<N, G>()];
+    __shared__ float2 stw[N];
+    ct_stage_twiddles<N, NT>(stw, tw);
     const long tile = xcd_tile(blockIdx.x, gridDim.x);
     if constexpr (mr_staged<N, G, TP>()) {
         if (nz % (2 * G) == 0) {
-            xfwd_mr_staged<N, G, NT, TP>(p, f, qk, bands, S, lsum, nz, ny, ix2, iz2, nlines, tw, tile, sm, slot);
+            xfwd_mr_staged<N, G, NT, TP>(p, f, qk, bands, S, lsum, nz, ny, ix2, iz2, nlines, stw, tile, sm, slot);
             return;
         }
     }
-    xfwd_mr_body<N, G, NT, TP>(p, f, qk, bands, S, lsum, nz, ny, ix2, iz2, nlines, tw, tile, sm, slot);
+    xfwd_mr_body<N, G, NT, TP>(p, f, qk, bands, S, lsum, nz, ny, ix2, iz2, nlines, stw, tile, sm, slot);
 }
 
 // Halfcomplex pairs of lines -> LDS complex lines ready for the inverse FFT
@@ -518,7 +523,9 @@ __global__ __launch_bounds__(NT) void kt_xbwd_mf(const float* S, float* pt, doub
                                                  long nlines, const float2* __restrict__ tw, float scale) {
     __shared__ float2 sm[G * ct_pitch(N)];
     __shared__ double slot[line_slots<N, G>()];
-    xbwd_mf_body<N, G, NT>(S, pt, lsum, nlines, tw, scale, xcd_tile(blockIdx.x, gridDim.x), sm, slot);
+    __shared__ float2 stw[N];
+    ct_stage_twiddles<N, NT>(stw, tw);
+    xbwd_mf_body<N, G, NT>(S, pt, lsum, nlines, stw, scale, xcd_tile(blockIdx.x, gridDim.x), sm, slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -578,7 +585,9 @@ __global__ __launch_bounds__(NT) void kt_xbwd_ma(const float* __restrict__ S, co
                                                  const float2* __restrict__ tw, double scale) {
     __shared__ float2 sm[G * ct_pitch(N)];
     __shared__ double slot[line_slots<N, G>()];
-    xbwd_ma_body<N, G, NT, TP>(S, base, pbar0, pbar1, P, lsum, nz, nlines, tw, scale,
+    __shared__ float2 stw[N];
+    ct_stage_twiddles<N, NT>(stw, tw);
+    xbwd_ma_body<N, G, NT, TP>(S, base, pbar0, pbar1, P, lsum, nz, nlines, stw, scale,
                                xcd_tile(blockIdx.x, gridDim.x), sm, slot);
 }
 
