@@ -82,33 +82,38 @@ __global__ __launch_bounds__(ct_xvar_threads(V)) void kb_x(const BInst* __restri
     const int u = tslot[task];
     const BInst& b = inst[ids[u]];
     const long tile = task - start[u];
+    // the instance's twiddles in LDS (the bodies' barrier before the
+    // transform publishes them), as the single-solve kernels
+    __shared__ VT stw[N];
+    if constexpr (F64) ct_stage_twiddles<N, NT>(stw, b.twx);
+    else ct_stage_twiddles<N, NT>(stw, b.twx32);
     if constexpr (KIND == BX_F64_FWD) {
-        xfwd_body<N, G, NT>(b.f, b.P, b.nlines, b.twx, tile, sm);
+        xfwd_body<N, G, NT>(b.f, b.P, b.nlines, stw, tile, sm);
     } else if constexpr (KIND == BX_F64_BWD) {
-        xbwd_body<N, G, NT>(b.P, b.P, b.nlines, b.twx, b.inv_nxnz, tile, sm);
+        xbwd_body<N, G, NT>(b.P, b.P, b.nlines, stw, b.inv_nxnz, tile, sm);
     } else if constexpr (KIND == BX_MX_FWD) {
-        xfwd_mf_body<N, G, NT>(b.f, b.A, b.lsA, b.nlines, b.twx32, tile, sm, slot);
+        xfwd_mf_body<N, G, NT>(b.f, b.A, b.lsA, b.nlines, stw, tile, sm, slot);
     } else if constexpr (KIND == BX_MX_RES) {
         // same dispatch as kt_xfwd_mr (the instance's nz is workgroup-uniform)
         bool done = false;
         if constexpr (mr_staged<N, G, float>()) {
             if (b.nz % (2 * G) == 0) {
                 xfwd_mr_staged<N, G, NT, float>(b.A, b.f, b.small + 3 * b.ny, b.bands, b.B, b.lsB, b.nz, b.ny, b.ix2,
-                                                b.iz2, b.nlines, b.twx32, tile, sm, slot);
+                                                b.iz2, b.nlines, stw, tile, sm, slot);
                 done = true;
             }
         }
         if (!done)
             xfwd_mr_body<N, G, NT, float>(b.A, b.f, b.small + 3 * b.ny, b.bands, b.B, b.lsB, b.nz, b.ny, b.ix2,
-                                          b.iz2, b.nlines, b.twx32, tile, sm, slot);
+                                          b.iz2, b.nlines, stw, tile, sm, slot);
     } else if constexpr (KIND == BX_MX_BWD) {
-        xbwd_mf_body<N, G, NT>(b.A, b.A, b.lsA, b.nlines, b.twx32, (float)b.inv_nxnz, tile, sm, slot);
+        xbwd_mf_body<N, G, NT>(b.A, b.A, b.lsA, b.nlines, stw, (float)b.inv_nxnz, tile, sm, slot);
     } else if constexpr (KIND == BX_MX_ACC) {
         xbwd_ma_body<N, G, NT, float>(b.B, b.A, b.small + b.ny, b.small + 2 * b.ny, b.P, sums ? b.lsA : nullptr,
-                                      b.nz, b.nlines, b.twx32, b.inv_nxnz, tile, sm, slot);
+                                      b.nz, b.nlines, stw, b.inv_nxnz, tile, sm, slot);
     } else {  // BX_MX_OUT0: sweep 0 written straight to P (no refinement)
         xbwd_ma_body<N, G, NT, float>(b.A, nullptr, b.small + b.ny, nullptr, b.P, nullptr, b.nz, b.nlines,
-                                      b.twx32, b.inv_nxnz, tile, sm, slot);
+                                      stw, b.inv_nxnz, tile, sm, slot);
     }
 }
 
@@ -128,13 +133,16 @@ __global__ __launch_bounds__(ct_zvar_threads(V)) void kb_z(const BInst* __restri
     const int u = tslot[task];
     const BInst& b = inst[ids[u]];
     const int tile = (int)(task - start[u]);
+    __shared__ VT stw[N];
+    if constexpr (F64) ct_stage_twiddles<N, NT>(stw, b.twz);
+    else ct_stage_twiddles<N, NT>(stw, b.twz32);
     const int ntc = ((b.nx + 1) / 2 + TA - 1) / TA;
-    if constexpr (KIND == BZ_F64_FWD) zfwd_body<N, TA, NT, double>(b.P, b.nx, ntc, b.twz, nullptr, nullptr, tile, sm);
-    else if constexpr (KIND == BZ_F64_BWD) zbwd_body<N, TA, NT, double>(b.P, b.nx, ntc, b.twz, tile, sm);
-    else if constexpr (KIND == BZ_A_FWD) zfwd_body<N, TA, NT, float>(b.A, b.nx, ntc, b.twz32, b.lsA, b.small, tile, sm);
-    else if constexpr (KIND == BZ_A_BWD) zbwd_body<N, TA, NT, float>(b.A, b.nx, ntc, b.twz32, tile, sm);
-    else if constexpr (KIND == BZ_B_FWD) zfwd_body<N, TA, NT, float>(b.B, b.nx, ntc, b.twz32, b.lsB, b.small, tile, sm);
-    else zbwd_body<N, TA, NT, float>(b.B, b.nx, ntc, b.twz32, tile, sm);
+    if constexpr (KIND == BZ_F64_FWD) zfwd_body<N, TA, NT, double>(b.P, b.nx, ntc, stw, nullptr, nullptr, tile, sm);
+    else if constexpr (KIND == BZ_F64_BWD) zbwd_body<N, TA, NT, double>(b.P, b.nx, ntc, stw, tile, sm);
+    else if constexpr (KIND == BZ_A_FWD) zfwd_body<N, TA, NT, float>(b.A, b.nx, ntc, stw, b.lsA, b.small, tile, sm);
+    else if constexpr (KIND == BZ_A_BWD) zbwd_body<N, TA, NT, float>(b.A, b.nx, ntc, stw, tile, sm);
+    else if constexpr (KIND == BZ_B_FWD) zfwd_body<N, TA, NT, float>(b.B, b.nx, ntc, stw, b.lsB, b.small, tile, sm);
+    else zbwd_body<N, TA, NT, float>(b.B, b.nx, ntc, stw, tile, sm);
 }
 
 // Tridiagonal: one mode group of 8 modes per task.  KIND 0: FP64 on P;

@@ -124,9 +124,9 @@ __device__ __forceinline__ void xfwd_body(const double* P, double* S, long nline
     }
 }
 
-// The forward x wrappers stage the twiddles in LDS (ct_stage_twiddles);
-// xfwd_body's barrier before the transform publishes them (at 768x384x384:
-// 0.370 -> 0.322 ms for the x-blocked kernel, G = 2).
+// Every transform wrapper stages the twiddles in LDS (ct_stage_twiddles); the
+// body's barrier before its transform publishes them (at 768x384x384 the
+// x-blocked forward kernel went from 0.370 to 0.336 ms, G = 2).
 template <int N, int G, int NT>
 __global__ __launch_bounds__(NT) void kt_xfwd(const double* P, double* S, long nlines,
                                               const double2* __restrict__ tw) {
