@@ -639,36 +639,62 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
 
 // RHS_energy and div_now = divjac RHS_energy at interior points
 // (mod_cfd_subroutines.f90:651-694), zero in rough-wall columns (:697-716).
+// T and cond are read at seven points each: a workgroup stages them in LDS
+// (PlaneRing, stencil_grid.h) and marches through kRunK planes of its 64 x 4
+// tile, as k_momentum (at 384x256x192: 0.240 -> 0.230 ms; the per-thread
+// version served its 14 neighbour reads per point from L1 and L2).  Same
+// expressions on the same values.
 __global__ __launch_bounds__(kThreads) void k_rhs_energy(const double* __restrict__ T,
                                                          const double* __restrict__ cond,
                                                          const double* __restrict__ divjac,
                                                          double* __restrict__ RHS, double* __restrict__ dvn,
                                                          const int* __restrict__ bot, const int* __restrict__ top,
                                                          Grid g, BandGrid bg, Geom G, Phys p) {
-    int i, j, kk;
-    if (!band_point(bg, i, j, kk)) return;
-    const long c = g.at(i, j, kk);
-    const long sz = g.sz(), sy = g.sy();
-    const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * kk);
-    auto t = [&](int di, int dj, int dk) { return T[c + di + dj * sz + dk * sy]; };
-    auto q = [&](int di, int dj, int dk) { return cond[c + di + dj * sz + dk * sy]; };
-    const double Ra = G.ruw[3 * kk], Rb = G.ruw[3 * kk + 1], Rc = G.ruw[3 * kk + 2];
-    const double R2a = G.ruw2[3 * kk], R2b = G.ruw2[3 * kk + 1], R2c = G.ruw2[3 * kk + 2];
-    const double d2tx = ((t(1, 0, 0) - 2 * t(0, 0, 0)) + t(-1, 0, 0)) * p.idx2;
-    const double d2tz = ((t(0, 1, 0) - 2 * t(0, 0, 0)) + t(0, -1, 0)) * p.idz2;
-    const double d2ty = (t(0, 0, -1) * R2a + t(0, 0, 0) * R2b) + t(0, 0, 1) * R2c;
-    const double grad = (((q(1, 0, 0) - q(-1, 0, 0)) * p.idx__2) * ((t(1, 0, 0) - t(-1, 0, 0)) * p.idx__2) +
-                         ((q(0, 1, 0) - q(0, -1, 0)) * p.idz__2) * ((t(0, 1, 0) - t(0, -1, 0)) * p.idz__2)) +
-                        ((q(0, 0, -1) * Ra + q(0, 0, 0) * Rb) + q(0, 0, 1) * Rc) *
-                            ((t(0, 0, -1) * Ra + t(0, 0, 0) * Rb) + t(0, 0, 1) * Rc);
-    const double rhs = (q(0, 0, 0) * ((d2tx + d2tz) + d2ty) + grad) + p.Sq;
-    RHS[o] = rhs;
-    double dn = divjac[c] * rhs;
-    if (bot != nullptr) {
-        const long col = (long)i + (long)g.nx * j;
-        if (kk <= bot[col] || kk >= top[col]) dn = 0.0;
+    __shared__ double sF[2][3][kRingP];
+    int i0, j0, k0, k1;
+    if (!ring_tile(bg, g.ny, i0, j0, k0, k1)) return;
+    const int i = i0 + (int)threadIdx.x, j = j0 + (int)threadIdx.y;
+    const bool active = i < g.nx && j < g.nz;
+    const double* fields[2] = {T, cond};
+    PlaneRing<2> ring(sF, fields, i0, j0, g.nx, g.nz);
+    ring.fetch(k0 - 1);
+    ring.put(k0 - 1);
+    ring.fetch(k0);
+    ring.put(k0);
+    ring.fetch(k0 + 1);
+    const int e0 = ring_point();
+    for (int kk = k0; kk < k1; ++kk) {
+        ring.put(kk + 1);  // the slot of plane kk - 2, free since the last barrier
+        __syncthreads();
+        if (kk + 2 <= k1) ring.fetch(kk + 2);  // in flight during this step's arithmetic
+        if (active) {
+            const int sm = (kk + 2) % 3, s0 = kk % 3, sp = (kk + 1) % 3;
+            auto slot = [&](int dk) { return dk < 0 ? sm : (dk > 0 ? sp : s0); };
+            auto t = [&](int di, int dj, int dk) { return sF[0][slot(dk)][e0 + di + dj * kRingX]; };
+            auto q = [&](int di, int dj, int dk) { return sF[1][slot(dk)][e0 + di + dj * kRingX]; };
+            const long c = g.at(i, j, kk);
+            const long o = (long)i + (long)g.nx * ((long)j + (long)g.nz * kk);
+            const double Ra = G.ruw[3 * kk], Rb = G.ruw[3 * kk + 1], Rc = G.ruw[3 * kk + 2];
+            const double R2a = G.ruw2[3 * kk], R2b = G.ruw2[3 * kk + 1], R2c = G.ruw2[3 * kk + 2];
+            const double d2tx = ((t(1, 0, 0) - 2 * t(0, 0, 0)) + t(-1, 0, 0)) * p.idx2;
+            const double d2tz = ((t(0, 1, 0) - 2 * t(0, 0, 0)) + t(0, -1, 0)) * p.idz2;
+            const double d2ty = (t(0, 0, -1) * R2a + t(0, 0, 0) * R2b) + t(0, 0, 1) * R2c;
+            const double grad =
+                (((q(1, 0, 0) - q(-1, 0, 0)) * p.idx__2) * ((t(1, 0, 0) - t(-1, 0, 0)) * p.idx__2) +
+                 ((q(0, 1, 0) - q(0, -1, 0)) * p.idz__2) * ((t(0, 1, 0) - t(0, -1, 0)) * p.idz__2)) +
+                ((q(0, 0, -1) * Ra + q(0, 0, 0) * Rb) + q(0, 0, 1) * Rc) *
+                    ((t(0, 0, -1) * Ra + t(0, 0, 0) * Rb) + t(0, 0, 1) * Rc);
+            const double rhs = (q(0, 0, 0) * ((d2tx + d2tz) + d2ty) + grad) + p.Sq;
+            RHS[o] = rhs;
+            double dn = divjac[c] * rhs;
+            if (bot != nullptr) {
+                const long col = (long)i + (long)g.nx * j;
+                if (kk <= bot[col] || kk >= top[col]) dn = 0.0;
+            }
+            dvn[c] = dn;
+        }
+        __syncthreads();  // slot (kk + 2) % 3 is restaged next
     }
-    dvn[c] = dn;
 }
 
 }  // namespace
@@ -1210,8 +1236,8 @@ int dnsp_predictor_enqueue(dnsp_predictor* pr, void* stream) {
             if (!rc) rc = dnsp_set_val_slice(T, g.nx, g.nz, g.ny, -1, -1, wl[7], vs);
             if (rc) return rc;
         }
-        k_rhs_energy<<<grid3, blk3, 0, st>>>(T, pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_DIVJAC],
-                                                 pr->f[DNSP_PRED_RHS_ENERGY], dvn, pr->d_bot[3], pr->d_top[3], g, bg,
+        k_rhs_energy<<<gridr, blk3, 0, st>>>(T, pr->f[DNSP_PRED_COND], pr->f[DNSP_PRED_DIVJAC],
+                                                 pr->f[DNSP_PRED_RHS_ENERGY], dvn, pr->d_bot[3], pr->d_top[3], g, br,
                                                  pr->G, pr->p);
         k_div_finish<<<blocks_for(2 * g.sy() + per_plane * g.ny), kThreads, 0, st>>>(dvn, g);
     }
