@@ -176,26 +176,33 @@ __global__ void k_min_init(double* out) {
         *reinterpret_cast<unsigned long long*>(out) = 0xffffffffffffffffull;
 }
 
-// Workgroups stride over groups of kMinRows interior x rows (row -> (j, k)
-// by uniform divisions), the threads of a workgroup over the rows, with the
-// loads of all kMinRows rows issued before their keys are compared.
-constexpr int kMinRows = 4;
+// Workgroups stride over groups of kMinRows interior x rows; a group's
+// kMinRows * nx values are spread over all the workgroup's threads (whole
+// rows per thread left half the lanes idle at nx = 384), and each thread
+// issues kMinLoads loads before comparing their keys.
+constexpr int kMinRows = 8, kMinLoads = 4;
 __global__ void k_min_interior(const double* __restrict__ A, Halo h, double* out) {
     __shared__ unsigned long long red[kStencilThreads];
     const int rows = h.nz * h.ny;
+    const int per = kMinRows * h.nx;
     unsigned long long best = 0xffffffffffffffffull;
     for (int row0 = (int)blockIdx.x * kMinRows; row0 < rows; row0 += (int)gridDim.x * kMinRows) {
-        for (int i = (int)threadIdx.x; i < h.nx; i += kStencilThreads) {
-            double v[kMinRows];
+        for (int e0 = (int)threadIdx.x; e0 < per; e0 += kMinLoads * kStencilThreads) {
+            double v[kMinLoads];
 #pragma unroll
-            for (int r = 0; r < kMinRows; ++r) {
+            for (int u = 0; u < kMinLoads; ++u) {
+                const int e = e0 + u * kStencilThreads;
+                const int r = (int)((unsigned)e / (unsigned)h.nx);
+                const int i = e - r * h.nx;
                 const int row = row0 + r;
                 const int k = row / h.nz, j = row - k * h.nz;
-                v[r] = row < rows ? A[h.at(i, j, k)] : A[h.at(i, 0, 0)];  // past the last row: a repeat
+                // past the group or the last row: a repeat of an interior value
+                const long at = e < per && row < rows ? h.at(i, j, k) : h.at(0, 0, 0);
+                v[u] = A[at];
             }
 #pragma unroll
-            for (int r = 0; r < kMinRows; ++r) {
-                const unsigned long long key = dkey(v[r]);
+            for (int u = 0; u < kMinLoads; ++u) {
+                const unsigned long long key = dkey(v[u]);
                 best = key < best ? key : best;
             }
         }
