@@ -16,7 +16,7 @@
 // Quadratic mode (make_rough_surface_quady :133-189, apply_quady :191-210):
 // the first fluid point of every column is the target, its value the
 // quadratic through the wall and the next two fluid points.
-// The driver then repeats the records over cc_x x cc_z surface tiles,
+// The generator then repeats the records over cc_x x cc_z surface tiles,
 // sorts them by (k, j, i) and writes the records and the column slabs.
 #include <algorithm>
 #include <cmath>
