@@ -147,15 +147,20 @@ __global__ __launch_bounds__(ct_zvar_threads(V)) void kb_z(const BInst* __restri
 
 // Tridiagonal: one mode group of 8 modes per task.  KIND 0: FP64 on P;
 // 1: FP32 on A with the (0,0) mode -> pbar0; 2: FP32 on B -> pbar1.
+// Variants 0..3: P = kPartChunks chunks per mode, {registers uncapped,
+// capped for 4 waves} x {one-pass, two-pass}; 4 and 5: 16 chunks of twice
+// the rows (one reduction step fewer; the groups' rows per chunk at 32
+// chunks, doubled, are again a listed row count), one-pass and two-pass.
+// (8 chunks of four times the rows measured the same as 16 on the
+// 48-instance batch: not kept.)
 constexpr int kBatchM = 8;
-constexpr int kBatchTriVariants = 4;  // {registers uncapped, capped for 4 waves} x {one-pass, two-pass}
+constexpr int kBatchTriVariants = 6;
 
-template <int MR, int KIND, int OCC, bool TP>
-__global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri(const BInst* __restrict__ inst,
-                                                                      const int* __restrict__ ids,
-                                                                      const long* __restrict__ start, const int* __restrict__ tslot) {
+template <int MR, int KIND, int OCC, bool TP, int P = kPartChunks>
+__global__ __launch_bounds__(kBatchM * P, OCC) void kb_tri(const BInst* __restrict__ inst,
+                                                           const int* __restrict__ ids,
+                                                           const long* __restrict__ start, const int* __restrict__ tslot) {
     constexpr int M = kBatchM;
-    constexpr int P = kPartChunks;
     using T = typename std::conditional<KIND == 0, double, float>::type;
     __shared__ TriRed<P, M> red;
     extern __shared__ double smb[];
@@ -177,12 +182,11 @@ __global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri(const BInst
 
 // The mixed form's tridiagonal from a task list: workgroup b solves mode
 // group tl[b].y of instance slot tl[b].x, in arithmetic Ar.  KIND 1 or 2.
-template <int MR, int KIND, int OCC, bool TP, typename Ar>
-__global__ __launch_bounds__(kBatchM * kPartChunks, OCC) void kb_tri_list(const BInst* __restrict__ inst,
-                                                                           const int* __restrict__ ids,
-                                                                           const int2* __restrict__ tl) {
+template <int MR, int KIND, int OCC, bool TP, typename Ar, int P = kPartChunks>
+__global__ __launch_bounds__(kBatchM * P, OCC) void kb_tri_list(const BInst* __restrict__ inst,
+                                                                const int* __restrict__ ids,
+                                                                const int2* __restrict__ tl) {
     constexpr int M = kBatchM;
-    constexpr int P = kPartChunks;
     __shared__ TriRed<P, M, Ar> red;
     extern __shared__ double smb_raw[];
     Ar* smb = reinterpret_cast<Ar*>(smb_raw);
@@ -314,30 +318,37 @@ static void bz_launch(int kind, int v, const BInst* inst, const BGroup& g, hipSt
     }
 }
 
-template <int MR, int OCC, bool TP>
+template <int MR, int OCC, bool TP, int P = kPartChunks>
 static void bt_launch_m(int kind, const BInst* inst, const BGroup& g, hipStream_t st) {
     const int nb = (int)g.total;
-    constexpr int NT = kBatchM * kPartChunks;
-    constexpr size_t lds = 3 * sizeof(double) * MR * kPartChunks;
+    constexpr int NT = kBatchM * P;
+    constexpr size_t lds = 3 * sizeof(double) * MR * P;
     if (kind == 0) {
-        kb_tri<MR, 0, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.slot);
+        kb_tri<MR, 0, OCC, TP, P><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.slot);
         return;
     }
     if (g.tl[0]) {
         // the mixed form's split: FP64 groups, then FP32 ones
-        constexpr size_t lds32 = 3 * sizeof(float) * MR * kPartChunks;
+        constexpr size_t lds32 = 3 * sizeof(float) * MR * P;
         if (g.ntl[0] > 0) {
-            if (kind == 1) kb_tri_list<MR, 1, OCC, TP, double><<<(int)g.ntl[0], NT, lds, st>>>(inst, g.ids, g.tl[0]);
-            else kb_tri_list<MR, 2, OCC, TP, double><<<(int)g.ntl[0], NT, lds, st>>>(inst, g.ids, g.tl[0]);
+            if (kind == 1) kb_tri_list<MR, 1, OCC, TP, double, P><<<(int)g.ntl[0], NT, lds, st>>>(inst, g.ids, g.tl[0]);
+            else kb_tri_list<MR, 2, OCC, TP, double, P><<<(int)g.ntl[0], NT, lds, st>>>(inst, g.ids, g.tl[0]);
         }
         if (g.ntl[1] > 0) {
-            if (kind == 1) kb_tri_list<MR, 1, OCC, TP, float><<<(int)g.ntl[1], NT, lds32, st>>>(inst, g.ids, g.tl[1]);
-            else kb_tri_list<MR, 2, OCC, TP, float><<<(int)g.ntl[1], NT, lds32, st>>>(inst, g.ids, g.tl[1]);
+            if (kind == 1) kb_tri_list<MR, 1, OCC, TP, float, P><<<(int)g.ntl[1], NT, lds32, st>>>(inst, g.ids, g.tl[1]);
+            else kb_tri_list<MR, 2, OCC, TP, float, P><<<(int)g.ntl[1], NT, lds32, st>>>(inst, g.ids, g.tl[1]);
         }
         return;
     }
-    if (kind == 1) kb_tri<MR, 1, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.slot);
-    else kb_tri<MR, 2, OCC, TP><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.slot);
+    if (kind == 1) kb_tri<MR, 1, OCC, TP, P><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.slot);
+    else kb_tri<MR, 2, OCC, TP, P><<<nb, NT, lds, st>>>(inst, g.ids, g.start, g.slot);
+}
+
+// 16 chunks of 2 MR rows for groups keyed by MR rows at 32 chunks (MR <= 16)
+template <int MR, bool TP>
+static void bt_launch_16(int kind, const BInst* inst, const BGroup& g, hipStream_t st) {
+    if constexpr (2 * MR <= 32) bt_launch_m<2 * MR, 1, TP, 16>(kind, inst, g, st);
+    else bt_launch_m<MR, 1, TP>(kind, inst, g, st);
 }
 
 static void bt_launch(int kind, int v, const BInst* inst, const BGroup& g, hipStream_t st) {
@@ -348,7 +359,9 @@ static void bt_launch(int kind, int v, const BInst* inst, const BGroup& g, hipSt
         if (v == 0) bt_launch_m<MR, 1, false>(kind, inst, g, st);      \
         else if (v == 1) bt_launch_m<MR, 4, false>(kind, inst, g, st); \
         else if (v == 2) bt_launch_m<MR, 1, true>(kind, inst, g, st);  \
-        else bt_launch_m<MR, 4, true>(kind, inst, g, st);              \
+        else if (v == 3) bt_launch_m<MR, 4, true>(kind, inst, g, st);  \
+        else if (v == 4) bt_launch_16<MR, false>(kind, inst, g, st);   \
+        else bt_launch_16<MR, true>(kind, inst, g, st);                \
         break;
         DNSP_PART_ROWS(DNSP_CASE)
 #undef DNSP_CASE

@@ -462,10 +462,11 @@ class PoissonBatch:
     def autotune(self, reps=3, stream=None):
         times = np.zeros(TUNE_SLOTS)
         _check(lib().dnsp_batch_autotune(self._h, int(reps), _vp(_stream_ptr(stream)), times.ctypes.data_as(_dp)))
-        out = {"x_ms": times[:4].tolist(), "z_ms": times[4:8].tolist(), "tri_ms": times[8:12].tolist()}
+        # tridiagonal: variants 0..3 (32 chunks) and 4, 5 (16 chunks)
+        out = {"x_ms": times[:4].tolist(), "z_ms": times[4:8].tolist(), "tri_ms": times[8:14].tolist()}
         if self.precision == PRECISION_MIXED:
             out.update({"mixed_x_ms": times[16:20].tolist(), "mixed_z_ms": times[20:24].tolist(),
-                        "mixed_tri_ms": times[24:28].tolist()})
+                        "mixed_tri_ms": times[24:30].tolist()})
         out["picked"] = self.variants
         return out
 

@@ -261,10 +261,12 @@ int dnsp_poisson_mixed_info(const dnsp_poisson* h, int* fp32_on, double* f32_min
  *   Both enqueue on `stream` only (no allocation, no synchronisation), so a
  *   caller may capture them in a graph.
  *   dnsp_batch_autotune: times the whole batched solve per variant (x, z:
- *     0..3 as dnsp_poisson_set_variants; tridiagonal: 0 = 8-mode groups,
- *     1 = the same with registers capped for 4 waves per SIMD, 2 and 3 =
- *     0 and 1 with two-pass chunks), keeps the fastest; overwrites the bound P arrays; times_ms as for
- *     dnsp_poisson_autotune.  Synchronises.
+ *     0..3 as dnsp_poisson_set_variants; tridiagonal: 0 = 8-mode groups of
+ *     32 chunks, 1 = the same with registers capped for 4 waves per SIMD, 2
+ *     and 3 = 0 and 1 with two-pass chunks, 4 and 5 = 8-mode groups of 16
+ *     chunks of twice the rows, one-pass and two-pass (32 chunks where the
+ *     rows would exceed 32)), keeps the fastest; overwrites the bound P
+ *     arrays; times_ms as for dnsp_poisson_autotune.  Synchronises.
  *   dnsp_batch_set_variants / get_variants: FP64 x, z, tridiagonal, then the
  *     mixed form's (6 ints).
  */

@@ -93,6 +93,39 @@ def test_batch_fp64_equals_single_and_oracle(cuda, tvar):
     b.close()
 
 
+@pytest.mark.parametrize("tvar", [4, 5])
+def test_batch_16_chunk_tridiagonal_variants(cuda, tvar):
+    """16 chunks of twice the rows: no single-solve kernel has this shape, so
+    the FP64 results are checked against the oracle and against variant 0
+    (another partition of the rows, another rounding), both to 1e-12; the
+    mixed form with the same shape reaches its 1e-10 residual."""
+    say("batch 16 chunks", tvar)
+    cases = small_cases(1)
+    b = D.PoissonBatch(cases)
+    f, P = bind(b, cases)
+    b.set_variants(2, 1, 0)
+    b.run()
+    ref0 = [to_host(Pi) for Pi in P]
+    b.set_variants(2, 1, tvar)
+    b.run()
+    for c, Pi, r0 in zip(cases, P, ref0):
+        got = to_host(Pi)
+        assert rel_mean_removed(got, O.solve(c["rhs"], c["inv_dx"], c["inv_dz"], c["bands"])) <= 1e-12, c["ny"]
+        assert rel_mean_removed(got, r0) <= 1e-12, c["ny"]
+    b.close()
+    cases = small_cases(2)
+    b = D.PoissonBatch(cases, precision=D.PRECISION_MIXED)
+    f, P = bind(b, cases)
+    b.set_variants(0, 0, 1, 2, 1, tvar)
+    b.run_mixed(1)
+    for c, Pi in zip(cases, P):
+        got = to_host(Pi)
+        r = O.apply_operator(got, c["inv_dx"], c["inv_dz"], c["bands"]) - c["rhs"]
+        rel = np.linalg.norm(r) / np.linalg.norm(c["rhs"])
+        assert rel <= 1e-10, (c["nx"], c["nz"], c["ny"], rel)
+    b.close()
+
+
 def test_batch_persistent_equals_per_size_launches(cuda):
     """One cooperative launch over all instances and stages gives the results
     of the per-size launches with the same variants (x 2, z 2, two-pass
@@ -219,10 +252,16 @@ def test_config3_batch(cuda):
         s = D.PoissonSolver(c["nx"], c["nz"], c["ny"], c["inv_dx"], c["inv_dz"], c["bands"],
                             precision=D.PRECISION_MIXED)
         s.set_strategy(D.FFT_FIXED, D.TRIDIAG_PARTITION)
-        s.set_variants(v[0], v[1], 1 if v[2] < 2 else 5)  # 8-mode groups, one- or two-pass chunks
+        # 8-mode groups of 32 chunks, one- or two-pass; the 16-chunk batch
+        # variants (4, 5) have no single-solve kernel: there the same solution
+        # to 1e-12, another partition of the rows rounding differently
+        s.set_variants(v[0], v[1], 1 if v[2] in (0, 1, 4) else 5)
         Q = torch.zeros_like(fi)
         s.solve(fi, Q)
-        assert torch.equal(Q, Pi)
+        if v[2] < 4:
+            assert torch.equal(Q, Pi)
+        else:
+            assert rel_mean_removed(to_host(Pi), to_host(Q)) <= 1e-12, (c["nx"], c["nz"], c["ny"])
         assert s.residual(fi, Pi) <= 1e-10
         s.close()
     say("config3 persistent")
