@@ -20,7 +20,7 @@ int part_rows_for(int ny) { return rows_for(ny, kPartChunks); }
 int part_group_modes(int variant, int ny, bool fp32) {
     switch (variant) {
         case 1: case 5: return 8;
-        case 3: case 7: return fp32 ? (ny <= 16 * 32 ? 32 : 16) : 8;
+        case 3: case 7: return fp32 ? (ny <= 16 * 32 ? 32 : 16) : (ny <= 16 * 32 ? 16 : 8);
         default: return 16;
     }
 }
@@ -86,15 +86,24 @@ void part_launch(int variant, T* S, const double* ax, const double* az, const do
         }
         return;
     }
+    // FP64: 3 and 7 are (16, 16) for ny <= 512 (16 chunks of twice the
+    // rows, one reduction step fewer), else (64, 8)
+    const bool c16 = ny <= 16 * 32;
     switch (variant) {
         case 0: DNSP_PV(32, 16, 4, false); break;
         case 1: DNSP_PV(32, 8, 1, false); break;
         case 2: DNSP_PV(48, 16, 1, false); break;
-        case 3: DNSP_PV(64, 8, 1, false); break;
+        case 3:
+            if (c16) DNSP_PV(16, 16, 1, false);
+            else DNSP_PV(64, 8, 1, false);
+            break;
         case 4: DNSP_PV(32, 16, 4, true); break;
         case 5: DNSP_PV(32, 8, 1, true); break;
         case 6: DNSP_PV(48, 16, 1, true); break;
-        default: DNSP_PV(64, 8, 1, true); break;
+        default:
+            if (c16) DNSP_PV(16, 16, 1, true);
+            else DNSP_PV(64, 8, 1, true);
+            break;
     }
 #undef DNSP_PV
 }
