@@ -177,12 +177,13 @@ int dnsp_poisson_get_strategy(const dnsp_poisson* h, int* fft_ct_x, int* fft_ct_
  * returns the forward ones) and the blocked solve's knobs alike.
  * tvar (0..7): partitioned tridiagonal workgroups of (chunks per mode, modes)
  * = (32, 16) with registers capped for 4 resident waves per SIMD (0),
- * (32, 8) (1), (48, 16) (2), (64, 8) (3); rows per chunk follow from ny.
- * 4..7: the same shapes with two-pass chunks (the chunk interior is solved
- * again once its end values are known, instead of being kept in registers
- * across the reduced solve).  The mixed-precision form has its own set; on
- * its FP32 spectra shapes 3 and 7 are (16, 32), whose 32 modes make
- * 128-byte rows.  No counterpart in the reference. */
+ * (32, 8) (1), (48, 16) (2), (16, 16) for ny <= 512, else (64, 8) (3);
+ * rows per chunk follow from ny.  4..7: the same shapes with two-pass chunks
+ * (the chunk interior is solved again once its end values are known,
+ * instead of being kept in registers across the reduced solve).  The
+ * mixed-precision form has its own set; on its FP32 spectra shapes 3 and 7
+ * are (16, 32), whose 32 modes make 128-byte rows.  No counterpart in the
+ * reference. */
 int dnsp_poisson_set_variants(dnsp_poisson* h, int xvar, int zvar, int tvar);
 int dnsp_poisson_get_variants(const dnsp_poisson* h, int* xvar, int* zvar, int* tvar);
 int dnsp_poisson_set_mixed_variants(dnsp_poisson* h, int xvar, int zvar, int tvar);
